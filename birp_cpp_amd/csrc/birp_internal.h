@@ -1,0 +1,109 @@
+// birp_internal.h -- device-side descriptors shared by the HIP kernels (render.hip) and the host
+// library (capi.cpp, server.cpp). Plain structs only: passed by value as kernel arguments or
+// stored in device arrays.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+namespace birp {
+
+// Storage layouts kept on the device for a cube.
+//  linear: the reference's [z][y][x] FP32 values (DataCube::slices flattened).
+//  quad:   FP32 "quad" records, one per cell (x,y,z) with x<nx-1, y<ny-1, z<nz:
+//          {v(x,y,z), v(x+1,y,z), v(x,y+1,z), v(x+1,y+1,z)} (16 B). Two records (z and z+1)
+//          hold a trilinear sample's eight corners. Records are grouped in bricks of 8x8 cells
+//          in x,y and 9 layers in z (the 9th is the next brick's first layer), bricks ordered
+//          [bz][by][bx], records inside a brick [lz][ly][lx].
+//  oct:    16-bit "octet" records, one per cell x<nx-1, y<ny-1, z<nz-1 holding all eight corners
+//          (16 B: the four z-layer corners in quad order, then the four at z+1), bricks of 8^3
+//          cells ordered [bz][by][bx]; F16 values are divided by a power-of-two per-brick scale.
+enum VolumeClass : int {
+    VC_POSITIVE = 0,  // every value finite and > 0: the lerps never straddle zero
+    VC_FINITE = 1,    // finite, any sign
+    VC_NONFINITE = 2  // some inf/nan: only the linear layout with ceil-corner semantics is exact
+};
+
+enum SpaceKind : int { SK_CUBE = 0, SK_CMEM = 1 };
+
+struct CmemParams {
+    float p0, p1, p2, p3;
+    float bs, A1, A2, B, alpha, beta, ay_bs, az_bs;
+    float a5, a21;
+    float beta_c[4];
+    float charlie, dn, ds, theta_n, theta_s, r0_lin;
+};
+
+struct DevVolume {
+    int kind;    // SpaceKind
+    int nx, ny, nz;
+    float ox, oy, oz;     // origin (coords[0])
+    float sx, sy, sz;     // spacing
+    float rsx, rsy, rsz;  // RN(1/spacing), host-computed
+    float fnx, fny, fnz;  // (float) n per axis, as the reference's bound test compares
+    int interp;           // 0 nearest, 1 trilinear
+    int dtype;            // BIRP_DTYPE_*
+    int vclass;           // VolumeClass
+    int nbx, nby, nbz;    // bricks per axis
+    const float* linear;  // may be null for 16-bit volumes
+    const void* quad;     // float4 records
+    const void* oct;      // uint4 records (bf16 or f16)
+    const float* bscale;  // per-brick scale (F16)
+    CmemParams cmem;
+};
+
+// Per-render constants of the exact path, computed on the host in the reference's float order.
+struct ExactView {
+    float rinv[9];
+    float px, py, pz;
+    float tanx, tany, aspect;
+    float L;
+    float w;  // ray_width = t[1] - t[0]
+    int W, H, n;
+    int semantics;    // BIRP_SEM_*
+    int accumulate;   // start from out's values
+    float margin;     // FP64 slab expansion (RE) that covers FP32 position error
+    float* out;       // device image H*W
+    unsigned long long* counter;  // optional: samples added
+};
+
+// Per-pose constants of the fast path: FP64 basis and origin, FP32 march.
+struct FastPose {
+    double rinv[9];
+    double px, py, pz;
+    double tanx_aspect, tany;  // tan(fov_w/2)*aspect, tan(fov_h/2), in double
+    double L;
+    int W, H, n;
+    int semantics;
+    int vol;          // index into the volume table
+    int tiles_x;      // wave tiles across
+    float w;          // step weight (t1 - t0 in double, rounded)
+    float pad;
+    float* out;       // device image
+};
+
+// FP64 render constants (every operation in double).
+struct F64View {
+    double rinv[9];
+    double px, py, pz, tanx, tany, aspect, L, w;
+    int W, H, n, semantics;
+    double* out;
+};
+
+}  // namespace birp
+
+// Launch wrappers implemented in render.hip (host side of the HIP translation unit).
+namespace birp {
+int launch_exact(const DevVolume& v, const ExactView& ev, int lanes_per_ray, void* stream);
+int launch_f64(const DevVolume& v, const F64View& fv, void* stream);
+int launch_sample_points(const DevVolume& v, const float* xyz, long long n, float* out, void* stream);
+int launch_fast_batch(const DevVolume* vols_dev, int dtype, const FastPose* poses_dev, const int* tile_prefix_dev,
+                      int npose, int total_tiles, unsigned* work_counter, int tile_w, int tile_h,
+                      int lanes_per_ray, int unroll, int grid, void* stream);
+int launch_reset_counter(unsigned* counter, void* stream);
+int launch_zero_f32(float* p, size_t n, void* stream);
+int launch_scan_class(const float* values, size_t n, unsigned* flags_dev, void* stream);
+int launch_build_quad(const DevVolume& v, void* quad, void* stream);
+int launch_build_oct(const DevVolume& v, void* oct, float* bscale, void* stream);
+int fast_grid_size(int dtype, int lanes_per_ray, int unroll);
+}  // namespace birp

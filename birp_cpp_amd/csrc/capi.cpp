@@ -1,0 +1,835 @@
+// capi.cpp -- the extern "C" boundary (include/birp.h): devices, volumes, single renders and
+// batched renders. Host code only; kernels are reached through the launch wrappers of render.hip.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/birp.h"
+#include "birp_internal.h"
+#include "capi_internal.h"
+#include "pose.h"
+
+using namespace birp;
+
+namespace birp {
+
+static thread_local std::string g_last_error;
+
+int fail(int status, const std::string& msg) {
+    g_last_error = msg;
+    return status;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    (void)hipGetLastError();
+    return e == hipErrorOutOfMemory ? BIRP_ERR_OUT_OF_MEMORY : BIRP_ERR_DEVICE;
+}
+
+bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+
+int validate_view(const birp_view* v, bool exact) {
+    if (!v) return fail(BIRP_ERR_INVALID_ARGUMENT, "view is null");
+    if (v->width <= 0 || v->height <= 0 || (long long)v->width * v->height > (1ll << 28))
+        return fail(BIRP_ERR_INVALID_ARGUMENT, "image size out of range");
+    if (v->samples < 2) return fail(BIRP_ERR_INVALID_ARGUMENT, "samples must be >= 2 (ray_width reads t[1])");
+    if (exact && v->samples > 8192) return fail(BIRP_ERR_UNSUPPORTED, "exact renders take at most 8192 samples per ray");
+    if (v->samples > (1 << 20)) return fail(BIRP_ERR_INVALID_ARGUMENT, "samples out of range");
+    if (!(v->ray_length > 0) || !std::isfinite(v->ray_length) || v->ray_length > 1e6f)
+        return fail(BIRP_ERR_INVALID_ARGUMENT, "ray_length must be finite and > 0");
+    if (!finite3(v->position) || !finite3(v->aim)) return fail(BIRP_ERR_INVALID_ARGUMENT, "non-finite pose");
+    for (int k = 0; k < 3; k++)
+        if (std::abs(v->position[k]) > 1e6f || std::abs(v->aim[k]) > 1e6f)
+            return fail(BIRP_ERR_INVALID_ARGUMENT, "pose coordinate out of range");
+    if (!std::isfinite(v->fov_w) || !std::isfinite(v->fov_h)) return fail(BIRP_ERR_INVALID_ARGUMENT, "non-finite fov");
+    if (v->semantics != BIRP_SEM_REFERENCE && v->semantics != BIRP_SEM_EXTENDED)
+        return fail(BIRP_ERR_INVALID_ARGUMENT, "unknown semantics");
+    if (v->math != BIRP_MATH_EXACT && v->math != BIRP_MATH_FAST) return fail(BIRP_ERR_INVALID_ARGUMENT, "unknown math");
+    return BIRP_OK;
+}
+
+// t_k and the step weight exactly as GenerateRayDistWidth (Camera.cpp:69-81).
+float ray_width(float L, int n) {
+    volatile float t0 = L * ((0 + 0.5f) / n);
+    volatile float t1 = L * ((1 + 0.5f) / n);
+    return t1 - t0;
+}
+
+int exact_view(const birp_volume* vol, const birp_view* v, float* out_dev, ExactView& ev) {
+    float angle_sun;
+    Mat3f rot, rinv;
+    if (!view_basis(v->position, v->aim, angle_sun, rot, rinv)) return fail(BIRP_ERR_SINGULAR, "Determinant is 0");
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            if (!std::isfinite(rinv.m[r][c])) return fail(BIRP_ERR_INVALID_ARGUMENT, "degenerate pose (camera at the aim point?)");
+            ev.rinv[r * 3 + c] = rinv.m[r][c];
+        }
+    const float to_rad = 3.14159265358979323846 / 180.f;  // Camera.cpp:85
+    ev.px = v->position[0];
+    ev.py = v->position[1];
+    ev.pz = v->position[2];
+    ev.tanx = std::tan(to_rad * (v->fov_w / 2));
+    ev.tany = std::tan(to_rad * (v->fov_h / 2));
+    ev.aspect = (float)v->width / (float)v->height;  // Camera.cpp:137
+    ev.L = v->ray_length;
+    ev.w = ray_width(v->ray_length, v->samples);
+    ev.W = v->width;
+    ev.H = v->height;
+    ev.n = v->samples;
+    ev.semantics = v->semantics;
+    ev.accumulate = (v->flags & BIRP_RENDER_ACCUMULATE) ? 1 : 0;
+    const DevVolume& d = vol->dv;
+    double scale = std::abs((double)v->position[0]) + std::abs((double)v->position[1]) + std::abs((double)v->position[2]) +
+                   v->ray_length + std::abs((double)d.ox) + std::abs((double)d.oy) + std::abs((double)d.oz) +
+                   (double)d.nx * d.sx + (double)d.ny * d.sy + (double)d.nz * d.sz + 1.0;
+    ev.margin = (float)(scale * 0x1p-16);
+    ev.out = out_dev;
+    ev.counter = nullptr;
+    return BIRP_OK;
+}
+
+int fast_pose(const birp_volume* vol, const birp_view* v, int vol_index, float* out_dev, int tile_w, int tile_h, FastPose& fp) {
+    Mat3d rot, rinv;
+    if (!view_basis64(v->position, v->aim, rot, rinv)) return fail(BIRP_ERR_SINGULAR, "Determinant is 0");
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            if (!std::isfinite(rinv.m[r][c])) return fail(BIRP_ERR_INVALID_ARGUMENT, "degenerate pose (camera at the aim point?)");
+            fp.rinv[r * 3 + c] = rinv.m[r][c];
+        }
+    const double to_rad = 3.14159265358979323846 / 180.0;
+    fp.px = v->position[0];
+    fp.py = v->position[1];
+    fp.pz = v->position[2];
+    fp.tanx_aspect = std::tan(to_rad * (v->fov_w / 2.0)) * ((double)v->width / v->height);
+    fp.tany = std::tan(to_rad * (v->fov_h / 2.0));
+    fp.L = v->ray_length;
+    fp.W = v->width;
+    fp.H = v->height;
+    fp.n = v->samples;
+    fp.semantics = v->semantics;
+    fp.vol = vol_index;
+    fp.tiles_x = (v->width + tile_w - 1) / tile_w;
+    fp.w = (float)((double)v->ray_length / v->samples);
+    fp.pad = 0.f;
+    fp.out = out_dev;
+    (void)vol;
+    (void)tile_h;
+    return BIRP_OK;
+}
+
+int tiles_of(const birp_view& v, int tile_w, int tile_h) {
+    return ((v.width + tile_w - 1) / tile_w) * ((v.height + tile_h - 1) / tile_h);
+}
+
+int device_scratch(birp_device* dev, size_t bytes, void** out) {
+    if (dev->scratch_bytes < bytes) {
+        if (dev->scratch) (void)hipFree(dev->scratch);
+        dev->scratch = nullptr;
+        dev->scratch_bytes = 0;
+        hipError_t e = hipMalloc(&dev->scratch, bytes);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(scratch)");
+        dev->scratch_bytes = bytes;
+    }
+    *out = dev->scratch;
+    return BIRP_OK;
+}
+
+bool take_injected_failure(birp_device* dev) {
+    int c = dev->inject_failures.load();
+    while (c > 0) {
+        if (dev->inject_failures.compare_exchange_weak(c, c - 1)) return true;
+    }
+    return false;
+}
+
+int default_exact_lanes() {
+    static int lanes = [] {
+        const char* e = getenv("BIRP_EXACT_LANES");
+        int v = e ? atoi(e) : 4;
+        return (v == 1 || v == 2 || v == 4) ? v : 4;
+    }();
+    return lanes;
+}
+
+}  // namespace birp
+
+// ============================================================================ library / device
+extern "C" const char* birp_strerror(int status) {
+    switch (status) {
+        case BIRP_OK: return "ok";
+        case BIRP_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case BIRP_ERR_SINGULAR: return "Determinant is 0";
+        case BIRP_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case BIRP_ERR_DEVICE: return "HIP runtime error";
+        case BIRP_ERR_NO_DEVICE: return "no GPU device";
+        case BIRP_ERR_ADMISSION: return "request exceeds the serving memory budget";
+        case BIRP_ERR_NOT_FOUND: return "not found";
+        case BIRP_ERR_TIMEOUT: return "timed out";
+        case BIRP_ERR_INJECTED: return "injected failure";
+        case BIRP_ERR_UNSUPPORTED: return "unsupported";
+        case BIRP_ERR_CANCELLED: return "cancelled";
+        default: return "unknown status";
+    }
+}
+
+extern "C" const char* birp_last_error(void) { return g_last_error.c_str(); }
+extern "C" int birp_version(void) { return 10000; }
+
+extern "C" int birp_device_open(int ordinal, birp_device** out) {
+    if (!out) return fail(BIRP_ERR_INVALID_ARGUMENT, "out is null");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return fail(BIRP_ERR_NO_DEVICE, "no HIP device visible");
+    }
+    if (ordinal < 0 || ordinal >= count) return fail(BIRP_ERR_INVALID_ARGUMENT, "device ordinal out of range");
+    if ((e = hipSetDevice(ordinal)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+    birp_device* d = new birp_device();
+    d->ordinal = ordinal;
+    if ((e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking)) != hipSuccess) {
+        delete d;
+        return hip_fail(e, "hipStreamCreate");
+    }
+    (void)hipEventCreate(&d->ev0);
+    (void)hipEventCreate(&d->ev1);
+    (void)hipEventCreate(&d->ev_a);
+    (void)hipEventCreate(&d->ev_b);
+    if ((e = hipMalloc(&d->d_counter, 64)) != hipSuccess) {
+        delete d;
+        return hip_fail(e, "hipMalloc(counter)");
+    }
+    *out = d;
+    return BIRP_OK;
+}
+
+extern "C" int birp_device_close(birp_device* dev) {
+    if (!dev) return BIRP_OK;
+    (void)hipSetDevice(dev->ordinal);
+    (void)hipStreamSynchronize(dev->stream);
+    if (dev->scratch) (void)hipFree(dev->scratch);
+    if (dev->scratch2) (void)hipFree(dev->scratch2);
+    if (dev->d_counter) (void)hipFree(dev->d_counter);
+    (void)hipEventDestroy(dev->ev0);
+    (void)hipEventDestroy(dev->ev1);
+    (void)hipEventDestroy(dev->ev_a);
+    (void)hipEventDestroy(dev->ev_b);
+    (void)hipStreamDestroy(dev->stream);
+    delete dev;
+    return BIRP_OK;
+}
+
+extern "C" int birp_device_synchronize(birp_device* dev) {
+    if (!dev) return fail(BIRP_ERR_INVALID_ARGUMENT, "device is null");
+    hipError_t e = hipStreamSynchronize(dev->stream);
+    return e == hipSuccess ? BIRP_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
+extern "C" int birp_timer_begin(birp_device* dev) {
+    if (!dev) return fail(BIRP_ERR_INVALID_ARGUMENT, "device is null");
+    hipError_t e = hipEventRecord(dev->ev_a, dev->stream);
+    return e == hipSuccess ? BIRP_OK : hip_fail(e, "hipEventRecord");
+}
+
+extern "C" int birp_timer_end(birp_device* dev, double* ms) {
+    if (!dev || !ms) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
+    hipError_t e = hipEventRecord(dev->ev_b, dev->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(dev->ev_b);
+    float f = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&f, dev->ev_a, dev->ev_b);
+    if (e != hipSuccess) return hip_fail(e, "timer");
+    *ms = f;
+    return BIRP_OK;
+}
+
+extern "C" int birp_debug_inject_failure(birp_device* dev, int count) {
+    if (!dev || count < 0) return fail(BIRP_ERR_INVALID_ARGUMENT, "bad argument");
+    dev->inject_failures.store(count);
+    return BIRP_OK;
+}
+
+// ============================================================================ volumes
+static void free_volume_buffers(birp_volume* v) {
+    if (v->linear) (void)hipFree(v->linear);
+    if (v->quad) (void)hipFree(v->quad);
+    if (v->oct) (void)hipFree(v->oct);
+    if (v->bscale) (void)hipFree(v->bscale);
+    v->linear = nullptr;
+    v->quad = nullptr;
+    v->oct = nullptr;
+    v->bscale = nullptr;
+}
+
+extern "C" int birp_volume_upload(birp_device* dev, const birp_volume_desc* desc, const float* values, birp_volume** out) {
+    if (!dev || !desc || !values || !out) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    const int nx = desc->nx, ny = desc->ny, nz = desc->nz;
+    if (nx < 2 || ny < 2 || nz < 2 || nx > 16384 || ny > 16384 || nz > 16384)
+        return fail(BIRP_ERR_INVALID_ARGUMENT, "cube dimensions must be in [2, 16384]");
+    const size_t nvox = (size_t)nx * ny * nz;
+    if (nvox > (size_t)1 << 31) return fail(BIRP_ERR_INVALID_ARGUMENT, "cube larger than 2^31 voxels");
+    for (int k = 0; k < 3; k++) {
+        float s = desc->spacing[k], o = desc->origin[k];
+        if (!std::isfinite(s) || !(s >= 0x1p-30f) || !(s <= 0x1p30f))
+            return fail(BIRP_ERR_INVALID_ARGUMENT, "spacing must be finite and in [2^-30, 2^30]");
+        if (!std::isfinite(o) || std::abs(o) > 1e6f) return fail(BIRP_ERR_INVALID_ARGUMENT, "origin out of range");
+    }
+    if (desc->dtype != BIRP_DTYPE_F32 && desc->dtype != BIRP_DTYPE_BF16 && desc->dtype != BIRP_DTYPE_F16)
+        return fail(BIRP_ERR_INVALID_ARGUMENT, "unknown dtype");
+    if (desc->interp != BIRP_INTERP_NEAREST && desc->interp != BIRP_INTERP_TRILINEAR)
+        return fail(BIRP_ERR_INVALID_ARGUMENT, "unknown interpolation");
+    if (desc->dtype != BIRP_DTYPE_F32 && desc->interp != BIRP_INTERP_TRILINEAR)
+        return fail(BIRP_ERR_UNSUPPORTED, "16-bit storage supports trilinear only");
+    (void)hipSetDevice(dev->ordinal);
+
+    birp_volume* v = new birp_volume();
+    v->dev = dev;
+    v->desc = *desc;
+    DevVolume& d = v->dv;
+    memset(&d, 0, sizeof(d));
+    d.kind = SK_CUBE;
+    d.nx = nx;
+    d.ny = ny;
+    d.nz = nz;
+    d.ox = desc->origin[0];
+    d.oy = desc->origin[1];
+    d.oz = desc->origin[2];
+    d.sx = desc->spacing[0];
+    d.sy = desc->spacing[1];
+    d.sz = desc->spacing[2];
+    {
+        volatile float one = 1.0f;
+        d.rsx = one / d.sx;
+        d.rsy = one / d.sy;
+        d.rsz = one / d.sz;
+    }
+    d.fnx = (float)nx;
+    d.fny = (float)ny;
+    d.fnz = (float)nz;
+    d.interp = desc->interp;
+    d.dtype = desc->dtype;
+    d.nbx = (nx - 1 + 7) / 8;
+    d.nby = (ny - 1 + 7) / 8;
+    d.nbz = (nz - 1 + 7) / 8;
+
+    hipError_t e;
+    int rc = BIRP_OK;
+    unsigned flags = 0;
+    unsigned* dflags = (unsigned*)dev->d_counter;
+    if ((e = hipMalloc(&v->linear, nvox * sizeof(float))) != hipSuccess) { rc = hip_fail(e, "hipMalloc(cube)"); goto fail_out; }
+    if ((e = hipMemcpyAsync(v->linear, values, nvox * sizeof(float), hipMemcpyHostToDevice, dev->stream)) != hipSuccess) {
+        rc = hip_fail(e, "hipMemcpy(cube)");
+        goto fail_out;
+    }
+    d.linear = v->linear;
+    if ((rc = launch_zero_f32((float*)dflags, 1, dev->stream)) != BIRP_OK) goto fail_out;
+    if ((rc = launch_scan_class(v->linear, nvox, dflags, dev->stream)) != BIRP_OK) goto fail_out;
+    if ((e = hipMemcpyAsync(&flags, dflags, sizeof(unsigned), hipMemcpyDeviceToHost, dev->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(dev->stream)) != hipSuccess) {
+        rc = hip_fail(e, "class scan");
+        goto fail_out;
+    }
+    d.vclass = (flags & 2u) ? VC_NONFINITE : (flags & 1u) ? VC_FINITE : VC_POSITIVE;
+    {
+        const size_t nb = (size_t)d.nbx * d.nby * d.nbz;
+        if (desc->dtype == BIRP_DTYPE_F32) {
+            if (desc->interp == BIRP_INTERP_TRILINEAR && d.vclass != VC_NONFINITE) {
+                if ((e = hipMalloc(&v->quad, nb * 576u * 16u)) != hipSuccess) { rc = hip_fail(e, "hipMalloc(quad)"); goto fail_out; }
+                d.quad = v->quad;
+                if ((rc = launch_build_quad(d, v->quad, dev->stream)) != BIRP_OK) goto fail_out;
+            }
+        } else {
+            if (d.vclass == VC_NONFINITE) { rc = fail(BIRP_ERR_UNSUPPORTED, "16-bit storage needs finite values"); goto fail_out; }
+            if ((e = hipMalloc(&v->oct, nb * 512u * 16u)) != hipSuccess) { rc = hip_fail(e, "hipMalloc(oct)"); goto fail_out; }
+            if ((e = hipMalloc(&v->bscale, nb * sizeof(float))) != hipSuccess) { rc = hip_fail(e, "hipMalloc(scale)"); goto fail_out; }
+            d.oct = v->oct;
+            d.bscale = v->bscale;
+            if ((rc = launch_build_oct(d, v->oct, v->bscale, dev->stream)) != BIRP_OK) goto fail_out;
+        }
+        if ((e = hipStreamSynchronize(dev->stream)) != hipSuccess) { rc = hip_fail(e, "layout build"); goto fail_out; }
+        if (desc->dtype != BIRP_DTYPE_F32) {
+            // 16-bit volumes keep only their 16-bit records on the device
+            (void)hipFree(v->linear);
+            v->linear = nullptr;
+            d.linear = nullptr;
+        }
+        v->bytes = (v->linear ? nvox * 4 : 0) + (v->quad ? nb * 576u * 16u : 0) + (v->oct ? nb * 512u * 16u : 0) +
+                   (v->bscale ? nb * 4 : 0);
+    }
+    *out = v;
+    return BIRP_OK;
+fail_out:
+    (void)hipStreamSynchronize(dev->stream);
+    free_volume_buffers(v);
+    delete v;
+    return rc;
+}
+
+extern "C" int birp_cmem_create(birp_device* dev, const birp_cmem_state* s, birp_volume** out) {
+    if (!dev || !s || !out) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
+    birp_volume* v = new birp_volume();
+    v->dev = dev;
+    memset(&v->desc, 0, sizeof(v->desc));
+    v->desc.dtype = BIRP_DTYPE_F32;
+    memset(&v->dv, 0, sizeof(v->dv));
+    v->dv.kind = SK_CMEM;
+    CmemParams& c = v->dv.cmem;
+    c.p0 = s->p0; c.p1 = s->p1; c.p2 = s->p2; c.p3 = s->p3;
+    c.bs = s->bs; c.A1 = s->A1; c.A2 = s->A2; c.B = s->B; c.alpha = s->alpha; c.beta = s->beta;
+    c.ay_bs = s->ay_bs; c.az_bs = s->az_bs; c.a5 = s->a5; c.a21 = s->a21;
+    for (int k = 0; k < 4; k++) c.beta_c[k] = s->beta_c[k];
+    c.charlie = s->charlie; c.dn = s->dn; c.ds = s->ds; c.theta_n = s->theta_n; c.theta_s = s->theta_s; c.r0_lin = s->r0_lin;
+    v->dv.dtype = BIRP_DTYPE_F32;
+    v->dv.interp = BIRP_INTERP_TRILINEAR;
+    v->bytes = 0;
+    *out = v;
+    return BIRP_OK;
+}
+
+extern "C" int birp_volume_free(birp_volume* v) {
+    if (!v) return BIRP_OK;
+    (void)hipSetDevice(v->dev->ordinal);
+    (void)hipStreamSynchronize(v->dev->stream);
+    free_volume_buffers(v);
+    delete v;
+    return BIRP_OK;
+}
+
+extern "C" int birp_volume_bytes(const birp_volume* v, size_t* bytes) {
+    if (!v || !bytes) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
+    *bytes = v->bytes;
+    return BIRP_OK;
+}
+
+extern "C" int birp_volume_sample(birp_volume* v, const float* xyz, long long n, float* out) {
+    if (!v || (n > 0 && (!xyz || !out)) || n < 0) return fail(BIRP_ERR_INVALID_ARGUMENT, "bad argument");
+    if (n == 0) return BIRP_OK;
+    if (v->dv.kind == SK_CUBE && !v->dv.linear) return fail(BIRP_ERR_UNSUPPORTED, "point sampling needs an FP32 volume");
+    birp_device* dev = v->dev;
+    (void)hipSetDevice(dev->ordinal);
+    void* buf = nullptr;
+    size_t bytes = (size_t)n * 16;
+    int rc = device_scratch(dev, bytes, &buf);
+    if (rc) return rc;
+    float* dxyz = (float*)buf;
+    float* dout = dxyz + 3 * n;
+    hipError_t e = hipMemcpyAsync(dxyz, xyz, (size_t)n * 12, hipMemcpyHostToDevice, dev->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(points)");
+    if ((rc = launch_sample_points(v->dv, dxyz, n, dout, dev->stream)) != BIRP_OK) return fail(rc, "sample launch failed");
+    e = hipMemcpyAsync(out, dout, (size_t)n * 4, hipMemcpyDeviceToHost, dev->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
+    return e == hipSuccess ? BIRP_OK : hip_fail(e, "sample");
+}
+
+// ============================================================================ single renders
+extern "C" int birp_view_basis(const birp_view* view, float* angle_sun, float rot[9], float rinv[9]) {
+    if (!view || !finite3(view->position) || !finite3(view->aim)) return fail(BIRP_ERR_INVALID_ARGUMENT, "bad view");
+    float a;
+    Mat3f r, ri;
+    bool ok = view_basis(view->position, view->aim, a, r, ri);
+    if (angle_sun) *angle_sun = a;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            if (rot) rot[i * 3 + j] = r.m[i][j];
+            if (rinv && ok) rinv[i * 3 + j] = ri.m[i][j];
+        }
+    return ok ? BIRP_OK : fail(BIRP_ERR_SINGULAR, "Determinant is 0");
+}
+
+static int render_exact_into(birp_volume* vol, const birp_view* view, float* dimg, unsigned long long* counter) {
+    birp_device* dev = vol->dev;
+    if (vol->dv.kind == SK_CUBE && !vol->dv.linear) return fail(BIRP_ERR_UNSUPPORTED, "exact renders need an FP32 volume");
+    ExactView ev;
+    int rc = exact_view(vol, view, dimg, ev);
+    if (rc) return rc;
+    ev.counter = counter;
+    if (take_injected_failure(dev)) return fail(BIRP_ERR_INJECTED, "injected failure");
+    rc = launch_exact(vol->dv, ev, default_exact_lanes(), dev->stream);
+    return rc ? fail(rc, "exact render launch failed") : BIRP_OK;
+}
+
+static int render_fast_into(birp_volume* vol, const birp_view* view, float* dimg);
+
+extern "C" int birp_render(birp_volume* vol, const birp_view* view, float* image, birp_render_stats* stats) {
+    if (!vol || !image) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = validate_view(view, view && view->math == BIRP_MATH_EXACT);
+    if (rc) return rc;
+    birp_device* dev = vol->dev;
+    (void)hipSetDevice(dev->ordinal);
+    const size_t npix = (size_t)view->width * view->height;
+    void* buf = nullptr;
+    if ((rc = device_scratch(dev, npix * sizeof(float), &buf))) return rc;
+    float* dimg = (float*)buf;
+    hipError_t e;
+    if (view->flags & BIRP_RENDER_ACCUMULATE) {
+        if ((e = hipMemcpyAsync(dimg, image, npix * 4, hipMemcpyHostToDevice, dev->stream)) != hipSuccess)
+            return hip_fail(e, "hipMemcpy(image)");
+    }
+    unsigned long long* cnt = (unsigned long long*)dev->d_counter + 1;
+    if ((rc = launch_zero_f32((float*)cnt, 2, dev->stream))) return fail(rc, "counter reset");
+    (void)hipEventRecord(dev->ev0, dev->stream);
+    if (view->math == BIRP_MATH_EXACT) rc = render_exact_into(vol, view, dimg, cnt);
+    else rc = render_fast_into(vol, view, dimg);
+    if (rc) {
+        (void)hipStreamSynchronize(dev->stream);
+        return rc;
+    }
+    (void)hipEventRecord(dev->ev1, dev->stream);
+    unsigned long long count = 0;
+    if ((e = hipMemcpyAsync(image, dimg, npix * 4, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess)
+        return hip_fail(e, "hipMemcpy(image)");
+    if ((e = hipMemcpyAsync(&count, cnt, 8, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess) return hip_fail(e, "hipMemcpy(count)");
+    if ((e = hipStreamSynchronize(dev->stream)) != hipSuccess) return hip_fail(e, "render");
+    if (stats) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, dev->ev0, dev->ev1);
+        stats->kernel_ms = ms;
+        stats->samples_nominal = (long long)npix * view->samples;
+        stats->samples_in_cube = (view->math == BIRP_MATH_EXACT) ? (long long)count : -1;
+    }
+    return BIRP_OK;
+}
+
+extern "C" int birp_image_create(birp_device* dev, int width, int height, birp_image** out) {
+    if (!dev || !out || width <= 0 || height <= 0) return fail(BIRP_ERR_INVALID_ARGUMENT, "bad argument");
+    (void)hipSetDevice(dev->ordinal);
+    birp_image* img = new birp_image();
+    img->dev = dev;
+    img->W = width;
+    img->H = height;
+    hipError_t e = hipMalloc(&img->d, (size_t)width * height * sizeof(float));
+    if (e != hipSuccess) {
+        delete img;
+        return hip_fail(e, "hipMalloc(image)");
+    }
+    int rc = launch_zero_f32(img->d, (size_t)width * height, dev->stream);
+    if (rc) {
+        (void)hipFree(img->d);
+        delete img;
+        return rc;
+    }
+    *out = img;
+    return BIRP_OK;
+}
+
+extern "C" int birp_image_free(birp_image* img) {
+    if (!img) return BIRP_OK;
+    (void)hipStreamSynchronize(img->dev->stream);
+    (void)hipFree(img->d);
+    delete img;
+    return BIRP_OK;
+}
+
+extern "C" int birp_image_download(birp_image* img, float* host) {
+    if (!img || !host) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
+    hipError_t e = hipMemcpyAsync(host, img->d, (size_t)img->W * img->H * 4, hipMemcpyDeviceToHost, img->dev->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(img->dev->stream);
+    return e == hipSuccess ? BIRP_OK : hip_fail(e, "image download");
+}
+
+extern "C" int birp_image_upload(birp_image* img, const float* host) {
+    if (!img || !host) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
+    hipError_t e = hipMemcpyAsync(img->d, host, (size_t)img->W * img->H * 4, hipMemcpyHostToDevice, img->dev->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(img->dev->stream);
+    return e == hipSuccess ? BIRP_OK : hip_fail(e, "image upload");
+}
+
+extern "C" int birp_render_async(birp_volume* vol, const birp_view* view, birp_image* img) {
+    if (!vol || !img) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = validate_view(view, view && view->math == BIRP_MATH_EXACT);
+    if (rc) return rc;
+    if (img->W != view->width || img->H != view->height) return fail(BIRP_ERR_INVALID_ARGUMENT, "image size mismatch");
+    if (img->dev != vol->dev) return fail(BIRP_ERR_INVALID_ARGUMENT, "image and volume on different devices");
+    if (view->math == BIRP_MATH_EXACT) return render_exact_into(vol, view, img->d, nullptr);
+    return render_fast_into(vol, view, img->d);
+}
+
+extern "C" int birp_count_samples(birp_volume* vol, const birp_view* view, long long* in_cube) {
+    if (!vol || !in_cube) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
+    birp_view v = *view;
+    v.math = BIRP_MATH_EXACT;
+    v.flags = 0;
+    int rc = validate_view(&v, true);
+    if (rc) return rc;
+    birp_device* dev = vol->dev;
+    (void)hipSetDevice(dev->ordinal);
+    void* buf = nullptr;
+    if ((rc = device_scratch(dev, (size_t)v.width * v.height * 4, &buf))) return rc;
+    unsigned long long* cnt = (unsigned long long*)dev->d_counter + 1;
+    if ((rc = launch_zero_f32((float*)cnt, 2, dev->stream))) return rc;
+    if ((rc = render_exact_into(vol, &v, (float*)buf, cnt))) return rc;
+    unsigned long long c = 0;
+    hipError_t e = hipMemcpyAsync(&c, cnt, 8, hipMemcpyDeviceToHost, dev->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
+    if (e != hipSuccess) return hip_fail(e, "count");
+    *in_cube = (long long)c;
+    return BIRP_OK;
+}
+
+extern "C" int birp_render_f64(birp_volume* vol, const birp_view* view, double* image) {
+    if (!vol || !image) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = validate_view(view, false);
+    if (rc) return rc;
+    if (vol->dv.kind == SK_CUBE && !vol->dv.linear) return fail(BIRP_ERR_UNSUPPORTED, "FP64 renders need an FP32 volume");
+    birp_device* dev = vol->dev;
+    (void)hipSetDevice(dev->ordinal);
+    Mat3d rot, rinv;
+    if (!view_basis64(view->position, view->aim, rot, rinv)) return fail(BIRP_ERR_SINGULAR, "Determinant is 0");
+    F64View fv;
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) fv.rinv[r * 3 + c] = rinv.m[r][c];
+    const double to_rad = 3.14159265358979323846 / 180.0;
+    fv.px = view->position[0];
+    fv.py = view->position[1];
+    fv.pz = view->position[2];
+    fv.tanx = std::tan(to_rad * (view->fov_w / 2.0));
+    fv.tany = std::tan(to_rad * (view->fov_h / 2.0));
+    fv.aspect = (double)view->width / view->height;
+    fv.L = view->ray_length;
+    {
+        double t0 = fv.L * (0.5 / view->samples), t1 = fv.L * (1.5 / view->samples);
+        fv.w = t1 - t0;
+    }
+    fv.W = view->width;
+    fv.H = view->height;
+    fv.n = view->samples;
+    fv.semantics = view->semantics;
+    const size_t npix = (size_t)view->width * view->height;
+    void* buf = nullptr;
+    if ((rc = device_scratch(dev, npix * 8, &buf))) return rc;
+    fv.out = (double*)buf;
+    if ((rc = launch_f64(vol->dv, fv, dev->stream))) return fail(rc, "f64 launch failed");
+    hipError_t e = hipMemcpyAsync(image, buf, npix * 8, hipMemcpyDeviceToHost, dev->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
+    return e == hipSuccess ? BIRP_OK : hip_fail(e, "f64 render");
+}
+
+// ============================================================================ batches
+namespace birp {
+
+int batch_setup(birp_batch* b);
+
+int choose_tuning(const birp_volume* vol, const std::vector<birp_view>& views, birp_tuning& t) {
+    // the tuning of the batch's dominant (largest) image class
+    size_t best = 0;
+    int bi = 0;
+    for (size_t k = 0; k < views.size(); k++) {
+        size_t w = (size_t)views[k].width * views[k].height * views[k].samples;
+        if (w > best) { best = w; bi = (int)k; }
+    }
+    if (!views.empty() && birp_autotune_lookup(vol->desc.dtype, views[bi].width, views[bi].height, views[bi].samples, &t) == BIRP_OK)
+        return BIRP_OK;
+    t.tile_w = 4;
+    t.tile_h = 4;
+    t.lanes_per_ray = 4;
+    t.unroll = 2;
+    return BIRP_OK;
+}
+
+void batch_release(birp_batch* b) {
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+    if (b->d_poses) (void)hipFree(b->d_poses);
+    if (b->d_prefix) (void)hipFree(b->d_prefix);
+    if (b->d_vols) (void)hipFree(b->d_vols);
+    b->d_poses = nullptr;
+    b->d_prefix = nullptr;
+    b->d_vols = nullptr;
+}
+
+int batch_setup(birp_batch* b) {
+    birp_device* dev = b->vol->dev;
+    batch_release(b);
+    const birp_tuning& t = b->tuning;
+    const int n = (int)b->views.size();
+    std::vector<FastPose> poses(n);
+    std::vector<int> prefix(n + 1, 0);
+    for (int k = 0; k < n; k++) {
+        int rc = fast_pose(b->vol, &b->views[k], 0, b->d_images + b->offsets[k], t.tile_w, t.tile_h, poses[k]);
+        if (rc) return rc;
+        prefix[k + 1] = prefix[k] + tiles_of(b->views[k], t.tile_w, t.tile_h);
+    }
+    b->total_tiles = prefix[n];
+    hipError_t e;
+    if ((e = hipMalloc(&b->d_poses, sizeof(FastPose) * n)) != hipSuccess) return hip_fail(e, "hipMalloc(poses)");
+    if ((e = hipMalloc(&b->d_prefix, sizeof(int) * (n + 1))) != hipSuccess) return hip_fail(e, "hipMalloc(prefix)");
+    if ((e = hipMalloc(&b->d_vols, sizeof(DevVolume))) != hipSuccess) return hip_fail(e, "hipMalloc(vols)");
+    e = hipMemcpyAsync(b->d_poses, poses.data(), sizeof(FastPose) * n, hipMemcpyHostToDevice, dev->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(b->d_prefix, prefix.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, dev->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(b->d_vols, &b->vol->dv, sizeof(DevVolume), hipMemcpyHostToDevice, dev->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
+    if (e != hipSuccess) return hip_fail(e, "batch upload");
+    b->grid = fast_grid_size(b->vol->desc.dtype, t.lanes_per_ray, t.unroll);
+    // capture: reset the work counter (a kernel, not a memset node), then the persistent kernel
+    hipStream_t cs = b->capture_stream;
+    if ((e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal)) != hipSuccess) return hip_fail(e, "begin capture");
+    int rc = launch_reset_counter(b->d_work, cs);
+    if (rc == BIRP_OK)
+        rc = launch_fast_batch(b->d_vols, b->vol->desc.dtype, b->d_poses, b->d_prefix, n, b->total_tiles, b->d_work, t.tile_w,
+                               t.tile_h, t.lanes_per_ray, t.unroll, b->grid, cs);
+    e = hipStreamEndCapture(cs, &b->graph);
+    if (rc != BIRP_OK) return fail(rc, "batch capture launch failed");
+    if (e != hipSuccess) return hip_fail(e, "end capture");
+    if ((e = hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0)) != hipSuccess) return hip_fail(e, "graph instantiate");
+    return BIRP_OK;
+}
+
+}  // namespace birp
+
+extern "C" int birp_batch_create(birp_volume* vol, const birp_view* views, int count, birp_batch** out) {
+    if (!vol || !views || count <= 0 || !out) return fail(BIRP_ERR_INVALID_ARGUMENT, "bad argument");
+    if (count > (1 << 20)) return fail(BIRP_ERR_INVALID_ARGUMENT, "too many views");
+    *out = nullptr;
+    for (int k = 0; k < count; k++) {
+        int rc = validate_view(&views[k], false);
+        if (rc) return rc;
+    }
+    birp_device* dev = vol->dev;
+    (void)hipSetDevice(dev->ordinal);
+    birp_batch* b = new birp_batch();
+    b->vol = vol;
+    b->views.assign(views, views + count);
+    size_t off = 0;
+    for (int k = 0; k < count; k++) {
+        b->views[k].math = BIRP_MATH_FAST;
+        b->offsets.push_back(off);
+        off += (size_t)views[k].width * views[k].height;
+        b->nominal += (long long)views[k].width * views[k].height * views[k].samples;
+    }
+    b->total_pixels = off;
+    hipError_t e;
+    int rc;
+    if ((e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking)) != hipSuccess) { rc = hip_fail(e, "stream"); goto bad; }
+    if ((e = hipMalloc(&b->d_images, off * sizeof(float))) != hipSuccess) { rc = hip_fail(e, "hipMalloc(images)"); goto bad; }
+    if ((e = hipMalloc(&b->d_work, 64)) != hipSuccess) { rc = hip_fail(e, "hipMalloc(counter)"); goto bad; }
+    (void)hipEventCreate(&b->ev0);
+    (void)hipEventCreate(&b->ev1);
+    choose_tuning(vol, b->views, b->tuning);
+    if ((rc = batch_setup(b)) != BIRP_OK) goto bad;
+    *out = b;
+    return BIRP_OK;
+bad:
+    birp_batch_free(b);
+    return rc;
+}
+
+extern "C" int birp_batch_free(birp_batch* b) {
+    if (!b) return BIRP_OK;
+    if (b->capture_stream) (void)hipStreamSynchronize(b->capture_stream);
+    (void)hipStreamSynchronize(b->vol->dev->stream);
+    batch_release(b);
+    if (b->d_images) (void)hipFree(b->d_images);
+    if (b->d_work) (void)hipFree(b->d_work);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+    delete b;
+    return BIRP_OK;
+}
+
+extern "C" int birp_batch_set_tuning(birp_batch* b, const birp_tuning* t) {
+    if (!b) return fail(BIRP_ERR_INVALID_ARGUMENT, "null batch");
+    birp_tuning nt;
+    if (t) {
+        nt = *t;
+        if (!(nt.lanes_per_ray == 1 || nt.lanes_per_ray == 2 || nt.lanes_per_ray == 4) ||
+            !(nt.unroll == 1 || nt.unroll == 2 || nt.unroll == 4) || nt.tile_w <= 0 || nt.tile_h <= 0 ||
+            nt.tile_w * nt.tile_h * nt.lanes_per_ray != 64 || (nt.tile_w & (nt.tile_w - 1)))
+            return fail(BIRP_ERR_INVALID_ARGUMENT, "tuning: tile_w*tile_h*lanes_per_ray must be 64, tile_w a power of two");
+    } else {
+        choose_tuning(b->vol, b->views, nt);
+    }
+    (void)hipStreamSynchronize(b->vol->dev->stream);
+    b->tuning = nt;
+    return batch_setup(b);
+}
+
+extern "C" int birp_batch_launch(birp_batch* b) {
+    if (!b || !b->graph_exec) return fail(BIRP_ERR_INVALID_ARGUMENT, "batch not ready");
+    birp_device* dev = b->vol->dev;
+    if (take_injected_failure(dev)) return fail(BIRP_ERR_INJECTED, "injected failure");
+    (void)hipEventRecord(b->ev0, dev->stream);
+    hipError_t e = hipGraphLaunch(b->graph_exec, dev->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    (void)hipEventRecord(b->ev1, dev->stream);
+    b->launched = true;
+    return BIRP_OK;
+}
+
+extern "C" int birp_batch_wait(birp_batch* b) {
+    if (!b) return fail(BIRP_ERR_INVALID_ARGUMENT, "null batch");
+    hipError_t e = hipStreamSynchronize(b->vol->dev->stream);
+    if (e != hipSuccess) return hip_fail(e, "batch wait");
+    if (b->launched) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, b->ev0, b->ev1) == hipSuccess) b->last_ms = ms;
+        b->launched = false;
+    }
+    return BIRP_OK;
+}
+
+extern "C" int birp_batch_download(birp_batch* b, int index, float* host) {
+    if (!b || !host || index < 0 || index >= (int)b->views.size()) return fail(BIRP_ERR_INVALID_ARGUMENT, "bad argument");
+    const birp_view& v = b->views[index];
+    hipError_t e = hipMemcpyAsync(host, b->d_images + b->offsets[index], (size_t)v.width * v.height * 4, hipMemcpyDeviceToHost,
+                                  b->vol->dev->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->vol->dev->stream);
+    return e == hipSuccess ? BIRP_OK : hip_fail(e, "batch download");
+}
+
+extern "C" int birp_batch_stats(birp_batch* b, long long* nominal, double* last_ms) {
+    if (!b) return fail(BIRP_ERR_INVALID_ARGUMENT, "null batch");
+    if (nominal) *nominal = b->nominal;
+    if (last_ms) *last_ms = b->last_ms;
+    return BIRP_OK;
+}
+
+// A single fast render: a one-pose launch of the persistent kernel (no graph).
+static int render_fast_into(birp_volume* vol, const birp_view* view, float* dimg) {
+    birp_device* dev = vol->dev;
+    if (vol->dv.kind != SK_CUBE) return fail(BIRP_ERR_UNSUPPORTED, "fast renders take cube volumes");
+    std::vector<birp_view> vv(1, *view);
+    birp_tuning t;
+    choose_tuning(vol, vv, t);
+    FastPose fp;
+    int rc = fast_pose(vol, view, 0, dimg, t.tile_w, t.tile_h, fp);
+    if (rc) return rc;
+    if (view->flags & BIRP_RENDER_ACCUMULATE) return fail(BIRP_ERR_UNSUPPORTED, "fast renders do not accumulate");
+    int prefix[2] = {0, tiles_of(*view, t.tile_w, t.tile_h)};
+    // staging in the device's second scratch buffer: [DevVolume][FastPose][prefix][counter]
+    size_t need = sizeof(DevVolume) + sizeof(FastPose) + 64 + 64;
+    if (dev->scratch2_bytes < need) {
+        if (dev->scratch2) (void)hipFree(dev->scratch2);
+        dev->scratch2 = nullptr;
+        hipError_t e = hipMalloc(&dev->scratch2, 4096);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(scratch2)");
+        dev->scratch2_bytes = 4096;
+    }
+    char* base = (char*)dev->scratch2;
+    DevVolume* dv = (DevVolume*)base;
+    FastPose* dp = (FastPose*)(base + 1024);
+    int* dpre = (int*)(base + 2048);
+    unsigned* dwork = (unsigned*)(base + 3072);
+    hipError_t e = hipMemcpyAsync(dv, &vol->dv, sizeof(DevVolume), hipMemcpyHostToDevice, dev->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dp, &fp, sizeof(FastPose), hipMemcpyHostToDevice, dev->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dpre, prefix, sizeof(prefix), hipMemcpyHostToDevice, dev->stream);
+    // the sources are this frame's locals: wait for the copies before returning
+    if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
+    if (e != hipSuccess) return hip_fail(e, "fast render upload");
+    if (take_injected_failure(dev)) return fail(BIRP_ERR_INJECTED, "injected failure");
+    if ((rc = launch_reset_counter(dwork, dev->stream))) return rc;
+    int grid = std::min(fast_grid_size(vol->desc.dtype, t.lanes_per_ray, t.unroll), (prefix[1] + 3) / 4 + 1);
+    return launch_fast_batch(dv, vol->desc.dtype, dp, dpre, 1, prefix[1], dwork, t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll,
+                             grid, dev->stream);
+}

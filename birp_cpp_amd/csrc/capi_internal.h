@@ -1,0 +1,74 @@
+// capi_internal.h -- definitions of the opaque C-ABI handles (include/birp.h) and helpers shared
+// by capi.cpp, server.cpp and autotune.cpp.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <atomic>
+#include <string>
+#include <vector>
+
+#include "../../include/birp.h"
+#include "birp_internal.h"
+
+struct birp_device {
+    int ordinal = 0;
+    hipStream_t stream = nullptr;      // render stream (default priority)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;   // per-render timing
+    hipEvent_t ev_a = nullptr, ev_b = nullptr; // birp_timer_begin/end
+    void* d_counter = nullptr;         // 64 B: class-scan flags, sample counter
+    void* scratch = nullptr;           // image / point staging, grown on demand
+    size_t scratch_bytes = 0;
+    void* scratch2 = nullptr;          // single fast-render descriptors
+    size_t scratch2_bytes = 0;
+    std::atomic<int> inject_failures{0};
+};
+
+struct birp_volume {
+    birp_device* dev = nullptr;
+    birp_volume_desc desc{};
+    birp::DevVolume dv{};
+    float* linear = nullptr;
+    void* quad = nullptr;
+    void* oct = nullptr;
+    float* bscale = nullptr;
+    size_t bytes = 0;
+};
+
+struct birp_image {
+    birp_device* dev = nullptr;
+    int W = 0, H = 0;
+    float* d = nullptr;
+};
+
+struct birp_batch {
+    birp_volume* vol = nullptr;
+    std::vector<birp_view> views;
+    std::vector<size_t> offsets;
+    size_t total_pixels = 0;
+    long long nominal = 0;
+    float* d_images = nullptr;
+    birp::FastPose* d_poses = nullptr;
+    int* d_prefix = nullptr;
+    birp::DevVolume* d_vols = nullptr;
+    unsigned* d_work = nullptr;
+    int total_tiles = 0;
+    int grid = 0;
+    birp_tuning tuning{4, 4, 4, 2};
+    hipStream_t capture_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool launched = false;
+    double last_ms = 0.0;
+};
+
+namespace birp {
+int fail(int status, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+int validate_view(const birp_view* v, bool exact);
+int fast_pose(const birp_volume* vol, const birp_view* v, int vol_index, float* out_dev, int tile_w, int tile_h, FastPose& fp);
+int tiles_of(const birp_view& v, int tile_w, int tile_h);
+bool take_injected_failure(birp_device* dev);
+int choose_tuning(const birp_volume* vol, const std::vector<birp_view>& views, birp_tuning& t);
+}  // namespace birp

@@ -1,0 +1,931 @@
+// render.hip -- HIP kernels for gfx950 (MI355X): BIRP's per-pixel ray generation, fixed-step
+// march through the emissivity cube (or the analytic CMEM model) and per-ray integration
+// (reference Camera.cpp:140-229, DataCube.cpp:203-277, Cmem.cpp:117-288), plus the layout
+// builders the renders read from.
+//
+// Two arithmetic families:
+//  * exact  -- FP32 in the reference's operation order (no FMA contraction: the file is built
+//              with -ffp-contract=off), correctly rounded divisions via birp_div_rn, the double
+//              scale chain via birp_scale_sample, std::lerp's branch structure, and the image sum
+//              in k order. Bit-identical to the reference CPU render for finite cubes.
+//  * fast   -- FP64 camera basis, ray origin and cube entry; FP32 FMA march in index space;
+//              BF16/F16/FP32 storage; FP32 Kahan sums. Batched in a persistent work-queue kernel.
+//
+// Wave mapping: a wave64 owns a tile of 64/S rays; S lanes ("sub-lanes") share one ray and take
+// its samples k = kb + sub, kb advancing by S each trip. In the exact kernels every sub-lane then
+// reads its quad-mates' samples through DPP quad_perm moves and all of them add the S samples in
+// k order, so the sequential FP32 sum of the reference is kept while S lanes sample in parallel.
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include "../../include/birp.h"
+#include "birp_internal.h"
+#include "exact_math.h"
+
+namespace birp {
+
+#define BIRP_WAVE 64
+#define BIRP_BLOCK 256
+#define BIRP_MAX_EXACT_SAMPLES 8192
+
+// ------------------------------------------------------------------------------------------
+// small device helpers
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// DPP quad_perm broadcast: every lane of a quad receives lane M of its quad (S == 4), or
+// lane M of its pair (S == 2).
+template <int S, int M>
+__device__ __forceinline__ int quad_bcast(int v) {
+    if constexpr (S == 4) {
+        constexpr int ctrl = M | (M << 2) | (M << 4) | (M << 6);
+        return __builtin_amdgcn_mov_dpp(v, ctrl, 0xF, 0xF, false);
+    } else {
+        constexpr int ctrl = M | (M << 2) | ((2 + M) << 4) | ((2 + M) << 6);
+        return __builtin_amdgcn_mov_dpp(v, ctrl, 0xF, 0xF, false);
+    }
+}
+template <int S, int M>
+__device__ __forceinline__ float quad_bcast_f(float v) {
+    return __int_as_float(quad_bcast<S, M>(__float_as_int(v)));
+}
+
+// std::lerp (libstdc++ 11, cmath:1894-1908) for any finite or non-finite input.
+__device__ __forceinline__ float lerp_general(float a, float b, float t) {
+    bool straddle = (a <= 0 && b >= 0) || (a >= 0 && b <= 0);
+    float s1 = t * b + (1 - t) * a;
+    float x = a + t * (b - a);
+    float r = ((t > 1) == (b > a)) ? (b < x ? x : b) : (b > x ? x : b);
+    r = (t == 1) ? b : r;
+    return straddle ? s1 : r;
+}
+
+// std::lerp when a, b > 0 and finite and 0 <= t < 1: a + t(b-a) clamped towards b, which is
+// the median of (x, a, b) since x never passes a.
+__device__ __forceinline__ float lerp_positive(float a, float b, float t) {
+    float x = a + t * (b - a);
+    return __builtin_amdgcn_fmed3f(x, a, b);
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+// Clamp in the float domain, then convert: defined for NaN and out-of-range inputs (NaN -> lo),
+// so every gather index is in [lo, hi] whatever a masked-off lane computed.
+__device__ __forceinline__ int to_index(float x, int hi) { return (int)fminf(fmaxf(x, 0.0f), (float)hi); }
+
+// ------------------------------------------------------------------------------------------
+// exact per-pixel ray (Camera.cpp:148-168), FP32 IEEE operations in the reference's order
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void exact_ray_dir(const ExactView& ev, int i, int j, float& dx, float& dy, float& dz) {
+    float device_x = ((float)i + 0.5f) / (float)ev.W;
+    float device_y = ((float)j + 0.5f) / (float)ev.H;
+    float screen_x = ((2 * device_x) - 1) * ev.tanx * ev.aspect;
+    float screen_y = (1 - (2 * device_y)) * ev.tany;
+    float nd = sqrtf(screen_x * screen_x + screen_y * screen_y + (-1.0f) * (-1.0f));
+    float ux = screen_x / nd, uy = screen_y / nd, uz = -1.0f / nd;
+    float w0 = (ev.rinv[0] * ux) + (ev.rinv[1] * uy) + (ev.rinv[2] * uz);
+    float w1 = (ev.rinv[3] * ux) + (ev.rinv[4] * uy) + (ev.rinv[5] * uz);
+    float w2 = (ev.rinv[6] * ux) + (ev.rinv[7] * uy) + (ev.rinv[8] * uz);
+    float wd = sqrtf(w0 * w0 + w1 * w1 + w2 * w2);
+    dx = w0 / wd;
+    dy = w1 / wd;
+    dz = w2 / wd;
+}
+
+// Index of the sample point on each axis: RN(RN(RN(d t) + p) - o) / s), exactly.
+struct ExactIdx {
+    float xi, yi, zi;  // ingress
+    float xd, yd, zd;  // index
+};
+
+__device__ __forceinline__ ExactIdx exact_index(const DevVolume& v, float dx, float dy, float dz, float px,
+                                                float py, float pz, float t) {
+    ExactIdx r;
+    float x = dx * t + px;
+    float y = dy * t + py;
+    float z = dz * t + pz;
+    r.xi = x - v.ox;
+    r.yi = y - v.oy;
+    r.zi = z - v.oz;
+    float m = fminf(fminf(fabsf(r.xi), fabsf(r.yi)), fabsf(r.zi));
+    if (__builtin_expect(m < 0x1p-64f && m != 0.0f, 0)) {
+        // Below the range where the residual-corrected division is proven exact.
+        r.xd = r.xi / v.sx;
+        r.yd = r.yi / v.sy;
+        r.zd = r.zi / v.sz;
+    } else {
+        r.xd = birp_div_rn(r.xi, v.sx, v.rsx);
+        r.yd = birp_div_rn(r.yi, v.sy, v.rsy);
+        r.zd = birp_div_rn(r.zi, v.sz, v.rsz);
+    }
+    return r;
+}
+
+// DataCube.cpp:214-218 negated: inside iff ingress >= 0 and index + 1 < n on every axis.
+// (NaN compares false here and counts as outside; the reference would index out of range.)
+__device__ __forceinline__ bool exact_inside(const DevVolume& v, const ExactIdx& r) {
+    return (r.xi >= 0) && (r.xd + 1 < v.fnx) && (r.yi >= 0) && (r.yd + 1 < v.fny) && (r.zi >= 0) &&
+           (r.zd + 1 < v.fnz);
+}
+
+// Trilinear sample from quad records (finite cubes). Corner x1 = x0 + 1 always: for finite data
+// std::lerp at t == 0 returns a, so this equals DataCube's ceil() corner choice.
+template <bool POS>
+__device__ __forceinline__ float sample_quad(const DevVolume& v, const ExactIdx& r) {
+    float x0f = floorf(r.xd), y0f = floorf(r.yd), z0f = floorf(r.zd);
+    float fx = r.xd - x0f, fy = r.yd - y0f, fz = r.zd - z0f;
+    int ix = to_index(x0f, v.nx - 2);
+    int iy = to_index(y0f, v.ny - 2);
+    int iz = to_index(z0f, v.nz - 2);
+    unsigned brick = ((unsigned)(iz >> 3) * (unsigned)v.nby + (unsigned)(iy >> 3)) * (unsigned)v.nbx + (unsigned)(ix >> 3);
+    unsigned local = (((unsigned)(iz & 7) * 8u + (unsigned)(iy & 7)) * 8u) + (unsigned)(ix & 7);
+    const float4* q = (const float4*)v.quad + ((size_t)brick * 576u + local);
+    float4 a = q[0];
+    float4 b = q[64];
+    float lx1, lx2, hx1, hx2, ly, hy;
+    if constexpr (POS) {
+        lx1 = lerp_positive(a.x, a.y, fx);
+        lx2 = lerp_positive(a.z, a.w, fx);
+        hx1 = lerp_positive(b.x, b.y, fx);
+        hx2 = lerp_positive(b.z, b.w, fx);
+        ly = lerp_positive(lx1, lx2, fy);
+        hy = lerp_positive(hx1, hx2, fy);
+        return lerp_positive(ly, hy, fz);
+    } else {
+        lx1 = lerp_general(a.x, a.y, fx);
+        lx2 = lerp_general(a.z, a.w, fx);
+        hx1 = lerp_general(b.x, b.y, fx);
+        hx2 = lerp_general(b.z, b.w, fx);
+        ly = lerp_general(lx1, lx2, fy);
+        hy = lerp_general(hx1, hx2, fy);
+        return lerp_general(ly, hy, fz);
+    }
+}
+
+// Trilinear sample from the [z][y][x] layout with floor/ceil corners (DataCube.cpp:224-270):
+// exact for any data, including inf/nan.
+__device__ __forceinline__ float sample_linear_tri(const DevVolume& v, const ExactIdx& r) {
+    float x0f = floorf(r.xd), y0f = floorf(r.yd), z0f = floorf(r.zd);
+    float fx = r.xd - x0f, fy = r.yd - y0f, fz = r.zd - z0f;
+    int x0 = to_index(x0f, v.nx - 2), y0 = to_index(y0f, v.ny - 2), z0 = to_index(z0f, v.nz - 2);
+    int x1 = x0 + (fx > 0.0f ? 1 : 0), y1 = y0 + (fy > 0.0f ? 1 : 0), z1 = z0 + (fz > 0.0f ? 1 : 0);
+    const float* L = v.linear;
+    size_t sy = (size_t)v.nx, sz = (size_t)v.nx * (size_t)v.ny;
+    float s000 = L[z0 * sz + y0 * sy + x0], s001 = L[z0 * sz + y0 * sy + x1];
+    float s010 = L[z0 * sz + y1 * sy + x0], s011 = L[z0 * sz + y1 * sy + x1];
+    float s100 = L[z1 * sz + y0 * sy + x0], s101 = L[z1 * sz + y0 * sy + x1];
+    float s110 = L[z1 * sz + y1 * sy + x0], s111 = L[z1 * sz + y1 * sy + x1];
+    float ly = lerp_general(lerp_general(s000, s001, fx), lerp_general(s010, s011, fx), fy);
+    float hy = lerp_general(lerp_general(s100, s101, fx), lerp_general(s110, s111, fx), fy);
+    return lerp_general(ly, hy, fz);
+}
+
+// Nearest: slices[(int)z][(int)y][(int)x] (DataCube.cpp:273), truncation.
+__device__ __forceinline__ float sample_linear_nearest(const DevVolume& v, const ExactIdx& r) {
+    int x = to_index(truncf(r.xd), v.nx - 2), y = to_index(truncf(r.yd), v.ny - 2), z = to_index(truncf(r.zd), v.nz - 2);
+    return v.linear[((size_t)z * v.ny + y) * v.nx + x];
+}
+
+// ------------------------------------------------------------------------------------------
+// CMEM::GetSample (Cmem.cpp:117-288) on the device: the reference's float/double types and
+// order; the transcendental functions are ROCm's device libm (within ~1 ulp of glibc's).
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ float cmem_sample(const CmemParams& m, float x, float y, float z) {
+    float r = sqrtf(x * x + y * y + z * z);
+    float theta = (x != 0) ? acosf(x / r) : 0.f;
+    float phi = (y != 0) ? acosf(y / sqrtf(y * y + z * z)) : 0.f;
+    // LinScaled (Cmem.cpp:188-222)
+    const double PI = 3.14159265358979323846;
+    float ct = cosf(theta), st = sinf(theta);
+    float phi_n = (float)acos((double)(ct * cosf(m.theta_n)) + (double)(st * sinf(m.theta_n)) * cos((double)phi - PI / 2));
+    float phi_s = (float)acos((double)(ct * cosf(m.theta_s)) + (double)(st * sinf(m.theta_s)) * cos((double)phi - (3 * PI / 2)));
+    float sin_phi = sinf(phi);
+    float f = powf(cosf(theta / 2) + m.a5 * sinf(2 * theta) * (1 - expf(-theta)),
+                   (m.p1 * (m.beta_c[0] + m.beta_c[1] * cosf(phi) + m.beta_c[3] * (sin_phi * sin_phi))));
+    float q = m.p2 * m.charlie * expf(m.p3 * m.dn * (powf(phi_n, m.a21))) +
+              m.p2 * m.charlie * expf(m.p3 * m.ds * (powf(phi_s, m.a21)));
+    float radius_mp = m.p0 * m.r0_lin * f + q;
+    // ShueModel (Cmem.cpp:280-288)
+    float base = 2 / (1 + ct);
+    float ry = m.bs * powf(base, m.ay_bs);
+    float rz = m.bs * powf(base, m.az_bs);
+    float rzc = rz * cosf(phi), rys = ry * sin_phi;
+    float radius_bs = (ry * rz) / sqrtf((rzc * rzc) + (rys * rys));
+    float h = 0.05f / 2;
+    if (r < radius_mp - h) return 0;
+    float half_t = theta / 2;
+    float inner = m.A1 * (expf(-m.B * (half_t * half_t * half_t * half_t))) * powf(r / 10, (-m.alpha - (m.beta * (st * st))));
+    if (r >= radius_mp - h && r <= radius_mp + h) {
+        float norm = (r - (radius_mp - h)) / (radius_mp + h);
+        return lerp_general(0.f, inner, norm);
+    } else if (r > radius_mp + h && r < radius_bs - h) {
+        return inner;
+    } else if (r >= radius_bs - h && r <= radius_bs + h) {
+        double sd = r / 10;
+        float norm = (r - (radius_bs - h)) / (radius_bs + h);
+        float b2 = (float)(m.A2 * (1 / (sd * sd * sd)));
+        return lerp_general(inner, b2, norm);
+    }
+    double sd = r / 10;
+    return (float)(m.A2 * (1 / (sd * sd * sd)));
+}
+
+// ------------------------------------------------------------------------------------------
+// cube entry: FP64 slab interval, widened by `margin`, mapped to a sample range, then walked
+// with the exact FP32 predicate so that the first and last samples are the ones the
+// reference's own bounds test accepts. The in-cube samples of a ray form one contiguous range
+// because every term of the FP32 index is monotone in t.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void slab_axis(double p, double d, double lo, double hi, double& tin, double& tout) {
+    if (d == 0.0) {
+        if (p < lo || p > hi) { tin = 1e300; tout = -1e300; }
+    } else {
+        double inv = 1.0 / d;
+        double t1 = (lo - p) * inv, t2 = (hi - p) * inv;
+        tin = fmax(tin, fmin(t1, t2));
+        tout = fmin(tout, fmax(t1, t2));
+    }
+}
+
+__device__ void exact_clip(const DevVolume& v, const ExactView& ev, const float* s_t, float dx, float dy, float dz,
+                           int& kf, int& kl) {
+    double m = (double)ev.margin;
+    double tin = -1e300, tout = 1e300;
+    slab_axis(ev.px, dx, (double)v.ox - m, (double)v.ox + (double)(v.nx - 1) * (double)v.sx + m, tin, tout);
+    slab_axis(ev.py, dy, (double)v.oy - m, (double)v.oy + (double)(v.ny - 1) * (double)v.sy + m, tin, tout);
+    slab_axis(ev.pz, dz, (double)v.oz - m, (double)v.oz + (double)(v.nz - 1) * (double)v.sz + m, tin, tout);
+    int n = ev.n;
+    if (!(tin <= tout)) { kf = n; kl = -1; return; }
+    double sc = (double)n / (double)ev.L;
+    double klo = ceil(tin * sc - 0.5) - 2.0, khi = floor(tout * sc - 0.5) + 2.0;
+    klo = fmin(fmax(klo, 0.0), (double)n);
+    khi = fmin(fmax(khi, -1.0), (double)(n - 1));
+    int a = (int)klo, b = (int)khi;
+    while (a <= b && !exact_inside(v, exact_index(v, dx, dy, dz, ev.px, ev.py, ev.pz, s_t[a]))) a++;
+    while (b >= a && !exact_inside(v, exact_index(v, dx, dy, dz, ev.px, ev.py, ev.pz, s_t[b]))) b--;
+    kf = a;
+    kl = b;
+}
+
+// Tile geometry of the exact kernels: a wave holds 64/S rays as TWxTH, a block 2x2 waves.
+template <int S> struct ExactTile {
+    static constexpr int TW = (S == 4) ? 4 : 8;
+    static constexpr int TH = (64 / S) / TW;
+};
+
+enum SampleMode { SM_QUAD_POS = 0, SM_QUAD_GEN = 1, SM_LIN_TRI = 2, SM_LIN_NEAREST = 3, SM_CMEM = 4 };
+
+template <int MODE>
+__device__ __forceinline__ float exact_sample(const DevVolume& v, float dx, float dy, float dz, float px, float py,
+                                              float pz, float t) {
+    if constexpr (MODE == SM_CMEM) {
+        float x = dx * t + px, y = dy * t + py, z = dz * t + pz;
+        return cmem_sample(v.cmem, x, y, z);
+    } else {
+        ExactIdx r = exact_index(v, dx, dy, dz, px, py, pz, t);
+        if constexpr (MODE == SM_QUAD_POS) return sample_quad<true>(v, r);
+        else if constexpr (MODE == SM_QUAD_GEN) return sample_quad<false>(v, r);
+        else if constexpr (MODE == SM_LIN_TRI) return sample_linear_tri(v, r);
+        else return sample_linear_nearest(v, r);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// the exact render kernel
+// ------------------------------------------------------------------------------------------
+template <int S, int MODE>
+__global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact(DevVolume v, ExactView ev, int wg_x, int nblocks) {
+    extern __shared__ float s_t[];
+    // ray distances t_k = L * ((k + 0.5f) / n) (Camera.cpp:73), IEEE division as on the host
+    for (int k = threadIdx.x; k < ev.n; k += BIRP_BLOCK) s_t[k] = ev.L * (((float)k + 0.5f) / (float)ev.n);
+    __syncthreads();
+
+    // XCD-aware block order: consecutive dispatch slots go round-robin over the 8 XCDs, so give
+    // each XCD a contiguous band of tiles (neighbouring rays share cube bricks in its L2).
+    int per_xcd = (nblocks + 7) / 8;
+    int lb = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (lb >= nblocks) return;
+
+    constexpr int TW = ExactTile<S>::TW, TH = ExactTile<S>::TH;
+    const int wave = threadIdx.x >> 6, lane = lane_id();
+    const int ray = lane / S, sub = lane % S;
+    const int i = ((lb % wg_x) * 2 + (wave & 1)) * TW + ray % TW;
+    const int j = ((lb / wg_x) * 2 + (wave >> 1)) * TH + ray / TW;
+    const bool valid = (i < ev.W) && (j < ev.H);
+
+    float dx = 0.f, dy = 0.f, dz = 1.f;
+    int kf = ev.n, kl = -1;
+    if (valid) {
+        exact_ray_dir(ev, i, j, dx, dy, dz);
+        if constexpr (MODE == SM_CMEM) {
+            kf = 0;
+            kl = ev.n - 1;
+        } else {
+            exact_clip(v, ev, s_t, dx, dy, dz, kf, kl);
+            if (ev.semantics == BIRP_SEM_REFERENCE && kf != 0) kl = -1;  // first sample outside: ray dead
+        }
+    }
+    const int kmin = wave_min_i(kf), kmax = wave_max_i(kl);
+
+    float acc = 0.f;
+    if (valid && ev.accumulate) acc = ev.out[(size_t)j * ev.W + i];
+    bool alive = true;
+    unsigned added = 0;
+    constexpr bool POS = (MODE == SM_QUAD_POS);
+    const float w = ev.w;
+
+    for (int kb = kmin; kb <= kmax; kb += S) {
+        const int k = kb + sub;
+        const float t = s_t[min(k, ev.n - 1)];
+        const bool act = (k >= kf) && (k <= kl);
+        const float val = exact_sample<MODE>(v, dx, dy, dz, ev.px, ev.py, ev.pz, t);
+        const float s = birp_scale_sample(val * w);  // Camera.cpp:201
+        if constexpr (S == 1) {
+            if constexpr (POS) {
+                acc = act ? acc + s : acc;
+                added += act ? 1u : 0u;
+            } else {
+                bool add = act && alive;
+                bool stop = add && (s < 0);  // Camera.cpp:207-210
+                acc = (add && !stop) ? acc + s : acc;
+                added += (add && !stop) ? 1u : 0u;
+                alive = alive && !stop;
+            }
+        } else {
+            // Gather the S samples of this trip in every lane of the ray, add them in k order.
+            float sv[S];
+            int av[S];
+            sv[0] = quad_bcast_f<S, 0>(s);
+            av[0] = quad_bcast<S, 0>(act ? 1 : 0);
+            sv[1] = quad_bcast_f<S, 1>(s);
+            av[1] = quad_bcast<S, 1>(act ? 1 : 0);
+            if constexpr (S == 4) {
+                sv[2] = quad_bcast_f<S, 2>(s);
+                av[2] = quad_bcast<S, 2>(act ? 1 : 0);
+                sv[3] = quad_bcast_f<S, 3>(s);
+                av[3] = quad_bcast<S, 3>(act ? 1 : 0);
+            }
+#pragma unroll
+            for (int mm = 0; mm < S; ++mm) {
+                if constexpr (POS) {
+                    acc = av[mm] ? acc + sv[mm] : acc;
+                    added += av[mm] ? 1u : 0u;
+                } else {
+                    bool add = av[mm] && alive;
+                    bool stop = add && (sv[mm] < 0);
+                    acc = (add && !stop) ? acc + sv[mm] : acc;
+                    added += (add && !stop) ? 1u : 0u;
+                    alive = alive && !stop;
+                }
+            }
+        }
+    }
+    if (valid && sub == 0) ev.out[(size_t)j * ev.W + i] = acc;
+    if (ev.counter) {
+        unsigned tot = wave_sum_u((valid && sub == 0) ? added : 0u);
+        if (lane == 0 && tot) atomicAdd(ev.counter, (unsigned long long)tot);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// FP64 render (every operation in double) for error reporting on large configurations
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ double lerp64(double a, double b, double t) {
+    if ((a <= 0 && b >= 0) || (a >= 0 && b <= 0)) return t * b + (1 - t) * a;
+    if (t == 1) return b;
+    double x = a + t * (b - a);
+    return ((t > 1) == (b > a)) ? (b < x ? x : b) : (b > x ? x : b);
+}
+
+__device__ double cmem_sample64(const CmemParams& m, double x, double y, double z) {
+    const double PI = 3.14159265358979323846;
+    double r = sqrt(x * x + y * y + z * z);
+    double theta = (x != 0) ? acos(x / r) : 0.0;
+    double phi = (y != 0) ? acos(y / sqrt(y * y + z * z)) : 0.0;
+    double phi_n = acos(cos(theta) * cos((double)m.theta_n) + sin(theta) * sin((double)m.theta_n) * cos(phi - PI / 2));
+    double phi_s = acos(cos(theta) * cos((double)m.theta_s) + sin(theta) * sin((double)m.theta_s) * cos(phi - 3 * PI / 2));
+    double sp = sin(phi);
+    double f = pow(cos(theta / 2) + m.a5 * sin(2 * theta) * (1 - exp(-theta)),
+                   m.p1 * (m.beta_c[0] + m.beta_c[1] * cos(phi) + (double)m.beta_c[3] * (sp * sp)));
+    double q = (double)m.p2 * m.charlie * exp((double)m.p3 * m.dn * pow(phi_n, (double)m.a21)) +
+               (double)m.p2 * m.charlie * exp((double)m.p3 * m.ds * pow(phi_s, (double)m.a21));
+    double rmp = (double)m.p0 * m.r0_lin * f + q;
+    double ry = m.bs * pow(2 / (1 + cos(theta)), (double)m.ay_bs);
+    double rz = m.bs * pow(2 / (1 + cos(theta)), (double)m.az_bs);
+    double rbs = (ry * rz) / sqrt((rz * cos(phi)) * (rz * cos(phi)) + (ry * sin(phi)) * (ry * sin(phi)));
+    double h = 0.025, hs = theta / 2, ss = sin(theta);
+    double inner = (double)m.A1 * exp(-(double)m.B * (hs * hs * hs * hs)) * pow(r / 10, -(double)m.alpha - (double)m.beta * (ss * ss));
+    double outer = (double)m.A2 / ((r / 10) * (r / 10) * (r / 10));
+    if (r < rmp - h) return 0.0;
+    if (r <= rmp + h) return lerp64(0.0, inner, (r - (rmp - h)) / (rmp + h));
+    if (r < rbs - h) return inner;
+    if (r <= rbs + h) return lerp64(inner, outer, (r - (rbs - h)) / (rbs + h));
+    return outer;
+}
+
+__global__ __launch_bounds__(BIRP_BLOCK) void k_render_f64(DevVolume v, F64View fv) {
+    int pix = blockIdx.x * BIRP_BLOCK + threadIdx.x;
+    if (pix >= fv.W * fv.H) return;
+    int i = pix % fv.W, j = pix / fv.W;
+    double sx = (2 * ((i + 0.5) / fv.W) - 1) * fv.tanx * fv.aspect;
+    double sy = (1 - 2 * ((j + 0.5) / fv.H)) * fv.tany;
+    double nd = sqrt(sx * sx + sy * sy + 1);
+    double u0 = sx / nd, u1 = sy / nd, u2 = -1 / nd;
+    double w0 = fv.rinv[0] * u0 + fv.rinv[1] * u1 + fv.rinv[2] * u2;
+    double w1 = fv.rinv[3] * u0 + fv.rinv[4] * u1 + fv.rinv[5] * u2;
+    double w2 = fv.rinv[6] * u0 + fv.rinv[7] * u1 + fv.rinv[8] * u2;
+    double wd = sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+    double dx = w0 / wd, dy = w1 / wd, dz = w2 / wd;
+    double acc = 0.0;
+    const double K = BIRP_SCALE_K, C = BIRP_SCALE_C;
+    for (int k = 0; k < fv.n; ++k) {
+        double t = fv.L * ((k + 0.5) / fv.n);
+        double x = dx * t + fv.px, y = dy * t + fv.py, z = dz * t + fv.pz;
+        double val;
+        if (v.kind == SK_CMEM) {
+            val = cmem_sample64(v.cmem, x, y, z);
+        } else {
+            double xi = x - (double)v.ox, yi = y - (double)v.oy, zi = z - (double)v.oz;
+            double xd = xi / (double)v.sx, yd = yi / (double)v.sy, zd = zi / (double)v.sz;
+            bool in = (xi >= 0) && (xd + 1 < v.nx) && (yi >= 0) && (yd + 1 < v.ny) && (zi >= 0) && (zd + 1 < v.nz);
+            if (!in) {
+                if (fv.semantics == BIRP_SEM_REFERENCE) break;
+                continue;
+            }
+            if (v.interp == BIRP_INTERP_TRILINEAR) {
+                int x0 = (int)floor(xd), y0 = (int)floor(yd), z0 = (int)floor(zd);
+                int x1 = (int)ceil(xd), y1 = (int)ceil(yd), z1 = (int)ceil(zd);
+                double fx = xd - x0, fy = yd - y0, fz = zd - z0;
+                const float* L = v.linear;
+                size_t sy_ = v.nx, sz_ = (size_t)v.nx * v.ny;
+                double lx1 = lerp64(L[z0 * sz_ + y0 * sy_ + x0], L[z0 * sz_ + y0 * sy_ + x1], fx);
+                double lx2 = lerp64(L[z0 * sz_ + y1 * sy_ + x0], L[z0 * sz_ + y1 * sy_ + x1], fx);
+                double hx1 = lerp64(L[z1 * sz_ + y0 * sy_ + x0], L[z1 * sz_ + y0 * sy_ + x1], fx);
+                double hx2 = lerp64(L[z1 * sz_ + y1 * sy_ + x0], L[z1 * sz_ + y1 * sy_ + x1], fx);
+                val = lerp64(lerp64(lx1, lx2, fy), lerp64(hx1, hx2, fy), fz);
+            } else {
+                val = v.linear[((size_t)(int)zd * v.ny + (int)yd) * v.nx + (int)xd];
+            }
+        }
+        double s = val * fv.w * K / C / 10000;
+        if (s < 0) break;
+        acc += s;
+    }
+    fv.out[pix] = acc;
+}
+
+// ------------------------------------------------------------------------------------------
+// GetSample at arbitrary points (DataCube.cpp:203-277 / Cmem.cpp:117-186), exact arithmetic
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(BIRP_BLOCK) void k_sample_points(DevVolume v, const float* xyz, long long n, float* out) {
+    long long p = (long long)blockIdx.x * BIRP_BLOCK + threadIdx.x;
+    if (p >= n) return;
+    float x = xyz[3 * p], y = xyz[3 * p + 1], z = xyz[3 * p + 2];
+    if (v.kind == SK_CMEM) {
+        out[p] = cmem_sample(v.cmem, x, y, z);
+        return;
+    }
+    ExactIdx r;
+    r.xi = x - v.ox;
+    r.xd = r.xi / v.sx;
+    r.yi = y - v.oy;
+    r.yd = r.yi / v.sy;
+    r.zi = z - v.oz;
+    r.zd = r.zi / v.sz;
+    if (!exact_inside(v, r)) {
+        out[p] = -1.f;
+        return;
+    }
+    out[p] = (v.interp == BIRP_INTERP_TRILINEAR) ? sample_linear_tri(v, r) : sample_linear_nearest(v, r);
+}
+
+// ------------------------------------------------------------------------------------------
+// layout builders and scans
+// ------------------------------------------------------------------------------------------
+// flags: bit 0 = some value <= 0, bit 1 = some value non-finite
+__global__ __launch_bounds__(BIRP_BLOCK) void k_scan_class(const float* vals, size_t n, unsigned* flags) {
+    unsigned f = 0;
+    for (size_t idx = (size_t)blockIdx.x * BIRP_BLOCK + threadIdx.x; idx < n; idx += (size_t)gridDim.x * BIRP_BLOCK) {
+        float x = vals[idx];
+        if (!(x > 0.f)) f |= 1u;
+        if (!isfinite(x)) f |= 2u;
+    }
+    for (int o = 32; o > 0; o >>= 1) f |= (unsigned)__shfl_xor((int)f, o, 64);
+    if (lane_id() == 0 && f) atomicOr(flags, f);
+}
+
+__global__ __launch_bounds__(BIRP_BLOCK) void k_build_quad(DevVolume v, float4* quad) {
+    // one thread per record slot (including padding inside edge bricks)
+    size_t nrec = (size_t)v.nbx * v.nby * v.nbz * 576u;
+    for (size_t r = (size_t)blockIdx.x * BIRP_BLOCK + threadIdx.x; r < nrec; r += (size_t)gridDim.x * BIRP_BLOCK) {
+        size_t brick = r / 576u;
+        int local = (int)(r % 576u);
+        int lx = local & 7, ly = (local >> 3) & 7, lz = local >> 6;  // lz in [0, 8]
+        int bx = (int)(brick % v.nbx), by = (int)((brick / v.nbx) % v.nby), bz = (int)(brick / ((size_t)v.nbx * v.nby));
+        int x = bx * 8 + lx, y = by * 8 + ly, z = bz * 8 + lz;
+        float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (x < v.nx - 1 && y < v.ny - 1 && z < v.nz) {
+            const float* L = v.linear;
+            size_t base = ((size_t)z * v.ny + y) * v.nx + x;
+            q.x = L[base];
+            q.y = L[base + 1];
+            q.z = L[base + v.nx];
+            q.w = L[base + v.nx + 1];
+        }
+        quad[r] = q;
+    }
+}
+
+__device__ __forceinline__ unsigned short f32_to_bf16_bits(float x) {
+    __hip_bfloat16 h = __float2bfloat16(x);
+    return *reinterpret_cast<unsigned short*>(&h);
+}
+__device__ __forceinline__ unsigned short f32_to_f16_bits(float x) {
+    __half h = __float2half_rn(x);
+    return *reinterpret_cast<unsigned short*>(&h);
+}
+
+// Per-brick power-of-two scale for F16: max|v| over the brick's corners maps to <= 2^15.
+__global__ __launch_bounds__(BIRP_BLOCK) void k_brick_scale(DevVolume v, float* bscale) {
+    int brick = blockIdx.x;
+    int bx = brick % v.nbx, by = (brick / v.nbx) % v.nby, bz = brick / (v.nbx * v.nby);
+    float mx = 0.f;
+    for (int t = threadIdx.x; t < 729; t += BIRP_BLOCK) {
+        int lx = t % 9, ly = (t / 9) % 9, lz = t / 81;
+        int x = bx * 8 + lx, y = by * 8 + ly, z = bz * 8 + lz;
+        if (x < v.nx && y < v.ny && z < v.nz) mx = fmaxf(mx, fabsf(v.linear[((size_t)z * v.ny + y) * v.nx + x]));
+    }
+    __shared__ float red[BIRP_BLOCK / 64];
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    if (lane_id() == 0) red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        float s = 1.f;
+        if (m > 0.f && isfinite(m)) {
+            int e;
+            frexpf(m, &e);           // m = f * 2^e, f in [0.5, 1)
+            s = ldexpf(1.f, e - 15); // m / s < 2^15
+        }
+        bscale[brick] = s;
+    }
+}
+
+template <bool F16>
+__global__ __launch_bounds__(BIRP_BLOCK) void k_build_oct(DevVolume v, uint4* oct, const float* bscale) {
+    size_t nrec = (size_t)v.nbx * v.nby * v.nbz * 512u;
+    for (size_t r = (size_t)blockIdx.x * BIRP_BLOCK + threadIdx.x; r < nrec; r += (size_t)gridDim.x * BIRP_BLOCK) {
+        size_t brick = r >> 9;
+        int local = (int)(r & 511u);
+        int lx = local & 7, ly = (local >> 3) & 7, lz = local >> 6;
+        int bx = (int)(brick % v.nbx), by = (int)((brick / v.nbx) % v.nby), bz = (int)(brick / ((size_t)v.nbx * v.nby));
+        int x = bx * 8 + lx, y = by * 8 + ly, z = bz * 8 + lz;
+        unsigned short h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (x < v.nx - 1 && y < v.ny - 1 && z < v.nz - 1) {
+            const float* L = v.linear;
+            size_t sy = v.nx, sz = (size_t)v.nx * v.ny;
+            size_t b0 = (size_t)z * sz + (size_t)y * sy + x;
+            float c[8] = {L[b0], L[b0 + 1], L[b0 + sy], L[b0 + sy + 1],
+                          L[b0 + sz], L[b0 + sz + 1], L[b0 + sz + sy], L[b0 + sz + sy + 1]};
+            float inv = F16 ? 1.f / bscale[brick] : 1.f;  // power of two: exact
+#pragma unroll
+            for (int q = 0; q < 8; q++) h[q] = F16 ? f32_to_f16_bits(c[q] * inv) : f32_to_bf16_bits(c[q]);
+        }
+        uint4 o;
+        o.x = (unsigned)h[0] | ((unsigned)h[1] << 16);
+        o.y = (unsigned)h[2] | ((unsigned)h[3] << 16);
+        o.z = (unsigned)h[4] | ((unsigned)h[5] << 16);
+        o.w = (unsigned)h[6] | ((unsigned)h[7] << 16);
+        oct[r] = o;
+    }
+}
+
+__global__ void k_reset_counter(unsigned* c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0u;
+}
+
+__global__ __launch_bounds__(BIRP_BLOCK) void k_zero_f32(float* p, size_t n) {
+    for (size_t idx = (size_t)blockIdx.x * BIRP_BLOCK + threadIdx.x; idx < n; idx += (size_t)gridDim.x * BIRP_BLOCK) p[idx] = 0.f;
+}
+
+// ------------------------------------------------------------------------------------------
+// fast path: persistent work-queue kernel over (pose, tile) items
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ float bf16lo(unsigned u) { return __uint_as_float(u << 16); }
+__device__ __forceinline__ float bf16hi(unsigned u) { return __uint_as_float(u & 0xFFFF0000u); }
+__device__ __forceinline__ float f16lo(unsigned u) { return __half2float(__ushort_as_half((unsigned short)(u & 0xFFFFu))); }
+__device__ __forceinline__ float f16hi(unsigned u) { return __half2float(__ushort_as_half((unsigned short)(u >> 16))); }
+
+__device__ __forceinline__ float flerp(float a, float b, float t) { return __builtin_fmaf(t, b - a, a); }
+
+// Fast trilinear sample at index-space point (xd, yd, zd), already inside [0, n-1).
+template <int DT>
+__device__ __forceinline__ float fast_sample(const DevVolume& v, float xd, float yd, float zd) {
+    float x0f = floorf(xd), y0f = floorf(yd), z0f = floorf(zd);
+    float fx = xd - x0f, fy = yd - y0f, fz = zd - z0f;
+    int ix = to_index(x0f, v.nx - 2), iy = to_index(y0f, v.ny - 2), iz = to_index(z0f, v.nz - 2);
+    unsigned brick = ((unsigned)(iz >> 3) * (unsigned)v.nby + (unsigned)(iy >> 3)) * (unsigned)v.nbx + (unsigned)(ix >> 3);
+    if constexpr (DT == BIRP_DTYPE_F32) {
+        unsigned local = (((unsigned)(iz & 7) * 8u + (unsigned)(iy & 7)) * 8u) + (unsigned)(ix & 7);
+        const float4* q = (const float4*)v.quad + ((size_t)brick * 576u + local);
+        float4 a = q[0], b = q[64];
+        float ly = flerp(flerp(a.x, a.y, fx), flerp(a.z, a.w, fx), fy);
+        float hy = flerp(flerp(b.x, b.y, fx), flerp(b.z, b.w, fx), fy);
+        return flerp(ly, hy, fz);
+    } else {
+        unsigned local = (((unsigned)(iz & 7) * 8u + (unsigned)(iy & 7)) * 8u) + (unsigned)(ix & 7);
+        uint4 o = ((const uint4*)v.oct)[(size_t)brick * 512u + local];
+        float c0, c1, c2, c3, c4, c5, c6, c7;
+        if constexpr (DT == BIRP_DTYPE_BF16) {
+            c0 = bf16lo(o.x); c1 = bf16hi(o.x); c2 = bf16lo(o.y); c3 = bf16hi(o.y);
+            c4 = bf16lo(o.z); c5 = bf16hi(o.z); c6 = bf16lo(o.w); c7 = bf16hi(o.w);
+        } else {
+            c0 = f16lo(o.x); c1 = f16hi(o.x); c2 = f16lo(o.y); c3 = f16hi(o.y);
+            c4 = f16lo(o.z); c5 = f16hi(o.z); c6 = f16lo(o.w); c7 = f16hi(o.w);
+        }
+        float ly = flerp(flerp(c0, c1, fx), flerp(c2, c3, fx), fy);
+        float hy = flerp(flerp(c4, c5, fx), flerp(c6, c7, fx), fy);
+        float val = flerp(ly, hy, fz);
+        if constexpr (DT == BIRP_DTYPE_F16) val *= v.bscale[brick];
+        return val;
+    }
+}
+
+// Kahan step: sum += x with compensation c.
+__device__ __forceinline__ void kahan_add(float& sum, float& c, float x) {
+    float y = x - c;
+    float t = sum + y;
+    c = (t - sum) - y;
+    sum = t;
+}
+
+template <int DT, int S, int U>
+__global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __restrict__ vols, const FastPose* __restrict__ poses,
+                                                            const int* __restrict__ prefix, int npose, int total,
+                                                            unsigned* counter, int tw_log2) {
+    const int lane = lane_id();
+    const int ray = lane / S, sub = lane % S;
+    const int tw = 1 << tw_log2;
+    const double K = BIRP_SCALE_FOLDED;
+    while (true) {
+        int item = 0;
+        if (lane == 0) item = (int)atomicAdd(counter, 1u);
+        item = __builtin_amdgcn_readfirstlane(item);
+        if (item >= total) break;
+        // pose = last p with prefix[p] <= item
+        int lo = 0, hi = npose - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= item) lo = mid; else hi = mid - 1;
+        }
+        const int pidx = __builtin_amdgcn_readfirstlane(lo);
+        const FastPose& P = poses[pidx];
+        const DevVolume& V = vols[P.vol];
+        const int tile = item - prefix[pidx];
+        const int tx = tile % P.tiles_x, ty = tile / P.tiles_x;
+        const int i = tx * tw + (ray & (tw - 1));
+        const int j = ty * ((64 / S) >> tw_log2) + (ray >> tw_log2);
+        const bool valid = (i < P.W) && (j < P.H);
+
+        int kf = P.n, kl = -1;
+        float ix0 = 0.f, iy0 = 0.f, iz0 = 0.f, dix = 0.f, diy = 0.f, diz = 0.f;
+        if (valid) {
+            // FP64 ray (same formulas as the reference, in double)
+            double sx = (2 * ((i + 0.5) / P.W) - 1) * P.tanx_aspect;
+            double sy = (1 - 2 * ((j + 0.5) / P.H)) * P.tany;
+            double nd = sqrt(sx * sx + sy * sy + 1.0);
+            double u0 = sx / nd, u1 = sy / nd, u2 = -1.0 / nd;
+            double w0 = P.rinv[0] * u0 + P.rinv[1] * u1 + P.rinv[2] * u2;
+            double w1 = P.rinv[3] * u0 + P.rinv[4] * u1 + P.rinv[5] * u2;
+            double w2 = P.rinv[6] * u0 + P.rinv[7] * u1 + P.rinv[8] * u2;
+            double wd = sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+            double dx = w0 / wd, dy = w1 / wd, dz = w2 / wd;
+            // FP64 cube entry/exit: inside iff 0 <= (x - o)/s < n - 1 per axis
+            double tin = -1e300, tout = 1e300;
+            slab_axis(P.px, dx, (double)V.ox, (double)V.ox + (double)(V.nx - 1) * V.sx, tin, tout);
+            slab_axis(P.py, dy, (double)V.oy, (double)V.oy + (double)(V.ny - 1) * V.sy, tin, tout);
+            slab_axis(P.pz, dz, (double)V.oz, (double)V.oz + (double)(V.nz - 1) * V.sz, tin, tout);
+            if (tin <= tout) {
+                double sc = (double)P.n / P.L;
+                double a = ceil(tin * sc - 0.5), b = ceil(tout * sc - 0.5) - 1.0;
+                a = fmax(a, 0.0);
+                b = fmin(b, (double)(P.n - 1));
+                if (a <= b) {
+                    kf = (int)a;
+                    kl = (int)b;
+                }
+            }
+            if (P.semantics == BIRP_SEM_REFERENCE && kf != 0) kl = -1;
+            if (kf <= kl) {
+                double t0 = P.L * ((kf + 0.5) / P.n), dt = P.L / P.n;
+                ix0 = (float)((P.px + dx * t0 - V.ox) / V.sx);
+                iy0 = (float)((P.py + dy * t0 - V.oy) / V.sy);
+                iz0 = (float)((P.pz + dz * t0 - V.oz) / V.sz);
+                dix = (float)(dx * dt / V.sx);
+                diy = (float)(dy * dt / V.sy);
+                diz = (float)(dz * dt / V.sz);
+            }
+        }
+        const int cnt = (kl >= kf) ? (kl - kf + 1) : 0;
+        const int my = (cnt > sub) ? (cnt - sub + S - 1) / S : 0;  // samples of this sub-lane
+        const int trips = wave_max_i(my);
+        const float fmaxx = V.fnx - 1.0f, fmaxy = V.fny - 1.0f, fmaxz = V.fnz - 1.0f;
+        float sum = 0.f, comp = 0.f;
+        for (int q = 0; q < trips; q += U) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int js = (q + u) * S + sub;  // sample offset from kf
+                const bool act = js < cnt;
+                const float fj = (float)js;
+                // clamp into the cube so that every lane's gather stays in bounds
+                float xd = fminf(fmaxf(__builtin_fmaf(fj, dix, ix0), 0.f), fmaxx);
+                float yd = fminf(fmaxf(__builtin_fmaf(fj, diy, iy0), 0.f), fmaxy);
+                float zd = fminf(fmaxf(__builtin_fmaf(fj, diz, iz0), 0.f), fmaxz);
+                float val = fast_sample<DT>(V, xd, yd, zd);
+                kahan_add(sum, comp, act ? val : 0.f);
+            }
+        }
+        // combine the S sub-lane partials in sub order, in double
+        double tot = (double)sum - (double)comp;
+        if constexpr (S > 1) {
+            double t1 = __shfl_xor(tot, 1, 64);
+            tot = (sub & 1) ? t1 + tot : tot + t1;
+            if constexpr (S == 4) {
+                double t2 = __shfl_xor(tot, 2, 64);
+                tot = (sub & 2) ? t2 + tot : tot + t2;
+            }
+        }
+        if (valid && sub == 0) P.out[(size_t)j * P.W + i] = (float)(tot * (double)P.w * K);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// host-side launch wrappers
+// ------------------------------------------------------------------------------------------
+static int hip_status(hipError_t e) { return e == hipSuccess ? BIRP_OK : BIRP_ERR_DEVICE; }
+
+template <int S>
+static int launch_exact_s(const DevVolume& v, const ExactView& ev, hipStream_t st) {
+    constexpr int TW = ExactTile<S>::TW, TH = ExactTile<S>::TH;
+    int wg_x = (ev.W + 2 * TW - 1) / (2 * TW);
+    int wg_y = (ev.H + 2 * TH - 1) / (2 * TH);
+    int nblocks = wg_x * wg_y;
+    int grid = ((nblocks + 7) / 8) * 8;
+    size_t lds = (size_t)ev.n * sizeof(float);
+    int mode;
+    if (v.kind == SK_CMEM) mode = SM_CMEM;
+    else if (v.interp == BIRP_INTERP_NEAREST) mode = SM_LIN_NEAREST;
+    else if (v.vclass == VC_POSITIVE) mode = SM_QUAD_POS;
+    else if (v.vclass == VC_FINITE) mode = SM_QUAD_GEN;
+    else mode = SM_LIN_TRI;
+    switch (mode) {
+        case SM_QUAD_POS: hipLaunchKernelGGL((k_render_exact<S, SM_QUAD_POS>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks); break;
+        case SM_QUAD_GEN: hipLaunchKernelGGL((k_render_exact<S, SM_QUAD_GEN>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks); break;
+        case SM_LIN_TRI: hipLaunchKernelGGL((k_render_exact<S, SM_LIN_TRI>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks); break;
+        case SM_LIN_NEAREST: hipLaunchKernelGGL((k_render_exact<S, SM_LIN_NEAREST>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks); break;
+        default: hipLaunchKernelGGL((k_render_exact<S, SM_CMEM>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks); break;
+    }
+    return hip_status(hipGetLastError());
+}
+
+int launch_exact(const DevVolume& v, const ExactView& ev, int lanes_per_ray, void* stream) {
+    if (ev.n < 2 || ev.n > BIRP_MAX_EXACT_SAMPLES || ev.W <= 0 || ev.H <= 0) return BIRP_ERR_INVALID_ARGUMENT;
+    hipStream_t st = (hipStream_t)stream;
+    if (lanes_per_ray == 4) return launch_exact_s<4>(v, ev, st);
+    if (lanes_per_ray == 2) return launch_exact_s<2>(v, ev, st);
+    return launch_exact_s<1>(v, ev, st);
+}
+
+int launch_f64(const DevVolume& v, const F64View& fv, void* stream) {
+    int npix = fv.W * fv.H;
+    hipLaunchKernelGGL(k_render_f64, dim3((npix + BIRP_BLOCK - 1) / BIRP_BLOCK), dim3(BIRP_BLOCK), 0, (hipStream_t)stream, v, fv);
+    return hip_status(hipGetLastError());
+}
+
+int launch_sample_points(const DevVolume& v, const float* xyz, long long n, float* out, void* stream) {
+    if (n <= 0) return BIRP_OK;
+    long long blocks = (n + BIRP_BLOCK - 1) / BIRP_BLOCK;
+    hipLaunchKernelGGL(k_sample_points, dim3((unsigned)blocks), dim3(BIRP_BLOCK), 0, (hipStream_t)stream, v, xyz, n, out);
+    return hip_status(hipGetLastError());
+}
+
+int launch_scan_class(const float* values, size_t n, unsigned* flags_dev, void* stream) {
+    size_t blocks = (n + BIRP_BLOCK - 1) / BIRP_BLOCK;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_scan_class, dim3((unsigned)blocks), dim3(BIRP_BLOCK), 0, (hipStream_t)stream, values, n, flags_dev);
+    return hip_status(hipGetLastError());
+}
+
+int launch_build_quad(const DevVolume& v, void* quad, void* stream) {
+    size_t nrec = (size_t)v.nbx * v.nby * v.nbz * 576u;
+    size_t blocks = (nrec + BIRP_BLOCK - 1) / BIRP_BLOCK;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_build_quad, dim3((unsigned)blocks), dim3(BIRP_BLOCK), 0, (hipStream_t)stream, v, (float4*)quad);
+    return hip_status(hipGetLastError());
+}
+
+int launch_build_oct(const DevVolume& v, void* oct, float* bscale, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    int nb = v.nbx * v.nby * v.nbz;
+    if (v.dtype == BIRP_DTYPE_F16) {
+        hipLaunchKernelGGL(k_brick_scale, dim3(nb), dim3(BIRP_BLOCK), 0, st, v, bscale);
+        if (hipGetLastError() != hipSuccess) return BIRP_ERR_DEVICE;
+    }
+    size_t nrec = (size_t)nb * 512u;
+    size_t blocks = (nrec + BIRP_BLOCK - 1) / BIRP_BLOCK;
+    if (blocks > 65536) blocks = 65536;
+    if (v.dtype == BIRP_DTYPE_F16)
+        hipLaunchKernelGGL((k_build_oct<true>), dim3((unsigned)blocks), dim3(BIRP_BLOCK), 0, st, v, (uint4*)oct, bscale);
+    else
+        hipLaunchKernelGGL((k_build_oct<false>), dim3((unsigned)blocks), dim3(BIRP_BLOCK), 0, st, v, (uint4*)oct, bscale);
+    return hip_status(hipGetLastError());
+}
+
+int launch_reset_counter(unsigned* counter, void* stream) {
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, (hipStream_t)stream, counter);
+    return hip_status(hipGetLastError());
+}
+
+int launch_zero_f32(float* p, size_t n, void* stream) {
+    size_t blocks = (n + BIRP_BLOCK - 1) / BIRP_BLOCK;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)blocks), dim3(BIRP_BLOCK), 0, (hipStream_t)stream, p, n);
+    return hip_status(hipGetLastError());
+}
+
+template <int DT, int S, int U>
+static void* fast_kernel_ptr() { return (void*)k_render_fast<DT, S, U>; }
+
+template <int DT, int S>
+static int launch_fast_su(int unroll, int grid, hipStream_t st, const DevVolume* vols, const FastPose* poses, const int* prefix,
+                          int npose, int total, unsigned* counter, int twl) {
+    switch (unroll) {
+        case 4: hipLaunchKernelGGL((k_render_fast<DT, S, 4>), dim3(grid), dim3(BIRP_BLOCK), 0, st, vols, poses, prefix, npose, total, counter, twl); break;
+        case 2: hipLaunchKernelGGL((k_render_fast<DT, S, 2>), dim3(grid), dim3(BIRP_BLOCK), 0, st, vols, poses, prefix, npose, total, counter, twl); break;
+        default: hipLaunchKernelGGL((k_render_fast<DT, S, 1>), dim3(grid), dim3(BIRP_BLOCK), 0, st, vols, poses, prefix, npose, total, counter, twl); break;
+    }
+    return hip_status(hipGetLastError());
+}
+
+template <int DT>
+static int launch_fast_d(int S, int unroll, int grid, hipStream_t st, const DevVolume* vols, const FastPose* poses,
+                         const int* prefix, int npose, int total, unsigned* counter, int twl) {
+    if (S == 4) return launch_fast_su<DT, 4>(unroll, grid, st, vols, poses, prefix, npose, total, counter, twl);
+    if (S == 2) return launch_fast_su<DT, 2>(unroll, grid, st, vols, poses, prefix, npose, total, counter, twl);
+    return launch_fast_su<DT, 1>(unroll, grid, st, vols, poses, prefix, npose, total, counter, twl);
+}
+
+int launch_fast_batch(const DevVolume* vols_dev, int dtype, const FastPose* poses_dev, const int* tile_prefix_dev, int npose,
+                      int total_tiles, unsigned* work_counter, int tile_w, int tile_h, int lanes_per_ray, int unroll,
+                      int grid, void* stream) {
+    int twl = 0;
+    while ((1 << twl) < tile_w) twl++;
+    if ((1 << twl) != tile_w || tile_w * tile_h * lanes_per_ray != 64) return BIRP_ERR_INVALID_ARGUMENT;
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == BIRP_DTYPE_F16)
+        return launch_fast_d<BIRP_DTYPE_F16>(lanes_per_ray, unroll, grid, st, vols_dev, poses_dev, tile_prefix_dev, npose, total_tiles, work_counter, twl);
+    if (dtype == BIRP_DTYPE_BF16)
+        return launch_fast_d<BIRP_DTYPE_BF16>(lanes_per_ray, unroll, grid, st, vols_dev, poses_dev, tile_prefix_dev, npose, total_tiles, work_counter, twl);
+    return launch_fast_d<BIRP_DTYPE_F32>(lanes_per_ray, unroll, grid, st, vols_dev, poses_dev, tile_prefix_dev, npose, total_tiles, work_counter, twl);
+}
+
+int fast_grid_size(int dtype, int lanes_per_ray, int unroll) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    int cus = 0;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (cus <= 0) cus = 256;
+    int per_cu = 0;
+    void* fn = nullptr;
+#define BIRP_FK(DT)                                                                          \
+    if (lanes_per_ray == 4) fn = unroll == 4 ? fast_kernel_ptr<DT, 4, 4>() : unroll == 2 ? fast_kernel_ptr<DT, 4, 2>() : fast_kernel_ptr<DT, 4, 1>(); \
+    else if (lanes_per_ray == 2) fn = unroll == 4 ? fast_kernel_ptr<DT, 2, 4>() : unroll == 2 ? fast_kernel_ptr<DT, 2, 2>() : fast_kernel_ptr<DT, 2, 1>(); \
+    else fn = unroll == 4 ? fast_kernel_ptr<DT, 1, 4>() : unroll == 2 ? fast_kernel_ptr<DT, 1, 2>() : fast_kernel_ptr<DT, 1, 1>();
+    if (dtype == BIRP_DTYPE_F16) { BIRP_FK(BIRP_DTYPE_F16) }
+    else if (dtype == BIRP_DTYPE_BF16) { BIRP_FK(BIRP_DTYPE_BF16) }
+    else { BIRP_FK(BIRP_DTYPE_F32) }
+#undef BIRP_FK
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BIRP_BLOCK, 0) != hipSuccess || per_cu <= 0) per_cu = 4;
+    return cus * per_cu;
+}
+
+}  // namespace birp

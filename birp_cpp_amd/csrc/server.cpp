@@ -1,0 +1,383 @@
+// server.cpp -- continuous-batching render server (include/birp.h "serving").
+//
+// One dispatcher thread owns `max_inflight` launch slots, each with its own HIP stream (default
+// priority), descriptor buffers and pinned result staging. Requests queue on submission; whenever
+// a slot is free the dispatcher admits queued requests in arrival order while their in-flight
+// footprint (device image + pinned staging) fits the memory budget left after the resident
+// volumes, and launches them as one persistent-kernel batch (one launch per storage dtype). A
+// request that arrives while other batches are on the GPU is launched in the next free slot, so
+// its work overlaps theirs on the device. Every request is rendered by the same kernel tuning and
+// the same per-tile arithmetic whatever it is batched with, so results do not depend on the mix.
+// No reference counterpart: the reference renders one pose per process run (main.cpp:266-316).
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/birp.h"
+#include "capi_internal.h"
+
+using namespace birp;
+
+extern "C" double birp_clock_s(void) {
+    using namespace std::chrono;
+    return duration_cast<duration<double>>(steady_clock::now().time_since_epoch()).count();
+}
+
+namespace {
+
+struct Request {
+    uint64_t id = 0;
+    birp_request req{};
+    birp_volume* vol = nullptr;  // captured at submission
+    float* result = nullptr;
+    size_t pixels = 0;
+    size_t bytes = 0;  // in-flight footprint
+    double submit_s = 0, start_s = 0, done_s = 0;
+    int batch_size = 0;
+    int status = BIRP_OK;
+    bool finished = false;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    FastPose* h_poses = nullptr;  // pinned
+    FastPose* d_poses = nullptr;
+    int* h_prefix = nullptr;      // pinned, 3 groups x (max_batch + 1)
+    int* d_prefix = nullptr;
+    unsigned* d_work = nullptr;   // one counter per dtype group
+    float* d_images = nullptr;
+    float* h_images = nullptr;    // pinned
+    size_t image_cap = 0;
+    std::vector<Request*> reqs;
+    std::vector<size_t> offsets;
+    size_t bytes = 0;
+    bool busy = false;
+};
+
+}  // namespace
+
+struct birp_server {
+    birp_device* dev = nullptr;
+    birp_server_config cfg{};
+    std::vector<birp_volume*> vols;
+    DevVolume* d_vols = nullptr;
+    int vol_cap = 64;
+    size_t resident = 0;
+    size_t inflight_bytes = 0;
+    birp_tuning tuning{4, 4, 4, 2};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> pending;
+    std::unordered_map<uint64_t, Request*> reqs;
+    uint64_t next_id = 1;
+    std::vector<Slot> slots;
+    std::thread worker;
+    bool stop = false;
+    int setup_error = BIRP_OK;
+};
+
+namespace {
+
+int slot_reserve_images(Slot& s, size_t pixels) {
+    if (s.image_cap >= pixels) return BIRP_OK;
+    if (s.d_images) (void)hipFree(s.d_images);
+    if (s.h_images) (void)hipHostFree(s.h_images);
+    s.d_images = nullptr;
+    s.h_images = nullptr;
+    s.image_cap = 0;
+    size_t cap = std::max(pixels, (size_t)1 << 20);
+    hipError_t e = hipMalloc(&s.d_images, cap * 4);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(server images)");
+    e = hipHostMalloc(&s.h_images, cap * 4, hipHostMallocDefault);
+    if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(server images)");
+    s.image_cap = cap;
+    return BIRP_OK;
+}
+
+// Launch the admitted requests of slot s (dispatcher thread, lock not held).
+int launch_slot(birp_server* S, Slot& s) {
+    const int nb = S->cfg.max_batch;
+    size_t total = 0;
+    s.offsets.clear();
+    for (Request* r : s.reqs) {
+        s.offsets.push_back(total);
+        total += r->pixels;
+    }
+    int rc = slot_reserve_images(s, total);
+    if (rc) return rc;
+    const birp_tuning& t = S->tuning;
+    // group by dtype: poses of group g occupy [g*nb, g*nb + count_g)
+    int counts[3] = {0, 0, 0};
+    for (size_t k = 0; k < s.reqs.size(); k++) {
+        Request* r = s.reqs[k];
+        birp_volume* vol = r->vol;
+        int g = vol->desc.dtype;
+        FastPose& fp = s.h_poses[g * nb + counts[g]];
+        rc = fast_pose(vol, &r->req.view, r->req.volume, s.d_images + s.offsets[k], t.tile_w, t.tile_h, fp);
+        if (rc) return rc;
+        counts[g]++;
+    }
+    for (int g = 0; g < 3; g++) {
+        int* pre = s.h_prefix + g * (nb + 1);
+        pre[0] = 0;
+        for (int k = 0; k < counts[g]; k++) {
+            const FastPose& fp = s.h_poses[g * nb + k];
+            birp_view tmp{};
+            tmp.width = fp.W;
+            tmp.height = fp.H;
+            pre[k + 1] = pre[k] + tiles_of(tmp, t.tile_w, t.tile_h);
+        }
+    }
+    hipError_t e = hipMemcpyAsync(s.d_poses, s.h_poses, sizeof(FastPose) * 3 * nb, hipMemcpyHostToDevice, s.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s.d_prefix, s.h_prefix, sizeof(int) * 3 * (nb + 1), hipMemcpyHostToDevice, s.stream);
+    if (e != hipSuccess) return hip_fail(e, "server descriptor upload");
+    for (int g = 0; g < 3; g++) {
+        if (!counts[g]) continue;
+        int total_tiles = s.h_prefix[g * (nb + 1) + counts[g]];
+        if ((rc = launch_reset_counter(s.d_work + g * 16, s.stream))) return rc;
+        int grid = std::min(fast_grid_size(g, t.lanes_per_ray, t.unroll), (total_tiles + 3) / 4 + 1);
+        rc = launch_fast_batch(S->d_vols, g, s.d_poses + g * nb, s.d_prefix + g * (nb + 1), counts[g], total_tiles, s.d_work + g * 16,
+                               t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll, grid, s.stream);
+        if (rc) return rc;
+    }
+    e = hipMemcpyAsync(s.h_images, s.d_images, total * 4, hipMemcpyDeviceToHost, s.stream);
+    if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
+    if (e != hipSuccess) return hip_fail(e, "server launch");
+    return BIRP_OK;
+}
+
+void finish_slot(birp_server* S, Slot& s, int status) {
+    double now = birp_clock_s();
+    if (status == BIRP_OK) {
+        for (size_t k = 0; k < s.reqs.size(); k++)
+            memcpy(s.reqs[k]->result, s.h_images + s.offsets[k], s.reqs[k]->pixels * 4);
+    }
+    std::lock_guard<std::mutex> lk(S->mu);
+    for (Request* r : s.reqs) {
+        r->status = status;
+        r->done_s = now;
+        r->finished = true;
+    }
+    S->inflight_bytes -= s.bytes;
+    s.bytes = 0;
+    s.reqs.clear();
+    s.busy = false;
+    S->cv_done.notify_all();
+}
+
+void dispatcher(birp_server* S) {
+    (void)hipSetDevice(S->dev->ordinal);
+    const size_t budget = S->cfg.memory_budget;
+    while (true) {
+        // 1. retire finished launches
+        bool any_busy = false;
+        for (Slot& s : S->slots) {
+            if (!s.busy) continue;
+            hipError_t q = hipEventQuery(s.done);
+            if (q == hipSuccess) finish_slot(S, s, BIRP_OK);
+            else if (q != hipErrorNotReady) finish_slot(S, s, hip_fail(q, "server batch"));
+            else any_busy = true;
+        }
+        // 2. admit into a free slot
+        Slot* free_slot = nullptr;
+        for (Slot& s : S->slots)
+            if (!s.busy) { free_slot = &s; break; }
+        std::vector<Request*> failed;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            if (S->stop && S->pending.empty() && !any_busy) break;
+            if (free_slot && !S->pending.empty()) {
+                // a short window lets a trickle of arrivals share a launch while the GPU is busy
+                if (any_busy && (int)S->pending.size() < S->cfg.max_batch && S->cfg.batch_window_ms > 0) {
+                    double oldest = S->pending.front()->submit_s;
+                    if (birp_clock_s() - oldest < S->cfg.batch_window_ms * 1e-3) {
+                        lk.unlock();
+                        std::this_thread::sleep_for(std::chrono::microseconds(20));
+                        continue;
+                    }
+                }
+                double now = birp_clock_s();
+                while (!S->pending.empty() && (int)free_slot->reqs.size() < S->cfg.max_batch) {
+                    Request* r = S->pending.front();
+                    size_t avail = budget > S->resident ? budget - S->resident : 0;
+                    if (r->bytes > avail) {  // can never fit
+                        S->pending.pop_front();
+                        r->status = BIRP_ERR_ADMISSION;
+                        failed.push_back(r);
+                        continue;
+                    }
+                    if (S->inflight_bytes + free_slot->bytes + r->bytes > avail) break;  // wait for memory
+                    S->pending.pop_front();
+                    if (take_injected_failure(S->dev)) {
+                        r->status = BIRP_ERR_INJECTED;
+                        failed.push_back(r);
+                        continue;
+                    }
+                    r->start_s = now;
+                    free_slot->reqs.push_back(r);
+                    free_slot->bytes += r->bytes;
+                }
+                for (Request* r : free_slot->reqs) r->batch_size = (int)free_slot->reqs.size();
+                S->inflight_bytes += free_slot->bytes;
+                if (!failed.empty()) {
+                    for (Request* r : failed) {
+                        r->done_s = now;
+                        r->finished = true;
+                    }
+                    S->cv_done.notify_all();
+                }
+            } else if (!any_busy && S->pending.empty()) {
+                S->cv_work.wait(lk, [&] { return S->stop || !S->pending.empty(); });
+                continue;
+            }
+        }
+        if (free_slot && !free_slot->reqs.empty()) {
+            free_slot->busy = true;
+            int rc = launch_slot(S, *free_slot);
+            if (rc) finish_slot(S, *free_slot, rc);
+            continue;
+        }
+        if (any_busy) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+}
+
+}  // namespace
+
+extern "C" int birp_server_create(birp_device* dev, const birp_server_config* cfg, birp_server** out) {
+    if (!dev || !cfg || !out) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
+    if (cfg->max_batch < 1 || cfg->max_batch > 4096 || cfg->max_inflight < 1 || cfg->max_inflight > 8)
+        return fail(BIRP_ERR_INVALID_ARGUMENT, "max_batch in [1,4096], max_inflight in [1,8]");
+    (void)hipSetDevice(dev->ordinal);
+    birp_server* S = new birp_server();
+    S->dev = dev;
+    S->cfg = *cfg;
+    birp_tuning t;
+    if (birp_autotune_lookup(BIRP_DTYPE_F16, 576, 336, 800, &t) == BIRP_OK) S->tuning = t;
+    hipError_t e = hipMalloc(&S->d_vols, sizeof(DevVolume) * S->vol_cap);
+    int rc = BIRP_OK;
+    if (e != hipSuccess) rc = hip_fail(e, "hipMalloc(server volumes)");
+    S->slots.resize(cfg->max_inflight);
+    const int nb = cfg->max_batch;
+    for (Slot& s : S->slots) {
+        if (rc) break;
+        if ((e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking)) != hipSuccess) { rc = hip_fail(e, "stream"); break; }
+        if ((e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming)) != hipSuccess) { rc = hip_fail(e, "event"); break; }
+        if ((e = hipHostMalloc(&s.h_poses, sizeof(FastPose) * 3 * nb, hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
+        if ((e = hipMalloc(&s.d_poses, sizeof(FastPose) * 3 * nb)) != hipSuccess) { rc = hip_fail(e, "poses"); break; }
+        if ((e = hipHostMalloc(&s.h_prefix, sizeof(int) * 3 * (nb + 1), hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
+        if ((e = hipMalloc(&s.d_prefix, sizeof(int) * 3 * (nb + 1))) != hipSuccess) { rc = hip_fail(e, "prefix"); break; }
+        if ((e = hipMalloc(&s.d_work, sizeof(unsigned) * 48)) != hipSuccess) { rc = hip_fail(e, "counter"); break; }
+    }
+    if (rc) {
+        birp_server_destroy(S);
+        return rc;
+    }
+    S->worker = std::thread(dispatcher, S);
+    *out = S;
+    return BIRP_OK;
+}
+
+extern "C" int birp_server_destroy(birp_server* S) {
+    if (!S) return BIRP_OK;
+    if (S->worker.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->stop = true;
+        }
+        S->cv_work.notify_all();
+        S->worker.join();
+    }
+    for (Slot& s : S->slots) {
+        if (s.stream) (void)hipStreamSynchronize(s.stream);
+        if (s.d_images) (void)hipFree(s.d_images);
+        if (s.h_images) (void)hipHostFree(s.h_images);
+        if (s.h_poses) (void)hipHostFree(s.h_poses);
+        if (s.d_poses) (void)hipFree(s.d_poses);
+        if (s.h_prefix) (void)hipHostFree(s.h_prefix);
+        if (s.d_prefix) (void)hipFree(s.d_prefix);
+        if (s.d_work) (void)hipFree(s.d_work);
+        if (s.done) (void)hipEventDestroy(s.done);
+        if (s.stream) (void)hipStreamDestroy(s.stream);
+    }
+    if (S->d_vols) (void)hipFree(S->d_vols);
+    for (auto& kv : S->reqs) delete kv.second;
+    delete S;
+    return BIRP_OK;
+}
+
+extern "C" int birp_server_add_volume(birp_server* S, birp_volume* vol, int* index) {
+    if (!S || !vol || !index) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
+    if (vol->dev != S->dev) return fail(BIRP_ERR_INVALID_ARGUMENT, "volume on another device");
+    if (vol->dv.kind != SK_CUBE) return fail(BIRP_ERR_UNSUPPORTED, "the server renders cube volumes");
+    std::lock_guard<std::mutex> lk(S->mu);
+    if ((int)S->vols.size() >= S->vol_cap) return fail(BIRP_ERR_OUT_OF_MEMORY, "volume table full");
+    if (S->resident + vol->bytes > S->cfg.memory_budget) return fail(BIRP_ERR_ADMISSION, "volume does not fit the memory budget");
+    hipError_t e = hipMemcpy(S->d_vols + S->vols.size(), &vol->dv, sizeof(DevVolume), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "volume table upload");
+    S->resident += vol->bytes;
+    *index = (int)S->vols.size();
+    S->vols.push_back(vol);
+    return BIRP_OK;
+}
+
+extern "C" int birp_server_submit(birp_server* S, const birp_request* req, float* result, uint64_t* id) {
+    if (!S || !req || !result || !id) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
+    birp_view v = req->view;
+    v.math = BIRP_MATH_FAST;
+    v.flags = 0;
+    int rc = validate_view(&v, false);
+    if (rc) return rc;
+    Request* r = new Request();
+    r->req = *req;
+    r->req.view = v;
+    r->result = result;
+    r->pixels = (size_t)v.width * v.height;
+    r->bytes = r->pixels * 8;  // device image + pinned staging
+    r->submit_s = birp_clock_s();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (req->volume < 0 || req->volume >= (int)S->vols.size()) {
+            delete r;
+            return fail(BIRP_ERR_NOT_FOUND, "unknown volume index");
+        }
+        r->vol = S->vols[req->volume];
+        r->id = S->next_id++;
+        S->reqs[r->id] = r;
+        S->pending.push_back(r);
+    }
+    S->cv_work.notify_one();
+    *id = r->id;
+    return BIRP_OK;
+}
+
+extern "C" int birp_server_wait(birp_server* S, uint64_t id, double timeout_ms, birp_result_info* info) {
+    if (!S) return fail(BIRP_ERR_INVALID_ARGUMENT, "null server");
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return fail(BIRP_ERR_NOT_FOUND, "unknown request id");
+    Request* r = it->second;
+    auto ready = [&] { return r->finished; };
+    if (timeout_ms < 0) S->cv_done.wait(lk, ready);
+    else if (!S->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready))
+        return fail(BIRP_ERR_TIMEOUT, "request not finished");
+    if (info) {
+        info->status = r->status;
+        info->submit_s = r->submit_s;
+        info->start_s = r->start_s;
+        info->done_s = r->done_s;
+        info->batch_size = r->batch_size;
+    }
+    int status = r->status;
+    S->reqs.erase(it);
+    delete r;
+    return status == BIRP_OK ? BIRP_OK : fail(status, birp_strerror(status));
+}

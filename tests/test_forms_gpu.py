@@ -1,0 +1,216 @@
+"""GPU tests of the other forms: CMEM renders, the FP64 render, the fast (FP64 entry, Kahan) path
+with FP32/BF16/F16 storage, batched persistent-kernel renders under a HIP graph, and the
+continuous-batching server.
+
+Tolerances (image relative L2 error):
+  CMEM exact path vs the FP32 oracle ......... 1e-5 (device libm vs glibc, ~1 ulp per call)
+  FP64 render vs the oracle's FP64 render .... 1e-12
+  fast FP32 storage vs FP64 .................. 1e-5
+  fast F16 storage (per-brick scale) vs FP64 . 1e-4
+  fast BF16 storage vs FP64 .................. reported; bounded by 1e-2
+"""
+import os
+
+import numpy as np
+import pytest
+
+from birp_cpp_amd import capi
+from birp_cpp_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+NT = min(16, os.cpu_count() or 1)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def oview(O, v):
+    return O.make_view(v.width, v.height, v.fov_w, v.fov_h, list(v.position), list(v.aim), v.ray_length, v.samples)
+
+
+@pytest.fixture(scope="module")
+def cube1(device, oracle):
+    vals, org, spc, _ = W.cube_config1()
+    vols = {dt: capi.Volume.upload(device, vals, org, spc, dtype=dt) for dt in (capi.DTYPE_F32, capi.DTYPE_BF16, capi.DTYPE_F16)}
+    yield vals, org, spc, vols, oracle.OracleCube(vals, org, spc)
+    for v in vols.values():
+        v.free()
+
+
+# ---------------------------------------------------------------------------------- CMEM
+@pytest.mark.parametrize("params", [None, dict(v=[400, 0, 0], b=[0, 0, 4], dipole=0.0, p0=1, p1=4, p2=6, p3=3, B=3.6, alpha=-1.0)])
+def test_cmem_points_and_render(device, oracle, params):
+    oc = oracle.OracleCmem(params)
+    vol = capi.Volume.cmem(device, capi.cmem_state_from(oc.state()))
+    try:
+        # the reference's own known answers (tests/CMEM_test.cpp:14-90)
+        if params is None:
+            pts = [(8, 4, 4), (4, 12, 10), (3, 1.5, 1.2), (8, 10, 10), (8, -10, -10), (8, 10, -10)]
+            want = [0.0, 4.46853346e-06, 0.0, 8.15947317e-07, 8.15947317e-07, 8.15947317e-07]
+        else:
+            pts = [(8, 4, 4), (12, 12, 10), (3, 1.5, 1.2), (12, 10, 10), (12, -10, -10), (12, 10, -10)]
+            want = [0.0, 4.57952495e-07, 0.0, 5.48567925e-07, 5.48567925e-07, 5.48567925e-07]
+        got = vol.sample(np.array(pts, np.float32))
+        np.testing.assert_allclose(got, np.array(want, np.float32), rtol=4e-6, atol=0)
+        rng = np.random.default_rng(3)
+        P = rng.uniform(-20, 20, (5000, 3)).astype(np.float32)
+        g = vol.sample(P)
+        r = np.array([oc.sample(*map(float, p)) for p in P], np.float32)
+        close = np.isclose(g, r, rtol=2e-5, atol=1e-12)
+        assert close.mean() > 0.999, f"{(~close).sum()} of {len(P)} CMEM points differ"
+        v = W.view(144, 84, 0.25, pos=(1.0, 1.0, 1.0), aim=(1.0, 0.0, 0.0), length=44.0)
+        img, st = vol.render(v)
+        ref, _ = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT)
+        assert rel_l2(img, ref) <= 1e-5
+        assert st.samples_in_cube == v.width * v.height * v.samples
+    finally:
+        vol.free()
+
+
+# ---------------------------------------------------------------------------------- FP64
+def test_f64_render_matches_oracle_f64(cube1, oracle):
+    vals, org, spc, vols, oc = cube1
+    v = W.view_config1()
+    got = vols[capi.DTYPE_F32].render_f64(v)
+    ref, _ = oracle.render_f64(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT)
+    assert rel_l2(got, ref) <= 1e-12
+    img, _ = vols[capi.DTYPE_F32].render(v)
+    e32 = rel_l2(img, ref)
+    print(f"exact FP32 path vs FP64 oracle: {e32:.3e}")
+    assert e32 <= 1e-5
+
+
+# ---------------------------------------------------------------------------------- fast path
+@pytest.mark.parametrize("dtype,tol", [(capi.DTYPE_F32, 1e-5), (capi.DTYPE_F16, 1e-4), (capi.DTYPE_BF16, 1e-2)])
+def test_fast_path_vs_fp64(cube1, oracle, dtype, tol):
+    vals, org, spc, vols, oc = cube1
+    for pose in [W.CAM_POS, (15.0, 5.0, 8.0), (3.0, 17.0, 12.0)]:
+        v = W.view(144, 84, 0.25, pos=pose, math=capi.MATH_FAST)
+        ref = vols[capi.DTYPE_F32].render_f64(v)
+        img, _ = vols[dtype].render(v)
+        err = rel_l2(img, ref)
+        mx = float(np.max(np.abs(img - ref) / np.maximum(np.abs(ref), 1e-30)))
+        print(f"dtype {dtype} pose {pose}: rel L2 {err:.3e} max rel {mx:.3e}")
+        assert err <= tol
+
+
+def test_fast_path_reference_semantics(cube1, oracle):
+    vals, org, spc, vols, oc = cube1
+    v = W.view(96, 64, 0.25, pos=(15.0, 5.0, 8.0), aim=(0.0, 0.0, 0.0), semantics=capi.SEM_REFERENCE, math=capi.MATH_FAST, length=44.0)
+    ref = vols[capi.DTYPE_F32].render_f64(v)
+    img, _ = vols[capi.DTYPE_F32].render(v)
+    assert rel_l2(img, ref) <= 1e-5
+    outside = W.view(32, 32, 0.5, semantics=capi.SEM_REFERENCE, math=capi.MATH_FAST)
+    img, _ = vols[capi.DTYPE_F32].render(outside)
+    assert not img.any()
+
+
+# ---------------------------------------------------------------------------------- batch
+def test_batch_matches_fp64_and_replays(cube1):
+    vals, org, spc, vols, oc = cube1
+    views = W.views_config3(count=24)
+    b = capi.Batch(vols[capi.DTYPE_F16], views)
+    try:
+        b.launch()
+        b.wait()
+        first = [b.image(k) for k in range(len(views))]
+        for k, v in enumerate(views):
+            ref = vols[capi.DTYPE_F32].render_f64(v)
+            assert rel_l2(first[k], ref) <= 1e-4, k
+        # graph replays with eager work in between give identical images
+        for _ in range(3):
+            _ = vols[capi.DTYPE_F32].render(W.view_config1())
+            b.launch()
+            b.wait()
+            for k in range(len(views)):
+                assert np.array_equal(b.image(k), first[k])
+        n, ms = b.stats()
+        assert n == sum(v.width * v.height * v.samples for v in views) and ms > 0
+    finally:
+        b.free()
+
+
+@pytest.mark.parametrize("tuning", [(8, 8, 1, 1), (16, 4, 1, 2), (8, 4, 2, 4), (4, 8, 2, 1), (4, 4, 4, 2), (8, 2, 4, 4)])
+def test_batch_tunings_agree(cube1, tuning):
+    vals, org, spc, vols, oc = cube1
+    views = W.views_config3(count=6)
+    b = capi.Batch(vols[capi.DTYPE_F32], views)
+    try:
+        b.set_tuning(capi.Tuning(*tuning))
+        b.launch()
+        b.wait()
+        for k, v in enumerate(views):
+            ref = vols[capi.DTYPE_F32].render_f64(v)
+            assert rel_l2(b.image(k), ref) <= 1e-5
+    finally:
+        b.free()
+
+
+def test_autotune_roundtrip(cube1, tmp_path):
+    vals, org, spc, vols, oc = cube1
+    views = W.views_config3(count=8)
+    path = str(tmp_path / "tune.jsonl")
+    capi.autotune(vols[capi.DTYPE_F16], views, path)
+    assert capi.lib().birp_autotune_load(path.encode()) == capi.OK
+    v = views[0]
+    t = capi.autotune_lookup(capi.DTYPE_F16, v.width, v.height, v.samples)
+    assert t is not None and t.tile_w * t.tile_h * t.lanes_per_ray == 64
+
+
+# ---------------------------------------------------------------------------------- server
+def test_server_alone_equals_in_mix(device, cube1):
+    vals, org, spc, vols, oc = cube1
+    srv = capi.Server(device, memory_budget=4 << 30, max_batch=16, max_inflight=3)
+    try:
+        i16 = srv.add_volume(vols[capi.DTYPE_F16])
+        i32 = srv.add_volume(vols[capi.DTYPE_F32])
+        views = W.views_config3(count=20)
+        alone = []
+        for k, v in enumerate(views):
+            rid = srv.submit(i16 if k % 2 else i32, v)
+            st, img, info = srv.wait(rid, 30000.0)
+            assert st == capi.OK and info.batch_size == 1
+            alone.append(img)
+        ids = [srv.submit(i16 if k % 2 else i32, v) for k, v in enumerate(views)]
+        sizes = []
+        for k, rid in enumerate(ids):
+            st, img, info = srv.wait(rid, 30000.0)
+            assert st == capi.OK
+            sizes.append(info.batch_size)
+            assert np.array_equal(img, alone[k]), k
+        assert max(sizes) > 1  # requests did share launches
+    finally:
+        srv.destroy()
+
+
+def test_server_admission_and_injected_failure(device, cube1):
+    vals, org, spc, vols, oc = cube1
+    vb = vols[capi.DTYPE_F16].nbytes()
+    # budget: the volume plus room for ~1.5 small images
+    srv = capi.Server(device, memory_budget=vb + int(1.5 * 144 * 84 * 8), max_batch=8, max_inflight=2)
+    try:
+        idx = srv.add_volume(vols[capi.DTYPE_F16])
+        big = W.view(576, 336, 0.5, math=capi.MATH_FAST)
+        st, img, info = srv.wait(srv.submit(idx, big), 30000.0)
+        assert st == capi.ERR_ADMISSION and img is None
+        small = [W.view(144, 84, 0.5, pos=(float(k), -20.0, 10.0), math=capi.MATH_FAST) for k in range(6)]
+        ids = [srv.submit(idx, v) for v in small]
+        for rid in ids:
+            st, img, info = srv.wait(rid, 30000.0)
+            assert st == capi.OK and info.batch_size == 1  # only one fits at a time
+        device.inject_failure(1)
+        ids = [srv.submit(idx, v) for v in small[:3]]
+        res = [srv.wait(rid, 30000.0)[0] for rid in ids]
+        assert res[0] == capi.ERR_INJECTED and res[1:] == [capi.OK, capi.OK]
+        tiny = capi.Server(device, memory_budget=1024)
+        try:
+            with pytest.raises(capi.BirpError) as e:
+                tiny.add_volume(vols[capi.DTYPE_F16])
+            assert e.value.status == capi.ERR_ADMISSION
+        finally:
+            tiny.destroy()
+    finally:
+        srv.destroy()

@@ -1,0 +1,230 @@
+"""GPU parity of the exact render path (C ABI -> HIP kernels) against the oracle's FP32 restatement
+of the reference render (Camera.cpp:83-233 with DataCube.cpp:203-277 / Cmem.cpp:117-288).
+
+Bar: the FP32 exact path is bit-identical to the oracle for finite cubes (and always within the
+stated image tolerance, relative L2 <= 1e-5); the in-cube sample count equals the oracle's."""
+import os
+
+import numpy as np
+import pytest
+
+from birp_cpp_amd import capi
+from birp_cpp_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+NT = min(16, os.cpu_count() or 1)
+REL_TOL_FP32 = 1e-5  # image relative L2 error bound for the FP32 path
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    d = np.linalg.norm(b.ravel())
+    return float(np.linalg.norm((a - b).ravel()) / (d if d > 0 else 1.0))
+
+
+def oview(O, v):
+    return O.make_view(v.width, v.height, v.fov_w, v.fov_h, list(v.position), list(v.aim), v.ray_length, v.samples)
+
+
+def assert_bitwise(img, ref):
+    assert img.shape == ref.shape
+    a = img.view(np.uint32)
+    b = ref.view(np.uint32)
+    bad = np.flatnonzero(a != b)
+    assert bad.size == 0, f"{bad.size} pixels differ, first {bad[:5]}: gpu {img.ravel()[bad[:5]]} oracle {ref.ravel()[bad[:5]]}"
+
+
+@pytest.fixture(scope="module")
+def cfg1(device, oracle):
+    vals, org, spc, _ = W.cube_config1()
+    vol = capi.Volume.upload(device, vals, org, spc)
+    oc = oracle.OracleCube(vals, org, spc)
+    yield vol, oc
+    vol.free()
+
+
+@pytest.fixture(scope="module")
+def cfg2(device, oracle):
+    vals, org, spc, _ = W.cube_config2()
+    vol = capi.Volume.upload(device, vals, org, spc)
+    oc = oracle.OracleCube(vals, org, spc)
+    yield vol, oc
+    vol.free()
+
+
+# in-cube camera poses (x in [-10,20], y,z in [-15,15]) for the reference's early-stop semantics
+INCUBE_POSES = [((15.0, 5.0, 8.0), (0.0, 0.0, 0.0)), ((-5.0, 10.0, -12.0), (10.0, 0.0, 0.0)),
+                ((2.5, -7.5, 3.0), (12.0, 0.0, 0.0)), ((8.0, 0.5, -0.5), (-4.0, 0.0, 0.0))]
+
+
+def test_config1_extended_bitwise(cfg1, oracle):
+    vol, oc = cfg1
+    v = W.view_config1()
+    img, st = vol.render(v)
+    ref, cnt = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT)
+    print(f"config1: in-cube {st.samples_in_cube} oracle {cnt} nominal {st.samples_nominal} kernel {st.kernel_ms:.3f} ms")
+    assert st.samples_in_cube == cnt > 0
+    assert rel_l2(img, ref) <= REL_TOL_FP32
+    assert_bitwise(img, ref)
+
+
+@pytest.mark.parametrize("pose", INCUBE_POSES)
+@pytest.mark.parametrize("semantics", [0, 1])
+def test_incube_poses_bitwise(cfg1, oracle, pose, semantics):
+    vol, oc = cfg1
+    v = W.view(144, 84, 0.25, pos=pose[0], aim=pose[1], semantics=semantics, length=44.0)
+    img, st = vol.render(v)
+    ref, cnt = oracle.render_f32(oc, oview(oracle, v), semantics, nthreads=NT)
+    assert st.samples_in_cube == cnt > 0
+    assert_bitwise(img, ref)
+
+
+def test_reference_semantics_outside_camera_is_zero(cfg1, oracle):
+    vol, oc = cfg1
+    v = W.view(144, 84, 0.5, semantics=capi.SEM_REFERENCE)  # camera at y=-20 is outside the cube
+    img, st = vol.render(v)
+    ref, cnt = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_REFERENCE, nthreads=NT)
+    assert cnt == 0 and st.samples_in_cube == 0
+    assert not img.any() and not ref.any()
+
+
+def test_reference_fixed_ray_200_samples_L44(cfg1, oracle):
+    """Camera::Render's own ray grid: L = 44, n = 200 (Camera.cpp:31,65), pixel 0.25 deg, 36x36 deg."""
+    vol, oc = cfg1
+    W_, H_ = oracle.image_dims(0.25, 36.0, 36.0)
+    assert (W_, H_) == (144, 144)
+    v = capi.make_view(W_, H_, 36.0, 36.0, (12.0, 4.0, 6.0), (1.0, 0.0, 0.0), 44.0, 200, capi.SEM_REFERENCE)
+    img, _ = vol.render(v)
+    ref, _ = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_REFERENCE, nthreads=NT)
+    assert_bitwise(img, ref)
+
+
+def test_config2_full_size_bitwise(cfg2, oracle):
+    vol, oc = cfg2
+    v = W.view_config2()
+    img, st = vol.render(v)
+    ref, cnt = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT)
+    print(f"config2: in-cube {st.samples_in_cube} oracle {cnt} nominal {st.samples_nominal} kernel {st.kernel_ms:.3f} ms")
+    assert st.samples_in_cube == cnt > 0
+    assert rel_l2(img, ref) <= REL_TOL_FP32
+    assert_bitwise(img, ref)
+
+
+def test_accumulate_matches_repeated_render(cfg1, oracle):
+    """Camera::Render never clears the image (Camera.cpp:223): a second render adds onto the first."""
+    vol, oc = cfg1
+    v = W.view_config1()
+    first, _ = vol.render(v)
+    v2 = W.view_config1()
+    v2.flags = capi.RENDER_ACCUMULATE
+    second, _ = vol.render(v2, init=first)
+    ref1, _ = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT)
+    ref2, _ = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT, init=ref1)
+    assert_bitwise(second, ref2)
+
+
+def test_nearest_bitwise(device, oracle):
+    vals, org, spc, _ = W.cube_config1()
+    vol = capi.Volume.upload(device, vals, org, spc, interp=capi.INTERP_NEAREST)
+    try:
+        oc = oracle.OracleCube(vals, org, spc, trilinear=False)
+        v = W.view_config1()
+        img, st = vol.render(v)
+        ref, cnt = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT)
+        assert st.samples_in_cube == cnt
+        assert_bitwise(img, ref)
+    finally:
+        vol.free()
+
+
+@pytest.mark.parametrize("semantics", [0, 1])
+def test_signed_cube_with_zeros_bitwise(device, oracle, semantics):
+    """Zeros and negative values: std::lerp's straddle branch and the early stop on a negative
+    scaled sample (Camera.cpp:207-210)."""
+    rng = np.random.default_rng(7)
+    n = 40
+    vals = (rng.standard_normal((n, n, n)) * 1e-5 + 2e-6).astype(np.float32)
+    vals[rng.random(vals.shape) < 0.15] = 0.0
+    org, spc = (-10.0, -15.0, -15.0), (30.0 / (n - 1),) * 3
+    vol = capi.Volume.upload(device, vals, org, spc)
+    try:
+        oc = oracle.OracleCube(vals, org, spc)
+        for pose in INCUBE_POSES:
+            v = W.view(96, 64, 0.2, pos=pose[0], aim=pose[1], semantics=semantics, length=44.0)
+            img, st = vol.render(v)
+            ref, cnt = oracle.render_f32(oc, oview(oracle, v), semantics, nthreads=NT)
+            assert st.samples_in_cube == cnt
+            assert_bitwise(img, ref)
+    finally:
+        vol.free()
+
+
+def test_nonfinite_cube(device, oracle):
+    """inf/nan voxels take the [z][y][x] layout with floor/ceil corners: results match the oracle,
+    NaN where the oracle has NaN."""
+    vals, org, spc, _ = W.cube_config1()
+    vals = vals.copy()
+    vals[64, 60:70, 40:90] = np.nan
+    vals[30, 10, 10:20] = np.inf
+    vol = capi.Volume.upload(device, vals, org, spc)
+    try:
+        oc = oracle.OracleCube(vals, org, spc)
+        v = W.view_config1()
+        img, _ = vol.render(v)
+        ref, _ = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT)
+        assert np.array_equal(np.isnan(img), np.isnan(ref))
+        fin = ~np.isnan(ref)
+        assert np.isnan(ref).any() and fin.any()
+        assert np.array_equal(img[fin].view(np.uint32), ref[fin].view(np.uint32))
+    finally:
+        vol.free()
+
+
+def test_get_sample_points_bitwise(cfg1, oracle):
+    """Space::GetSample at arbitrary points (DataCube.cpp:203-277), incl. out-of-bounds (-1)."""
+    vol, oc = cfg1
+    rng = np.random.default_rng(11)
+    pts = np.column_stack([rng.uniform(-12, 22, 20000), rng.uniform(-17, 17, 20000), rng.uniform(-17, 17, 20000)]).astype(np.float32)
+    # grid points, the upper boundary plane and corners
+    _, org, spc, _ = (None, oc.c.origin, oc.c.spacing, None)
+    edge = np.array([[org[0], org[1], org[2]], [org[0] + 127 * spc[0], 0, 0], [org[0] + 126.5 * spc[0], 0.0, 0.0],
+                     [org[0] + 3 * spc[0], org[1] + 5 * spc[1], org[2] + 7 * spc[2]]], np.float32)
+    pts = np.vstack([pts, edge])
+    got = vol.sample(pts)
+    ref = np.array([oc.sample(*map(float, p)) for p in pts], np.float32)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    assert (ref == -1).any() and (ref > 0).any()
+
+
+def test_count_samples_matches_render(cfg2):
+    vol, _ = cfg2
+    v = W.view_config2()
+    _, st = vol.render(v)
+    assert vol.count_samples(v) == st.samples_in_cube
+
+
+def test_singular_and_invalid_views(cfg1):
+    vol, _ = cfg1
+    v = W.view(16, 16, 0.5, pos=(10.0, 0.0, 0.0), aim=(10.0, 0.0, 0.0))  # camera at the aim point
+    with pytest.raises(capi.BirpError):
+        vol.render(v)
+    bad = W.view(16, 16, 0.5)
+    bad.samples = 1
+    with pytest.raises(capi.BirpError) as e:
+        vol.render(bad)
+    assert e.value.status == capi.ERR_INVALID_ARGUMENT
+
+
+def test_render_async_into_device_image(device, cfg1, oracle):
+    vol, oc = cfg1
+    v = W.view_config1()
+    img = capi.Image(device, v.width, v.height)
+    try:
+        capi.render_async(vol, v, img)
+        got = img.download()
+    finally:
+        img.free()
+    ref, _ = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT)
+    assert_bitwise(got, ref)
