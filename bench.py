@@ -1,0 +1,292 @@
+"""Benchmark of the MI355X BIRP renderer (one JSON line on stdout).
+
+Headline (`value`): in-cube trilinear samples per second of the plain exact FP32 path on the
+256^3 shell cube at 576x336, 800 samples per ray (step 0.1 RE over 80 RE): K back-to-back renders
+through the C ABI with the cube resident in HBM, between two device synchronisations.
+
+Also measured in the same run (keys under "forms"):
+  plain_renders_per_s   the drop-in Camera::Render contract: launch, wait, copy the image to host
+  batched               256 orbit poses, mixed 144x84/288x168/576x336 images, steps 0.1/0.25 RE,
+                        F16 storage, one persistent-kernel launch replayed from a HIP graph; plain
+                        renders/s of the same 256 views for the ratio
+  precision             512^3 cube stored FP32/BF16/F16 (per-brick scale), 576x336, step 0.05 RE:
+                        relative L2 and max relative error of every image against the FP64 render
+  serving               Poisson requests at 200/s over 8 resident cubes: p50/p99 latency, renders/s
+The CPU baseline is the oracle's FP32 restatement of the reference render (a port: the reference's
+own Camera.cpp cannot be compiled here) on the same 576x336 image, on the host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "Gsamples/s, renders/s vs plain GPU at 1e-4 rel err; p99 latency"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def max_rel(a, b, floor_frac=1e-3):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    m = np.abs(b) > floor_frac * np.abs(b).max()
+    return float(np.max(np.abs(a[m] - b[m]) / np.abs(b[m]))) if m.any() else 0.0
+
+
+def read_traffic():
+    """HBM bytes per launch of the plain kernel from the committed rocprofv3 --pmc summary."""
+    p = os.path.join(ROOT, "profiles", "plain_exact_config2_pmc.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def bench_plain(capi, W, dev, steps, warmup):
+    vals, org, spc, _ = W.cube_config2()
+    vol = capi.Volume.upload(dev, vals, org, spc)
+    v = W.view_config2()
+    img = capi.Image(dev, v.width, v.height)
+    host, st = vol.render(v)
+    n_in = st.samples_in_cube
+    for _ in range(warmup):
+        capi.render_async(vol, v, img)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    dev.timer_begin()
+    for _ in range(steps):
+        capi.render_async(vol, v, img)
+    dev_ms = dev.timer_end()  # waits for the last render
+    wall = time.perf_counter() - t0
+    # the drop-in contract: one launch + wait + host copy per image
+    k2 = max(10, steps // 2)
+    t1 = time.perf_counter()
+    for _ in range(k2):
+        vol.render(v)
+    sync_rps = k2 / (time.perf_counter() - t1)
+    img.free()
+    return dict(vol=vol, vals=vals, org=org, spc=spc, view=v, n_in=n_in, wall=wall, dev_ms=dev_ms, sync_rps=sync_rps,
+                image=host)
+
+
+def cpu_baseline(O, W, capi, vals, org, spc, v, n_in):
+    nthreads = min(16, os.cpu_count() or 1)
+    oc = O.OracleCube(vals, org, spc)
+    ov = O.make_view(v.width, v.height, v.fov_w, v.fov_h, list(v.position), list(v.aim), v.ray_length, v.samples)
+    t = time.perf_counter()
+    _, cnt = O.render_f32(oc, ov, O.SEM_EXTENDED, nthreads=nthreads)
+    el = time.perf_counter() - t
+    assert cnt == n_in
+    return {"value": round(cnt / el / 1e9, 6), "unit": "Gsamples/s", "cores": nthreads, "kind": "port",
+            "sample": f"one full 576x336 render (n=800, {cnt} in-cube samples) of the 256^3 cube by the oracle's FP32 "
+                      f"restatement of Camera::Render, {nthreads} threads, {el:.2f} s"}
+
+
+def bench_batched(capi, W, dev, vals, org, spc, fp32_vol):
+    vol16 = capi.Volume.upload(dev, vals, org, spc, dtype=capi.DTYPE_F16)
+    views = W.views_config3(256)
+    tune = os.path.join(ROOT, "birp_cpp_amd", "tuning.jsonl")
+    if os.path.exists(tune):
+        capi.lib().birp_autotune_load(tune.encode())
+    b = capi.Batch(vol16, views)
+    for _ in range(2):
+        b.launch()
+        b.wait()
+    reps = 10
+    t = time.perf_counter()
+    for _ in range(reps):
+        b.launch()
+        b.wait()
+    el = (time.perf_counter() - t) / reps
+    nominal, last_ms = b.stats()
+    # plain path on the same 256 views: one exact render, wait and host copy each
+    ex = [capi.make_view(x.width, x.height, x.fov_w, x.fov_h, list(x.position), list(x.aim), x.ray_length, x.samples,
+                         capi.SEM_EXTENDED, capi.MATH_EXACT) for x in views]
+    for x in ex[:8]:
+        fp32_vol.render(x)
+    t = time.perf_counter()
+    for x in ex:
+        fp32_vol.render(x)
+    plain_el = time.perf_counter() - t
+    # accuracy of the batch against the FP64 render (a sample of images)
+    errs = [rel_l2(b.image(k), fp32_vol.render_f64(views[k])) for k in range(0, 256, 16)]
+    b.free()
+    vol16.free()
+    return {"poses": 256, "storage": "f16", "renders_per_s": round(256 / el, 1), "ms_per_batch": round(el * 1e3, 3),
+            "device_ms_per_batch": round(last_ms, 3), "nominal_gsamples_per_s": round(nominal / el / 1e9, 2),
+            "plain_renders_per_s_same_views": round(256 / plain_el, 1), "speedup_vs_plain": round(plain_el / el, 2),
+            "max_rel_l2_vs_fp64": float(f"{max(errs):.3e}")}
+
+
+def bench_precision(capi, W, dev, n=512, count=64):
+    vals, org, spc, _ = W.cube_config4(n)
+    vols = {name: capi.Volume.upload(dev, vals, org, spc, dtype=dt)
+            for name, dt in (("fp32", capi.DTYPE_F32), ("bf16", capi.DTYPE_BF16), ("f16", capi.DTYPE_F16))}
+    del vals
+    views = W.views_config4(count)
+    out = {"cube": f"{n}^3", "poses": count, "image": "576x336", "samples_per_ray": views[0].samples}
+    errs = {k: [] for k in ("exact_fp32", "fast_fp32", "bf16", "f16")}
+    mx = {k: [] for k in errs}
+    for v in views:
+        ref = vols["fp32"].render_f64(v)
+        if not ref.any():
+            continue
+        ex = capi.make_view(v.width, v.height, v.fov_w, v.fov_h, list(v.position), list(v.aim), v.ray_length, v.samples,
+                            capi.SEM_EXTENDED, capi.MATH_EXACT)
+        for key, vol, view in (("exact_fp32", vols["fp32"], ex), ("fast_fp32", vols["fp32"], v), ("bf16", vols["bf16"], v),
+                               ("f16", vols["f16"], v)):
+            img, _ = vol.render(view)
+            errs[key].append(rel_l2(img, ref))
+            mx[key].append(max_rel(img, ref))
+    for k in errs:
+        out[k] = {"rel_l2_max": float(f"{max(errs[k]):.3e}"), "rel_l2_mean": float(f"{np.mean(errs[k]):.3e}"),
+                  "max_rel_err": float(f"{max(mx[k]):.3e}")}
+    # batched renders/s of the 64 poses per storage type
+    for name in ("fp32", "bf16", "f16"):
+        b = capi.Batch(vols[name], views)
+        b.launch()
+        b.wait()
+        t = time.perf_counter()
+        for _ in range(3):
+            b.launch()
+            b.wait()
+        el = (time.perf_counter() - t) / 3
+        out[name + "_batched_renders_per_s"] = round(count / el, 1)
+        b.free()
+    out["bytes"] = {k: v.nbytes() for k, v in vols.items()}
+    for v in vols.values():
+        v.free()
+    return out
+
+
+def bench_serving(capi, W, dev, num=2000, rate=200.0):
+    cubes = []
+    for k, (n, dt) in enumerate(W.cubes_config5()):
+        vals, org, spc, _ = capi.synth_shell(n, (7.0, 12.0), 0.1, seed=500 + k)
+        cubes.append(capi.Volume.upload(dev, vals, org, spc, dtype=dt))
+        del vals
+    budget = sum(c.nbytes() for c in cubes) + (256 << 20)
+    srv = capi.Server(dev, memory_budget=budget, max_batch=64, max_inflight=3, batch_window_ms=0.2)
+    idx = [srv.add_volume(c) for c in cubes]
+    t_arr, which, views = W.requests_config5(num, rate)
+    ids = []
+    start = capi.clock_s() + 0.05
+    for k in range(num):
+        target = start + t_arr[k]
+        while capi.clock_s() < target:
+            pass
+        ids.append(srv.submit(idx[which[k]], views[k]))
+    lat, done_max, fails, bsz = [], 0.0, 0, []
+    for rid in ids:
+        st, _, info = srv.wait(rid, 60000.0)
+        if st != capi.OK:
+            fails += 1
+            continue
+        lat.append(info.done_s - info.submit_s)
+        done_max = max(done_max, info.done_s)
+        bsz.append(info.batch_size)
+    srv.destroy()
+    for c in cubes:
+        c.free()
+    lat = np.array(lat) * 1e3
+    span = done_max - start
+    return {"requests": num, "rate_hz": rate, "failed": fails, "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3), "renders_per_s": round(len(lat) / span, 1),
+            "mean_batch": round(float(np.mean(bsz)), 2), "budget_bytes": budget}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--forms", default="batched,precision,serving", help="extra forms to measure ('' for none)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1 GiB device copy (torch) to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 1 << 28  # 1 GiB of float32
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 3}))
+        return
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+
+    from birp_cpp_amd import capi
+    from birp_cpp_amd import workloads as W
+
+    dev = capi.Device(0)
+    log("plain: config 2 exact renders")
+    p = bench_plain(capi, W, dev, args.steps, args.warmup)
+    v, n_in = p["view"], p["n_in"]
+    ms_step = p["wall"] * 1e3 / args.steps
+    kern_ms = p["dev_ms"] / args.steps
+    alg_bytes = 32.0 * n_in + 4.0 * v.width * v.height
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    traffic = read_traffic()
+    line = {
+        "metric": METRIC, "value": round(n_in * args.steps / p["wall"] / 1e9, 3), "unit": "Gsamples/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "plain exact FP32 render, 256^3 shell cube (seed 2), 576x336 px, 800 samples/ray "
+                               "(step 0.1 RE, L 80 RE), camera (0,-20,10) aim (10,0,0), extended semantics",
+                   "in_cube_samples": n_in, "nominal_samples": v.width * v.height * v.samples},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "kernel_ms": round(kern_ms, 5), "algorithmic_bytes": int(alg_bytes)},
+        "forms": {"plain_renders_per_s": round(p["sync_rps"], 1),
+                  "plain_renders_per_s_async": round(args.steps / p["wall"], 1)},
+    }
+    if not args.no_cpu:
+        log("cpu baseline: oracle render")
+        from oracle import oracle as O
+        line["cpu_baseline"] = cpu_baseline(O, W, capi, p["vals"], p["org"], p["spc"], v, n_in)
+    else:
+        line["cpu_baseline"] = None
+    forms = [f for f in args.forms.split(",") if f]
+    if "batched" in forms:
+        log("batched: 256 orbit poses, F16")
+        line["forms"]["batched"] = bench_batched(capi, W, dev, p["vals"], p["org"], p["spc"], p["vol"])
+    p["vol"].free()
+    del p["vals"]
+    if "precision" in forms:
+        log("precision: 512^3 FP32/BF16/F16")
+        line["forms"]["precision"] = bench_precision(capi, W, dev)
+    if "serving" in forms:
+        log("serving: Poisson 200 req/s")
+        line["forms"]["serving"] = bench_serving(capi, W, dev)
+    dev.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

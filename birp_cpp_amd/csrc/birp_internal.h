@@ -13,11 +13,17 @@ namespace birp {
 //  quad:   FP32 "quad" records, one per cell (x,y,z) with x<nx-1, y<ny-1, z<nz:
 //          {v(x,y,z), v(x+1,y,z), v(x,y+1,z), v(x+1,y+1,z)} (16 B). Two records (z and z+1)
 //          hold a trilinear sample's eight corners. Records are grouped in bricks of 8x8 cells
-//          in x,y and 9 layers in z (the 9th is the next brick's first layer), bricks ordered
-//          [bz][by][bx], records inside a brick [lz][ly][lx].
+//          in x,y and 9 layers in z (the 9th is the next brick's first layer; 9216 B per brick),
+//          records inside a brick [lz][ly][lx].
 //  oct:    16-bit "octet" records, one per cell x<nx-1, y<ny-1, z<nz-1 holding all eight corners
 //          (16 B: the four z-layer corners in quad order, then the four at z+1), bricks of 8^3
-//          cells ordered [bz][by][bx]; F16 values are divided by a power-of-two per-brick scale.
+//          cells (8192 B); F16 values are divided by a power-of-two per-brick scale.
+//  Bricks are placed by a code that is a sum of one term per axis: the Morton code of
+//  (bx,by,bz) when the Morton index space wastes at most 2x, else bz*nby*nbx + by*nbx + bx.
+//  Since Morton codes of different axes occupy disjoint bits, both are sums, so a record's
+//  byte offset is tab_x[x] + tab_y[y] + tab_z[z] with one small table per axis:
+//    tab_a[i] = code_a(i >> 3) * brick_bytes + (i & 7) * {16, 128, 1024}[a].
+//  The kernels keep these tables in LDS: one add-chain instead of bit interleaving per sample.
 enum VolumeClass : int {
     VC_POSITIVE = 0,  // every value finite and > 0: the lerps never straddle zero
     VC_FINITE = 1,    // finite, any sign
@@ -48,7 +54,13 @@ struct DevVolume {
     const float* linear;  // may be null for 16-bit volumes
     const void* quad;     // float4 records
     const void* oct;      // uint4 records (bf16 or f16)
-    const float* bscale;  // per-brick scale (F16)
+    const float* bscale;  // per-brick scale (F16), indexed by byte offset >> 13
+    const unsigned* qtab; // quad offset tables [nx | ny | nz] (bytes)
+    const unsigned* otab; // oct offset tables [nx | ny | nz] (bytes)
+    unsigned quad_bytes;  // buffer range of the quad records (hardware bounds check)
+    unsigned oct_bytes;
+    int origin_far;       // every |origin| >= 2^-30: a nonzero ingress is never below 2^-56
+    float vmin, vmax;     // value range (positive volumes)
     CmemParams cmem;
 };
 
@@ -63,6 +75,8 @@ struct ExactView {
     int semantics;    // BIRP_SEM_*
     int accumulate;   // start from out's values
     float margin;     // FP64 slab expansion (RE) that covers FP32 position error
+    int fast_guards;  // positive cube whose scaled samples stay normal floats: only the
+                      // scale's midpoint guard is needed
     float* out;       // device image H*W
     unsigned long long* counter;  // optional: samples added
 };

@@ -19,6 +19,9 @@
 
 using namespace birp;
 
+#define BIRP_SCALE_K_HOST (1000 * 100 * 6378.1)
+#define BIRP_SCALE_C_HOST (4 / 3.14159265358979323846)
+
 namespace birp {
 
 static thread_local std::string g_last_error;
@@ -91,12 +94,19 @@ int exact_view(const birp_volume* vol, const birp_view* v, float* out_dev, Exact
                    v->ray_length + std::abs((double)d.ox) + std::abs((double)d.oy) + std::abs((double)d.oz) +
                    (double)d.nx * d.sx + (double)d.ny * d.sy + (double)d.nz * d.sz + 1.0;
     ev.margin = (float)(scale * 0x1p-16);
+    // positive cube whose scaled samples v*w*K stay normal floats: the scale needs only its
+    // midpoint guard (interpolated values stay within [vmin, vmax])
+    const double kf = (1.0 * BIRP_SCALE_K_HOST / BIRP_SCALE_C_HOST) / 10000;
+    ev.fast_guards = (d.vclass == VC_POSITIVE && (double)d.vmin * ev.w * kf >= 0x1p-124 && (double)d.vmax * ev.w * kf <= 0x1p126) ? 1 : 0;
     ev.out = out_dev;
     ev.counter = nullptr;
     return BIRP_OK;
 }
 
 int fast_pose(const birp_volume* vol, const birp_view* v, int vol_index, float* out_dev, int tile_w, int tile_h, FastPose& fp) {
+    if (vol->dv.kind != SK_CUBE || (vol->desc.dtype == BIRP_DTYPE_F32 ? !vol->dv.quad : !vol->dv.oct) ||
+        vol->dv.nx + vol->dv.ny + vol->dv.nz > 2048)
+        return fail(BIRP_ERR_UNSUPPORTED, "fast renders need a bricked volume with nx+ny+nz <= 2048");
     Mat3d rot, rinv;
     if (!view_basis64(v->position, v->aim, rot, rinv)) return fail(BIRP_ERR_SINGULAR, "Determinant is 0");
     for (int r = 0; r < 3; r++)
@@ -142,6 +152,18 @@ int device_scratch(birp_device* dev, size_t bytes, void** out) {
     return BIRP_OK;
 }
 
+int pinned_staging(birp_device* dev, size_t bytes) {
+    if (dev->pinned_bytes < bytes) {
+        if (dev->pinned) (void)hipHostFree(dev->pinned);
+        dev->pinned = nullptr;
+        dev->pinned_bytes = 0;
+        hipError_t e = hipHostMalloc(&dev->pinned, bytes, hipHostMallocDefault);
+        if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(staging)");
+        dev->pinned_bytes = bytes;
+    }
+    return BIRP_OK;
+}
+
 bool take_injected_failure(birp_device* dev) {
     int c = dev->inject_failures.load();
     while (c > 0) {
@@ -150,13 +172,22 @@ bool take_injected_failure(birp_device* dev) {
     return false;
 }
 
+// Tuning knobs of the exact path, read once: BIRP_EXACT_LANES (lanes per ray: 1, 2, 4) and
+// BIRP_EXACT_LAYOUT ("linear" reads the [z][y][x] copy instead of the quad bricks).
 int default_exact_lanes() {
     static int lanes = [] {
         const char* e = getenv("BIRP_EXACT_LANES");
-        int v = e ? atoi(e) : 4;
-        return (v == 1 || v == 2 || v == 4) ? v : 4;
+        int v = e ? atoi(e) : 2;
+        return (v == 1 || v == 2 || v == 4) ? v : 2;
     }();
     return lanes;
+}
+bool exact_force_linear() {
+    static bool lin = [] {
+        const char* e = getenv("BIRP_EXACT_LAYOUT");
+        return e && strcmp(e, "linear") == 0;
+    }();
+    return lin;
 }
 
 }  // namespace birp
@@ -218,6 +249,7 @@ extern "C" int birp_device_close(birp_device* dev) {
     (void)hipStreamSynchronize(dev->stream);
     if (dev->scratch) (void)hipFree(dev->scratch);
     if (dev->scratch2) (void)hipFree(dev->scratch2);
+    if (dev->pinned) (void)hipHostFree(dev->pinned);
     if (dev->d_counter) (void)hipFree(dev->d_counter);
     (void)hipEventDestroy(dev->ev0);
     (void)hipEventDestroy(dev->ev1);
@@ -263,11 +295,54 @@ static void free_volume_buffers(birp_volume* v) {
     if (v->quad) (void)hipFree(v->quad);
     if (v->oct) (void)hipFree(v->oct);
     if (v->bscale) (void)hipFree(v->bscale);
+    if (v->qtab) (void)hipFree(v->qtab);
+    if (v->otab) (void)hipFree(v->otab);
     v->linear = nullptr;
     v->quad = nullptr;
     v->oct = nullptr;
     v->bscale = nullptr;
+    v->qtab = nullptr;
+    v->otab = nullptr;
 }
+
+namespace birp {
+// Spread the low 10 bits of b to every third bit (Morton interleave of one axis).
+static uint32_t spread3(uint32_t b) {
+    uint32_t x = b & 0x3FFu;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+// Per-axis byte-offset tables of a bricked record layout (see birp_internal.h): returns the
+// number of brick slots the layout spans, or 0 when the byte offsets would not fit 32 bits.
+static size_t brick_tables(int nx, int ny, int nz, unsigned brick_bytes, std::vector<unsigned>& tab, bool* morton) {
+    const int nbx = (nx - 1 + 7) / 8, nby = (ny - 1 + 7) / 8, nbz = (nz - 1 + 7) / 8;
+    int bits = 0;
+    while ((1 << bits) < std::max(nbx, std::max(nby, nbz))) bits++;
+    const double dense = (double)nbx * nby * nbz;
+    const double mspace = std::ldexp(1.0, 3 * bits);
+    *morton = bits <= 10 && mspace <= 2.0 * dense;
+    const double slots = *morton ? mspace : dense;
+    if (slots * brick_bytes >= 4294967296.0) return 0;
+    tab.assign((size_t)nx + ny + nz, 0u);
+    const unsigned sub[3] = {16u, 128u, 1024u};
+    const int dims[3] = {nx, ny, nz};
+    size_t base = 0;
+    for (int a = 0; a < 3; a++) {
+        for (int i = 0; i < dims[a]; i++) {
+            uint32_t b = (uint32_t)(i >> 3), code;
+            if (*morton) code = spread3(b) << a;
+            else code = a == 0 ? b : (a == 1 ? b * (uint32_t)nbx : b * (uint32_t)nbx * (uint32_t)nby);
+            tab[base + i] = code * brick_bytes + (uint32_t)(i & 7) * sub[a];
+        }
+        base += dims[a];
+    }
+    return (size_t)slots;
+}
+}  // namespace birp
 
 extern "C" int birp_volume_upload(birp_device* dev, const birp_volume_desc* desc, const float* values, birp_volume** out) {
     if (!dev || !desc || !values || !out) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
@@ -321,50 +396,74 @@ extern "C" int birp_volume_upload(birp_device* dev, const birp_volume_desc* desc
     d.nby = (ny - 1 + 7) / 8;
     d.nbz = (nz - 1 + 7) / 8;
 
+    d.origin_far = (std::abs(d.ox) >= 0x1p-30f && std::abs(d.oy) >= 0x1p-30f && std::abs(d.oz) >= 0x1p-30f) ? 1 : 0;
+
     hipError_t e;
     int rc = BIRP_OK;
-    unsigned flags = 0;
+    unsigned flags[3] = {0u, 0x7F7FFFFFu, 0u};
     unsigned* dflags = (unsigned*)dev->d_counter;
+    size_t quad_bytes = 0, oct_bytes = 0, slots = 0;
+    bool morton = false;
+    std::vector<unsigned> tab;
     if ((e = hipMalloc(&v->linear, nvox * sizeof(float))) != hipSuccess) { rc = hip_fail(e, "hipMalloc(cube)"); goto fail_out; }
     if ((e = hipMemcpyAsync(v->linear, values, nvox * sizeof(float), hipMemcpyHostToDevice, dev->stream)) != hipSuccess) {
         rc = hip_fail(e, "hipMemcpy(cube)");
         goto fail_out;
     }
     d.linear = v->linear;
-    if ((rc = launch_zero_f32((float*)dflags, 1, dev->stream)) != BIRP_OK) goto fail_out;
+    if ((e = hipMemcpyAsync(dflags, flags, sizeof(flags), hipMemcpyHostToDevice, dev->stream)) != hipSuccess) {
+        rc = hip_fail(e, "class scan init");
+        goto fail_out;
+    }
     if ((rc = launch_scan_class(v->linear, nvox, dflags, dev->stream)) != BIRP_OK) goto fail_out;
-    if ((e = hipMemcpyAsync(&flags, dflags, sizeof(unsigned), hipMemcpyDeviceToHost, dev->stream)) != hipSuccess ||
+    if ((e = hipMemcpyAsync(flags, dflags, sizeof(flags), hipMemcpyDeviceToHost, dev->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(dev->stream)) != hipSuccess) {
         rc = hip_fail(e, "class scan");
         goto fail_out;
     }
-    d.vclass = (flags & 2u) ? VC_NONFINITE : (flags & 1u) ? VC_FINITE : VC_POSITIVE;
-    {
-        const size_t nb = (size_t)d.nbx * d.nby * d.nbz;
-        if (desc->dtype == BIRP_DTYPE_F32) {
-            if (desc->interp == BIRP_INTERP_TRILINEAR && d.vclass != VC_NONFINITE) {
-                if ((e = hipMalloc(&v->quad, nb * 576u * 16u)) != hipSuccess) { rc = hip_fail(e, "hipMalloc(quad)"); goto fail_out; }
-                d.quad = v->quad;
-                if ((rc = launch_build_quad(d, v->quad, dev->stream)) != BIRP_OK) goto fail_out;
-            }
-        } else {
-            if (d.vclass == VC_NONFINITE) { rc = fail(BIRP_ERR_UNSUPPORTED, "16-bit storage needs finite values"); goto fail_out; }
-            if ((e = hipMalloc(&v->oct, nb * 512u * 16u)) != hipSuccess) { rc = hip_fail(e, "hipMalloc(oct)"); goto fail_out; }
-            if ((e = hipMalloc(&v->bscale, nb * sizeof(float))) != hipSuccess) { rc = hip_fail(e, "hipMalloc(scale)"); goto fail_out; }
-            d.oct = v->oct;
-            d.bscale = v->bscale;
-            if ((rc = launch_build_oct(d, v->oct, v->bscale, dev->stream)) != BIRP_OK) goto fail_out;
+    d.vclass = (flags[0] & 2u) ? VC_NONFINITE : (flags[0] & 1u) ? VC_FINITE : VC_POSITIVE;
+    memcpy(&d.vmin, &flags[1], 4);
+    memcpy(&d.vmax, &flags[2], 4);
+    if (desc->dtype == BIRP_DTYPE_F32) {
+        if (desc->interp == BIRP_INTERP_TRILINEAR && d.vclass != VC_NONFINITE &&
+            (slots = brick_tables(nx, ny, nz, 576u * 16u, tab, &morton)) != 0 && (size_t)nx + ny + nz <= 8192) {
+            quad_bytes = slots * 576u * 16u;
+            if ((e = hipMalloc(&v->qtab, tab.size() * 4)) != hipSuccess) { rc = hip_fail(e, "hipMalloc(tables)"); goto fail_out; }
+            if ((e = hipMemcpy(v->qtab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice)) != hipSuccess) { rc = hip_fail(e, "tables"); goto fail_out; }
+            if ((e = hipMalloc(&v->quad, quad_bytes)) != hipSuccess) { rc = hip_fail(e, "hipMalloc(quad)"); goto fail_out; }
+            if ((e = hipMemsetAsync(v->quad, 0, quad_bytes, dev->stream)) != hipSuccess) { rc = hip_fail(e, "memset(quad)"); goto fail_out; }
+            d.quad = v->quad;
+            d.qtab = v->qtab;
+            d.quad_bytes = (unsigned)quad_bytes;
+            if ((rc = launch_build_quad(d, v->quad, dev->stream)) != BIRP_OK) goto fail_out;
         }
-        if ((e = hipStreamSynchronize(dev->stream)) != hipSuccess) { rc = hip_fail(e, "layout build"); goto fail_out; }
-        if (desc->dtype != BIRP_DTYPE_F32) {
-            // 16-bit volumes keep only their 16-bit records on the device
-            (void)hipFree(v->linear);
-            v->linear = nullptr;
-            d.linear = nullptr;
+    } else {
+        if (d.vclass == VC_NONFINITE) { rc = fail(BIRP_ERR_UNSUPPORTED, "16-bit storage needs finite values"); goto fail_out; }
+        if ((slots = brick_tables(nx, ny, nz, 512u * 16u, tab, &morton)) == 0 || (size_t)nx + ny + nz > 2048) {
+            rc = fail(BIRP_ERR_UNSUPPORTED, "16-bit storage supports cubes with nx+ny+nz <= 2048");
+            goto fail_out;
         }
-        v->bytes = (v->linear ? nvox * 4 : 0) + (v->quad ? nb * 576u * 16u : 0) + (v->oct ? nb * 512u * 16u : 0) +
-                   (v->bscale ? nb * 4 : 0);
+        oct_bytes = slots * 512u * 16u;
+        if ((e = hipMalloc(&v->otab, tab.size() * 4)) != hipSuccess) { rc = hip_fail(e, "hipMalloc(tables)"); goto fail_out; }
+        if ((e = hipMemcpy(v->otab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice)) != hipSuccess) { rc = hip_fail(e, "tables"); goto fail_out; }
+        if ((e = hipMalloc(&v->oct, oct_bytes)) != hipSuccess) { rc = hip_fail(e, "hipMalloc(oct)"); goto fail_out; }
+        if ((e = hipMalloc(&v->bscale, slots * sizeof(float))) != hipSuccess) { rc = hip_fail(e, "hipMalloc(scale)"); goto fail_out; }
+        if ((e = hipMemsetAsync(v->oct, 0, oct_bytes, dev->stream)) != hipSuccess) { rc = hip_fail(e, "memset(oct)"); goto fail_out; }
+        d.oct = v->oct;
+        d.otab = v->otab;
+        d.oct_bytes = (unsigned)oct_bytes;
+        d.bscale = v->bscale;
+        if ((rc = launch_build_oct(d, v->oct, v->bscale, dev->stream)) != BIRP_OK) goto fail_out;
     }
+    if ((e = hipStreamSynchronize(dev->stream)) != hipSuccess) { rc = hip_fail(e, "layout build"); goto fail_out; }
+    if (desc->dtype != BIRP_DTYPE_F32) {
+        // 16-bit volumes keep only their 16-bit records on the device
+        (void)hipFree(v->linear);
+        v->linear = nullptr;
+        d.linear = nullptr;
+    }
+    v->morton = morton;
+    v->bytes = (v->linear ? nvox * 4 : 0) + quad_bytes + oct_bytes + (v->bscale ? slots * 4 : 0) + tab.size() * 4;
     *out = v;
     return BIRP_OK;
 fail_out:
@@ -453,7 +552,9 @@ static int render_exact_into(birp_volume* vol, const birp_view* view, float* dim
     if (rc) return rc;
     ev.counter = counter;
     if (take_injected_failure(dev)) return fail(BIRP_ERR_INJECTED, "injected failure");
-    rc = launch_exact(vol->dv, ev, default_exact_lanes(), dev->stream);
+    DevVolume dv = vol->dv;
+    if (exact_force_linear() && dv.vclass != VC_NONFINITE && dv.kind == SK_CUBE) dv.vclass = VC_NONFINITE;
+    rc = launch_exact(dv, ev, default_exact_lanes(), dev->stream);
     return rc ? fail(rc, "exact render launch failed") : BIRP_OK;
 }
 
@@ -484,11 +585,15 @@ extern "C" int birp_render(birp_volume* vol, const birp_view* view, float* image
         return rc;
     }
     (void)hipEventRecord(dev->ev1, dev->stream);
-    unsigned long long count = 0;
-    if ((e = hipMemcpyAsync(image, dimg, npix * 4, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess)
+    // device -> pinned staging -> caller: one DMA at full link speed, one host copy
+    if ((rc = pinned_staging(dev, npix * 4 + 64))) return rc;
+    unsigned long long* hcount = (unsigned long long*)((char*)dev->pinned + npix * 4 + 32);
+    if ((e = hipMemcpyAsync(dev->pinned, dimg, npix * 4, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess)
         return hip_fail(e, "hipMemcpy(image)");
-    if ((e = hipMemcpyAsync(&count, cnt, 8, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess) return hip_fail(e, "hipMemcpy(count)");
+    if ((e = hipMemcpyAsync(hcount, cnt, 8, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess) return hip_fail(e, "hipMemcpy(count)");
     if ((e = hipStreamSynchronize(dev->stream)) != hipSuccess) return hip_fail(e, "render");
+    memcpy(image, dev->pinned, npix * 4);
+    unsigned long long count = *hcount;
     if (stats) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, dev->ev0, dev->ev1);
@@ -712,7 +817,7 @@ extern "C" int birp_batch_create(birp_volume* vol, const birp_view* views, int c
     int rc;
     if ((e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking)) != hipSuccess) { rc = hip_fail(e, "stream"); goto bad; }
     if ((e = hipMalloc(&b->d_images, off * sizeof(float))) != hipSuccess) { rc = hip_fail(e, "hipMalloc(images)"); goto bad; }
-    if ((e = hipMalloc(&b->d_work, 64)) != hipSuccess) { rc = hip_fail(e, "hipMalloc(counter)"); goto bad; }
+    if ((e = hipMalloc(&b->d_work, 1024)) != hipSuccess) { rc = hip_fail(e, "hipMalloc(counter)"); goto bad; }
     (void)hipEventCreate(&b->ev0);
     (void)hipEventCreate(&b->ev1);
     choose_tuning(vol, b->views, b->tuning);

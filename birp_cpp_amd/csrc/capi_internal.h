@@ -21,6 +21,8 @@ struct birp_device {
     size_t scratch_bytes = 0;
     void* scratch2 = nullptr;          // single fast-render descriptors
     size_t scratch2_bytes = 0;
+    void* pinned = nullptr;            // pinned host staging for birp_render's image copy
+    size_t pinned_bytes = 0;
     std::atomic<int> inject_failures{0};
 };
 
@@ -32,6 +34,9 @@ struct birp_volume {
     void* quad = nullptr;
     void* oct = nullptr;
     float* bscale = nullptr;
+    unsigned* qtab = nullptr;
+    unsigned* otab = nullptr;
+    bool morton = false;
     size_t bytes = 0;
 };
 

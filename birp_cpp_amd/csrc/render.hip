@@ -13,8 +13,13 @@
 //
 // Wave mapping: a wave64 owns a tile of 64/S rays; S lanes ("sub-lanes") share one ray and take
 // its samples k = kb + sub, kb advancing by S each trip. In the exact kernels every sub-lane then
-// reads its quad-mates' samples through DPP quad_perm moves and all of them add the S samples in
-// k order, so the sequential FP32 sum of the reference is kept while S lanes sample in parallel.
+// reads its quad-mates' samples through DPP quad_perm operands and all of them add the S samples
+// in k order, so the sequential FP32 sum of the reference is kept while S lanes sample in
+// parallel. A masked-off sample contributes -0.0f, which leaves every partial sum unchanged.
+//
+// Memory safety: cube records are read with buffer loads whose descriptor carries the buffer's
+// byte size, so the hardware returns 0 for any offset past the end; indices are clamped before
+// the tables are read.
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
@@ -25,9 +30,10 @@
 
 namespace birp {
 
-#define BIRP_WAVE 64
 #define BIRP_BLOCK 256
 #define BIRP_MAX_EXACT_SAMPLES 8192
+#define BIRP_FAST_TAB 2048  // per-wave LDS table entries of the fast kernel (nx+ny+nz)
+#define BIRP_RSRC_WORD3 0x00020000
 
 // ------------------------------------------------------------------------------------------
 // small device helpers
@@ -54,13 +60,8 @@ __device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
 // lane M of its pair (S == 2).
 template <int S, int M>
 __device__ __forceinline__ int quad_bcast(int v) {
-    if constexpr (S == 4) {
-        constexpr int ctrl = M | (M << 2) | (M << 4) | (M << 6);
-        return __builtin_amdgcn_mov_dpp(v, ctrl, 0xF, 0xF, false);
-    } else {
-        constexpr int ctrl = M | (M << 2) | ((2 + M) << 4) | ((2 + M) << 6);
-        return __builtin_amdgcn_mov_dpp(v, ctrl, 0xF, 0xF, false);
-    }
+    constexpr int ctrl = (S == 4) ? (M | (M << 2) | (M << 4) | (M << 6)) : (M | (M << 2) | ((2 + M) << 4) | ((2 + M) << 6));
+    return __builtin_amdgcn_update_dpp(0, v, ctrl, 0xF, 0xF, true);
 }
 template <int S, int M>
 __device__ __forceinline__ float quad_bcast_f(float v) {
@@ -84,11 +85,24 @@ __device__ __forceinline__ float lerp_positive(float a, float b, float t) {
     return __builtin_amdgcn_fmed3f(x, a, b);
 }
 
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+// Clamp in the float domain to [0, hi], then convert (NaN -> 0 in hardware).
+__device__ __forceinline__ int to_index(float x, float hi) { return (int)__builtin_amdgcn_fmed3f(x, 0.0f, hi); }
 
-// Clamp in the float domain, then convert: defined for NaN and out-of-range inputs (NaN -> lo),
-// so every gather index is in [lo, hi] whatever a masked-off lane computed.
-__device__ __forceinline__ int to_index(float x, int hi) { return (int)fminf(fmaxf(x, 0.0f), (float)hi); }
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, BIRP_RSRC_WORD3);
+}
+__device__ __forceinline__ float4 load16(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+    float4 f;
+    __builtin_memcpy(&f, &v, 16);
+    return f;
+}
+__device__ __forceinline__ uint4 load16u(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+    uint4 f;
+    __builtin_memcpy(&f, &v, 16);
+    return f;
+}
 
 // ------------------------------------------------------------------------------------------
 // exact per-pixel ray (Camera.cpp:148-168), FP32 IEEE operations in the reference's order
@@ -115,8 +129,11 @@ struct ExactIdx {
     float xd, yd, zd;  // index
 };
 
-__device__ __forceinline__ ExactIdx exact_index(const DevVolume& v, float dx, float dy, float dz, float px,
-                                                float py, float pz, float t) {
+// GUARD: the residual-corrected division is proven exact for |ingress| >= 2^-64; below that
+// (possible only when an origin component is within 2^-30 of 0) the IEEE division is used.
+template <bool GUARD>
+__device__ __forceinline__ ExactIdx exact_index(const DevVolume& v, float dx, float dy, float dz, float px, float py,
+                                                float pz, float t) {
     ExactIdx r;
     float x = dx * t + px;
     float y = dy * t + py;
@@ -124,17 +141,18 @@ __device__ __forceinline__ ExactIdx exact_index(const DevVolume& v, float dx, fl
     r.xi = x - v.ox;
     r.yi = y - v.oy;
     r.zi = z - v.oz;
-    float m = fminf(fminf(fabsf(r.xi), fabsf(r.yi)), fabsf(r.zi));
-    if (__builtin_expect(m < 0x1p-64f && m != 0.0f, 0)) {
-        // Below the range where the residual-corrected division is proven exact.
-        r.xd = r.xi / v.sx;
-        r.yd = r.yi / v.sy;
-        r.zd = r.zi / v.sz;
-    } else {
-        r.xd = birp_div_rn(r.xi, v.sx, v.rsx);
-        r.yd = birp_div_rn(r.yi, v.sy, v.rsy);
-        r.zd = birp_div_rn(r.zi, v.sz, v.rsz);
+    if constexpr (GUARD) {
+        float m = fminf(fminf(fabsf(r.xi), fabsf(r.yi)), fabsf(r.zi));
+        if (__builtin_expect(m < 0x1p-64f && m != 0.0f, 0)) {
+            r.xd = r.xi / v.sx;
+            r.yd = r.yi / v.sy;
+            r.zd = r.zi / v.sz;
+            return r;
+        }
     }
+    r.xd = birp_div_rn(r.xi, v.sx, v.rsx);
+    r.yd = birp_div_rn(r.yi, v.sy, v.rsy);
+    r.zd = birp_div_rn(r.zi, v.sz, v.rsz);
     return r;
 }
 
@@ -145,36 +163,31 @@ __device__ __forceinline__ bool exact_inside(const DevVolume& v, const ExactIdx&
            (r.zd + 1 < v.fnz);
 }
 
-// Trilinear sample from quad records (finite cubes). Corner x1 = x0 + 1 always: for finite data
-// std::lerp at t == 0 returns a, so this equals DataCube's ceil() corner choice.
+// Trilinear sample from quad records (finite cubes) through the per-axis offset tables. Corner
+// x1 = x0 + 1 always: for finite data std::lerp at t == 0 returns a, so this equals DataCube's
+// ceil() corner choice.
+struct QuadCtx {
+    __amdgpu_buffer_rsrc_t rs;
+    const unsigned* tx;
+    const unsigned* ty;
+    const unsigned* tz;
+    float hx, hy, hz;  // n - 2 per axis
+};
+
 template <bool POS>
-__device__ __forceinline__ float sample_quad(const DevVolume& v, const ExactIdx& r) {
-    float x0f = floorf(r.xd), y0f = floorf(r.yd), z0f = floorf(r.zd);
-    float fx = r.xd - x0f, fy = r.yd - y0f, fz = r.zd - z0f;
-    int ix = to_index(x0f, v.nx - 2);
-    int iy = to_index(y0f, v.ny - 2);
-    int iz = to_index(z0f, v.nz - 2);
-    unsigned brick = ((unsigned)(iz >> 3) * (unsigned)v.nby + (unsigned)(iy >> 3)) * (unsigned)v.nbx + (unsigned)(ix >> 3);
-    unsigned local = (((unsigned)(iz & 7) * 8u + (unsigned)(iy & 7)) * 8u) + (unsigned)(ix & 7);
-    const float4* q = (const float4*)v.quad + ((size_t)brick * 576u + local);
-    float4 a = q[0];
-    float4 b = q[64];
-    float lx1, lx2, hx1, hx2, ly, hy;
+__device__ __forceinline__ float sample_quad(const QuadCtx& c, float xd, float yd, float zd) {
+    float x0f = floorf(xd), y0f = floorf(yd), z0f = floorf(zd);
+    float fx = xd - x0f, fy = yd - y0f, fz = zd - z0f;
+    unsigned off = c.tx[to_index(x0f, c.hx)] + c.ty[to_index(y0f, c.hy)] + c.tz[to_index(z0f, c.hz)];
+    float4 a = load16(c.rs, off);
+    float4 b = load16(c.rs, off + 1024u);
     if constexpr (POS) {
-        lx1 = lerp_positive(a.x, a.y, fx);
-        lx2 = lerp_positive(a.z, a.w, fx);
-        hx1 = lerp_positive(b.x, b.y, fx);
-        hx2 = lerp_positive(b.z, b.w, fx);
-        ly = lerp_positive(lx1, lx2, fy);
-        hy = lerp_positive(hx1, hx2, fy);
+        float ly = lerp_positive(lerp_positive(a.x, a.y, fx), lerp_positive(a.z, a.w, fx), fy);
+        float hy = lerp_positive(lerp_positive(b.x, b.y, fx), lerp_positive(b.z, b.w, fx), fy);
         return lerp_positive(ly, hy, fz);
     } else {
-        lx1 = lerp_general(a.x, a.y, fx);
-        lx2 = lerp_general(a.z, a.w, fx);
-        hx1 = lerp_general(b.x, b.y, fx);
-        hx2 = lerp_general(b.z, b.w, fx);
-        ly = lerp_general(lx1, lx2, fy);
-        hy = lerp_general(hx1, hx2, fy);
+        float ly = lerp_general(lerp_general(a.x, a.y, fx), lerp_general(a.z, a.w, fx), fy);
+        float hy = lerp_general(lerp_general(b.x, b.y, fx), lerp_general(b.z, b.w, fx), fy);
         return lerp_general(ly, hy, fz);
     }
 }
@@ -184,7 +197,7 @@ __device__ __forceinline__ float sample_quad(const DevVolume& v, const ExactIdx&
 __device__ __forceinline__ float sample_linear_tri(const DevVolume& v, const ExactIdx& r) {
     float x0f = floorf(r.xd), y0f = floorf(r.yd), z0f = floorf(r.zd);
     float fx = r.xd - x0f, fy = r.yd - y0f, fz = r.zd - z0f;
-    int x0 = to_index(x0f, v.nx - 2), y0 = to_index(y0f, v.ny - 2), z0 = to_index(z0f, v.nz - 2);
+    int x0 = to_index(x0f, v.fnx - 2), y0 = to_index(y0f, v.fny - 2), z0 = to_index(z0f, v.fnz - 2);
     int x1 = x0 + (fx > 0.0f ? 1 : 0), y1 = y0 + (fy > 0.0f ? 1 : 0), z1 = z0 + (fz > 0.0f ? 1 : 0);
     const float* L = v.linear;
     size_t sy = (size_t)v.nx, sz = (size_t)v.nx * (size_t)v.ny;
@@ -199,7 +212,7 @@ __device__ __forceinline__ float sample_linear_tri(const DevVolume& v, const Exa
 
 // Nearest: slices[(int)z][(int)y][(int)x] (DataCube.cpp:273), truncation.
 __device__ __forceinline__ float sample_linear_nearest(const DevVolume& v, const ExactIdx& r) {
-    int x = to_index(truncf(r.xd), v.nx - 2), y = to_index(truncf(r.yd), v.ny - 2), z = to_index(truncf(r.zd), v.nz - 2);
+    int x = to_index(truncf(r.xd), v.fnx - 2), y = to_index(truncf(r.yd), v.fny - 2), z = to_index(truncf(r.zd), v.fnz - 2);
     return v.linear[((size_t)z * v.ny + y) * v.nx + x];
 }
 
@@ -278,8 +291,8 @@ __device__ void exact_clip(const DevVolume& v, const ExactView& ev, const float*
     klo = fmin(fmax(klo, 0.0), (double)n);
     khi = fmin(fmax(khi, -1.0), (double)(n - 1));
     int a = (int)klo, b = (int)khi;
-    while (a <= b && !exact_inside(v, exact_index(v, dx, dy, dz, ev.px, ev.py, ev.pz, s_t[a]))) a++;
-    while (b >= a && !exact_inside(v, exact_index(v, dx, dy, dz, ev.px, ev.py, ev.pz, s_t[b]))) b--;
+    while (a <= b && !exact_inside(v, exact_index<true>(v, dx, dy, dz, ev.px, ev.py, ev.pz, s_t[a]))) a++;
+    while (b >= a && !exact_inside(v, exact_index<true>(v, dx, dy, dz, ev.px, ev.py, ev.pz, s_t[b]))) b--;
     kf = a;
     kl = b;
 }
@@ -290,37 +303,41 @@ template <int S> struct ExactTile {
     static constexpr int TH = (64 / S) / TW;
 };
 
+// SM_QUAD_POS: positive finite cube, |origin| >= 2^-30 on every axis and scaled samples that
+// stay normal floats, so the only guard left is the scale's midpoint test.
 enum SampleMode { SM_QUAD_POS = 0, SM_QUAD_GEN = 1, SM_LIN_TRI = 2, SM_LIN_NEAREST = 3, SM_CMEM = 4 };
-
-template <int MODE>
-__device__ __forceinline__ float exact_sample(const DevVolume& v, float dx, float dy, float dz, float px, float py,
-                                              float pz, float t) {
-    if constexpr (MODE == SM_CMEM) {
-        float x = dx * t + px, y = dy * t + py, z = dz * t + pz;
-        return cmem_sample(v.cmem, x, y, z);
-    } else {
-        ExactIdx r = exact_index(v, dx, dy, dz, px, py, pz, t);
-        if constexpr (MODE == SM_QUAD_POS) return sample_quad<true>(v, r);
-        else if constexpr (MODE == SM_QUAD_GEN) return sample_quad<false>(v, r);
-        else if constexpr (MODE == SM_LIN_TRI) return sample_linear_tri(v, r);
-        else return sample_linear_nearest(v, r);
-    }
-}
 
 // ------------------------------------------------------------------------------------------
 // the exact render kernel
 // ------------------------------------------------------------------------------------------
 template <int S, int MODE>
 __global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact(DevVolume v, ExactView ev, int wg_x, int nblocks) {
-    extern __shared__ float s_t[];
+    constexpr bool QUAD = (MODE == SM_QUAD_POS || MODE == SM_QUAD_GEN);
+    constexpr bool POS = (MODE == SM_QUAD_POS);
+    extern __shared__ unsigned s_mem[];
+    float* s_t = (float*)s_mem;
+    const int n = ev.n;
     // ray distances t_k = L * ((k + 0.5f) / n) (Camera.cpp:73), IEEE division as on the host
-    for (int k = threadIdx.x; k < ev.n; k += BIRP_BLOCK) s_t[k] = ev.L * (((float)k + 0.5f) / (float)ev.n);
+    for (int k = threadIdx.x; k < n; k += BIRP_BLOCK) s_t[k] = ev.L * (((float)k + 0.5f) / (float)n);
+    QuadCtx qc;
+    if constexpr (QUAD) {
+        unsigned* tab = s_mem + ((n + 3) & ~3);
+        const int ntab = v.nx + v.ny + v.nz;
+        for (int q = threadIdx.x; q < ntab; q += BIRP_BLOCK) tab[q] = v.qtab[q];
+        qc.rs = make_rsrc(v.quad, v.quad_bytes);
+        qc.tx = tab;
+        qc.ty = tab + v.nx;
+        qc.tz = tab + v.nx + v.ny;
+        qc.hx = v.fnx - 2.0f;
+        qc.hy = v.fny - 2.0f;
+        qc.hz = v.fnz - 2.0f;
+    }
     __syncthreads();
 
     // XCD-aware block order: consecutive dispatch slots go round-robin over the 8 XCDs, so give
     // each XCD a contiguous band of tiles (neighbouring rays share cube bricks in its L2).
-    int per_xcd = (nblocks + 7) / 8;
-    int lb = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    const int per_xcd = (nblocks + 7) / 8;
+    const int lb = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (lb >= nblocks) return;
 
     constexpr int TW = ExactTile<S>::TW, TH = ExactTile<S>::TH;
@@ -331,12 +348,12 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact(DevVolume v, ExactV
     const bool valid = (i < ev.W) && (j < ev.H);
 
     float dx = 0.f, dy = 0.f, dz = 1.f;
-    int kf = ev.n, kl = -1;
+    int kf = n, kl = -1;
     if (valid) {
         exact_ray_dir(ev, i, j, dx, dy, dz);
         if constexpr (MODE == SM_CMEM) {
             kf = 0;
-            kl = ev.n - 1;
+            kl = n - 1;
         } else {
             exact_clip(v, ev, s_t, dx, dy, dz, kf, kl);
             if (ev.semantics == BIRP_SEM_REFERENCE && kf != 0) kl = -1;  // first sample outside: ray dead
@@ -348,46 +365,64 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact(DevVolume v, ExactV
     if (valid && ev.accumulate) acc = ev.out[(size_t)j * ev.W + i];
     bool alive = true;
     unsigned added = 0;
-    constexpr bool POS = (MODE == SM_QUAD_POS);
     const float w = ev.w;
+    const float px = ev.px, py = ev.py, pz = ev.pz;
 
     for (int kb = kmin; kb <= kmax; kb += S) {
         const int k = kb + sub;
-        const float t = s_t[min(k, ev.n - 1)];
+        const float t = s_t[min(k, n - 1)];
         const bool act = (k >= kf) && (k <= kl);
-        const float val = exact_sample<MODE>(v, dx, dy, dz, ev.px, ev.py, ev.pz, t);
-        const float s = birp_scale_sample(val * w);  // Camera.cpp:201
-        if constexpr (S == 1) {
-            if constexpr (POS) {
-                acc = act ? acc + s : acc;
-                added += act ? 1u : 0u;
+        float val;
+        if constexpr (MODE == SM_CMEM) {
+            val = cmem_sample(v.cmem, dx * t + px, dy * t + py, dz * t + pz);
+        } else if constexpr (POS) {
+            ExactIdx r = exact_index<false>(v, dx, dy, dz, px, py, pz, t);
+            val = sample_quad<true>(qc, r.xd, r.yd, r.zd);
+        } else {
+            ExactIdx r = exact_index<true>(v, dx, dy, dz, px, py, pz, t);
+            if constexpr (MODE == SM_QUAD_GEN) val = sample_quad<false>(qc, r.xd, r.yd, r.zd);
+            else if constexpr (MODE == SM_LIN_TRI) val = sample_linear_tri(v, r);
+            else val = sample_linear_nearest(v, r);
+        }
+        const float vw = val * w;  // Camera.cpp:201
+        if constexpr (POS) {
+            double y = (double)vw * BIRP_SCALE_FOLDED;
+            const uint32_t m29 = (uint32_t)birp_f64_bits(y) & 0x1FFFFFFFu;
+            float s = ((m29 - 0x0FFFFFF0u) < 0x21u) ? birp_scale_chain(vw) : (float)y;
+            s = act ? s : -0.0f;
+            if constexpr (S == 1) {
+                acc = acc + s;
             } else {
+                acc = acc + quad_bcast_f<S, 0>(s);
+                acc = acc + quad_bcast_f<S, 1>(s);
+                if constexpr (S == 4) {
+                    acc = acc + quad_bcast_f<S, 2>(s);
+                    acc = acc + quad_bcast_f<S, 3>(s);
+                }
+            }
+        } else {
+            const float s = birp_scale_sample(vw);
+            if constexpr (S == 1) {
                 bool add = act && alive;
                 bool stop = add && (s < 0);  // Camera.cpp:207-210
                 acc = (add && !stop) ? acc + s : acc;
                 added += (add && !stop) ? 1u : 0u;
                 alive = alive && !stop;
-            }
-        } else {
-            // Gather the S samples of this trip in every lane of the ray, add them in k order.
-            float sv[S];
-            int av[S];
-            sv[0] = quad_bcast_f<S, 0>(s);
-            av[0] = quad_bcast<S, 0>(act ? 1 : 0);
-            sv[1] = quad_bcast_f<S, 1>(s);
-            av[1] = quad_bcast<S, 1>(act ? 1 : 0);
-            if constexpr (S == 4) {
-                sv[2] = quad_bcast_f<S, 2>(s);
-                av[2] = quad_bcast<S, 2>(act ? 1 : 0);
-                sv[3] = quad_bcast_f<S, 3>(s);
-                av[3] = quad_bcast<S, 3>(act ? 1 : 0);
-            }
+            } else {
+                float sv[S];
+                int av[S];
+                sv[0] = quad_bcast_f<S, 0>(s);
+                av[0] = quad_bcast<S, 0>(act ? 1 : 0);
+                sv[1] = quad_bcast_f<S, 1>(s);
+                av[1] = quad_bcast<S, 1>(act ? 1 : 0);
+                if constexpr (S == 4) {
+                    sv[2] = quad_bcast_f<S, 2>(s);
+                    av[2] = quad_bcast<S, 2>(act ? 1 : 0);
+                    sv[3] = quad_bcast_f<S, 3>(s);
+                    av[3] = quad_bcast<S, 3>(act ? 1 : 0);
+                }
 #pragma unroll
-            for (int mm = 0; mm < S; ++mm) {
-                if constexpr (POS) {
-                    acc = av[mm] ? acc + sv[mm] : acc;
-                    added += av[mm] ? 1u : 0u;
-                } else {
+                for (int mm = 0; mm < S; ++mm) {
                     bool add = av[mm] && alive;
                     bool stop = add && (sv[mm] < 0);
                     acc = (add && !stop) ? acc + sv[mm] : acc;
@@ -397,6 +432,7 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact(DevVolume v, ExactV
             }
         }
     }
+    if constexpr (POS) added = (kl >= kf) ? (unsigned)(kl - kf + 1) : 0u;
     if (valid && sub == 0) ev.out[(size_t)j * ev.W + i] = acc;
     if (ev.counter) {
         unsigned tot = wave_sum_u((valid && sub == 0) ? added : 0u);
@@ -519,37 +555,51 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_sample_points(DevVolume v, const
 // ------------------------------------------------------------------------------------------
 // layout builders and scans
 // ------------------------------------------------------------------------------------------
-// flags: bit 0 = some value <= 0, bit 1 = some value non-finite
+// flags[0]: bit 0 = some value <= 0, bit 1 = some value non-finite; flags[1], flags[2]: min and
+// max of the positive finite values as float bits (ordered as unsigned for positive floats).
 __global__ __launch_bounds__(BIRP_BLOCK) void k_scan_class(const float* vals, size_t n, unsigned* flags) {
-    unsigned f = 0;
+    unsigned f = 0, mn = 0x7F7FFFFFu, mx = 0u;
     for (size_t idx = (size_t)blockIdx.x * BIRP_BLOCK + threadIdx.x; idx < n; idx += (size_t)gridDim.x * BIRP_BLOCK) {
         float x = vals[idx];
         if (!(x > 0.f)) f |= 1u;
         if (!isfinite(x)) f |= 2u;
+        if (x > 0.f && isfinite(x)) {
+            unsigned u = __float_as_uint(x);
+            mn = min(mn, u);
+            mx = max(mx, u);
+        }
     }
-    for (int o = 32; o > 0; o >>= 1) f |= (unsigned)__shfl_xor((int)f, o, 64);
-    if (lane_id() == 0 && f) atomicOr(flags, f);
+    for (int o = 32; o > 0; o >>= 1) {
+        f |= (unsigned)__shfl_xor((int)f, o, 64);
+        mn = min(mn, (unsigned)__shfl_xor((int)mn, o, 64));
+        mx = max(mx, (unsigned)__shfl_xor((int)mx, o, 64));
+    }
+    if (lane_id() == 0) {
+        if (f) atomicOr(flags, f);
+        atomicMin(flags + 1, mn);
+        atomicMax(flags + 2, mx);
+    }
 }
 
-__global__ __launch_bounds__(BIRP_BLOCK) void k_build_quad(DevVolume v, float4* quad) {
-    // one thread per record slot (including padding inside edge bricks)
-    size_t nrec = (size_t)v.nbx * v.nby * v.nbz * 576u;
-    for (size_t r = (size_t)blockIdx.x * BIRP_BLOCK + threadIdx.x; r < nrec; r += (size_t)gridDim.x * BIRP_BLOCK) {
-        size_t brick = r / 576u;
-        int local = (int)(r % 576u);
-        int lx = local & 7, ly = (local >> 3) & 7, lz = local >> 6;  // lz in [0, 8]
-        int bx = (int)(brick % v.nbx), by = (int)((brick / v.nbx) % v.nby), bz = (int)(brick / ((size_t)v.nbx * v.nby));
-        int x = bx * 8 + lx, y = by * 8 + ly, z = bz * 8 + lz;
-        float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (x < v.nx - 1 && y < v.ny - 1 && z < v.nz) {
-            const float* L = v.linear;
-            size_t base = ((size_t)z * v.ny + y) * v.nx + x;
-            q.x = L[base];
-            q.y = L[base + 1];
-            q.z = L[base + v.nx];
-            q.w = L[base + v.nx + 1];
-        }
-        quad[r] = q;
+// Quad records: one thread per (x, y, brick z, layer 0..8) slot; tz[bz*8] is the brick's base.
+__global__ __launch_bounds__(BIRP_BLOCK) void k_build_quad(DevVolume v, unsigned char* quad) {
+    const unsigned* tx = v.qtab;
+    const unsigned* ty = v.qtab + v.nx;
+    const unsigned* tz = v.qtab + v.nx + v.ny;
+    size_t cx = v.nx - 1, cy = v.ny - 1;
+    size_t total = cx * cy * (size_t)v.nbz * 9u;
+    for (size_t r = (size_t)blockIdx.x * BIRP_BLOCK + threadIdx.x; r < total; r += (size_t)gridDim.x * BIRP_BLOCK) {
+        int x = (int)(r % cx);
+        int y = (int)((r / cx) % cy);
+        int zr = (int)(r / (cx * cy));
+        int bz = zr / 9, lz = zr % 9;
+        int z = bz * 8 + lz;
+        if (z >= v.nz) continue;
+        const float* L = v.linear;
+        size_t base = ((size_t)z * v.ny + y) * v.nx + x;
+        float4 q = make_float4(L[base], L[base + 1], L[base + v.nx], L[base + v.nx + 1]);
+        unsigned off = tx[x] + ty[y] + tz[bz * 8] + (unsigned)lz * 1024u;
+        *(float4*)(quad + off) = q;
     }
 }
 
@@ -562,7 +612,7 @@ __device__ __forceinline__ unsigned short f32_to_f16_bits(float x) {
     return *reinterpret_cast<unsigned short*>(&h);
 }
 
-// Per-brick power-of-two scale for F16: max|v| over the brick's corners maps to <= 2^15.
+// Per-brick power-of-two scale for F16: max|v| over the brick's corners maps to < 2^15.
 __global__ __launch_bounds__(BIRP_BLOCK) void k_brick_scale(DevVolume v, float* bscale) {
     int brick = blockIdx.x;
     int bx = brick % v.nbx, by = (brick / v.nbx) % v.nby, bz = brick / (v.nbx * v.nby);
@@ -584,41 +634,41 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_brick_scale(DevVolume v, float* 
             frexpf(m, &e);           // m = f * 2^e, f in [0.5, 1)
             s = ldexpf(1.f, e - 15); // m / s < 2^15
         }
-        bscale[brick] = s;
+        const unsigned* t = v.otab;
+        unsigned code = (t[bx * 8] + t[v.nx + by * 8] + t[v.nx + v.ny + bz * 8]) >> 13;
+        bscale[code] = s;
     }
 }
 
 template <bool F16>
-__global__ __launch_bounds__(BIRP_BLOCK) void k_build_oct(DevVolume v, uint4* oct, const float* bscale) {
-    size_t nrec = (size_t)v.nbx * v.nby * v.nbz * 512u;
-    for (size_t r = (size_t)blockIdx.x * BIRP_BLOCK + threadIdx.x; r < nrec; r += (size_t)gridDim.x * BIRP_BLOCK) {
-        size_t brick = r >> 9;
-        int local = (int)(r & 511u);
-        int lx = local & 7, ly = (local >> 3) & 7, lz = local >> 6;
-        int bx = (int)(brick % v.nbx), by = (int)((brick / v.nbx) % v.nby), bz = (int)(brick / ((size_t)v.nbx * v.nby));
-        int x = bx * 8 + lx, y = by * 8 + ly, z = bz * 8 + lz;
-        unsigned short h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (x < v.nx - 1 && y < v.ny - 1 && z < v.nz - 1) {
-            const float* L = v.linear;
-            size_t sy = v.nx, sz = (size_t)v.nx * v.ny;
-            size_t b0 = (size_t)z * sz + (size_t)y * sy + x;
-            float c[8] = {L[b0], L[b0 + 1], L[b0 + sy], L[b0 + sy + 1],
-                          L[b0 + sz], L[b0 + sz + 1], L[b0 + sz + sy], L[b0 + sz + sy + 1]};
-            float inv = F16 ? 1.f / bscale[brick] : 1.f;  // power of two: exact
+__global__ __launch_bounds__(BIRP_BLOCK) void k_build_oct(DevVolume v, unsigned char* oct, const float* bscale) {
+    const unsigned* tx = v.otab;
+    const unsigned* ty = v.otab + v.nx;
+    const unsigned* tz = v.otab + v.nx + v.ny;
+    size_t cx = v.nx - 1, cy = v.ny - 1, cz = v.nz - 1;
+    size_t total = cx * cy * cz;
+    for (size_t r = (size_t)blockIdx.x * BIRP_BLOCK + threadIdx.x; r < total; r += (size_t)gridDim.x * BIRP_BLOCK) {
+        int x = (int)(r % cx), y = (int)((r / cx) % cy), z = (int)(r / (cx * cy));
+        const float* L = v.linear;
+        size_t sy = v.nx, sz = (size_t)v.nx * v.ny;
+        size_t b0 = (size_t)z * sz + (size_t)y * sy + x;
+        float c[8] = {L[b0], L[b0 + 1], L[b0 + sy], L[b0 + sy + 1], L[b0 + sz], L[b0 + sz + 1], L[b0 + sz + sy], L[b0 + sz + sy + 1]};
+        unsigned off = tx[x] + ty[y] + tz[z];
+        float inv = F16 ? 1.f / bscale[off >> 13] : 1.f;  // power of two: exact
+        unsigned short h[8];
 #pragma unroll
-            for (int q = 0; q < 8; q++) h[q] = F16 ? f32_to_f16_bits(c[q] * inv) : f32_to_bf16_bits(c[q]);
-        }
+        for (int q = 0; q < 8; q++) h[q] = F16 ? f32_to_f16_bits(c[q] * inv) : f32_to_bf16_bits(c[q]);
         uint4 o;
         o.x = (unsigned)h[0] | ((unsigned)h[1] << 16);
         o.y = (unsigned)h[2] | ((unsigned)h[3] << 16);
         o.z = (unsigned)h[4] | ((unsigned)h[5] << 16);
         o.w = (unsigned)h[6] | ((unsigned)h[7] << 16);
-        oct[r] = o;
+        *(uint4*)(oct + off) = o;
     }
 }
 
 __global__ void k_reset_counter(unsigned* c) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < 256) c[threadIdx.x] = 0u;
 }
 
 __global__ __launch_bounds__(BIRP_BLOCK) void k_zero_f32(float* p, size_t n) {
@@ -635,23 +685,21 @@ __device__ __forceinline__ float f16hi(unsigned u) { return __half2float(__ushor
 
 __device__ __forceinline__ float flerp(float a, float b, float t) { return __builtin_fmaf(t, b - a, a); }
 
-// Fast trilinear sample at index-space point (xd, yd, zd), already inside [0, n-1).
+// Fast trilinear sample at index-space point (xd, yd, zd), already clamped into [0, n-1].
 template <int DT>
-__device__ __forceinline__ float fast_sample(const DevVolume& v, float xd, float yd, float zd) {
+__device__ __forceinline__ float fast_sample(const DevVolume& v, __amdgpu_buffer_rsrc_t rs, const unsigned* tab, float xd,
+                                             float yd, float zd) {
     float x0f = floorf(xd), y0f = floorf(yd), z0f = floorf(zd);
     float fx = xd - x0f, fy = yd - y0f, fz = zd - z0f;
-    int ix = to_index(x0f, v.nx - 2), iy = to_index(y0f, v.ny - 2), iz = to_index(z0f, v.nz - 2);
-    unsigned brick = ((unsigned)(iz >> 3) * (unsigned)v.nby + (unsigned)(iy >> 3)) * (unsigned)v.nbx + (unsigned)(ix >> 3);
+    int ix = to_index(x0f, v.fnx - 2.0f), iy = to_index(y0f, v.fny - 2.0f), iz = to_index(z0f, v.fnz - 2.0f);
+    unsigned off = tab[ix] + tab[v.nx + iy] + tab[v.nx + v.ny + iz];
     if constexpr (DT == BIRP_DTYPE_F32) {
-        unsigned local = (((unsigned)(iz & 7) * 8u + (unsigned)(iy & 7)) * 8u) + (unsigned)(ix & 7);
-        const float4* q = (const float4*)v.quad + ((size_t)brick * 576u + local);
-        float4 a = q[0], b = q[64];
+        float4 a = load16(rs, off), b = load16(rs, off + 1024u);
         float ly = flerp(flerp(a.x, a.y, fx), flerp(a.z, a.w, fx), fy);
         float hy = flerp(flerp(b.x, b.y, fx), flerp(b.z, b.w, fx), fy);
         return flerp(ly, hy, fz);
     } else {
-        unsigned local = (((unsigned)(iz & 7) * 8u + (unsigned)(iy & 7)) * 8u) + (unsigned)(ix & 7);
-        uint4 o = ((const uint4*)v.oct)[(size_t)brick * 512u + local];
+        uint4 o = load16u(rs, off);
         float c0, c1, c2, c3, c4, c5, c6, c7;
         if constexpr (DT == BIRP_DTYPE_BF16) {
             c0 = bf16lo(o.x); c1 = bf16hi(o.x); c2 = bf16lo(o.y); c3 = bf16hi(o.y);
@@ -663,7 +711,7 @@ __device__ __forceinline__ float fast_sample(const DevVolume& v, float xd, float
         float ly = flerp(flerp(c0, c1, fx), flerp(c2, c3, fx), fy);
         float hy = flerp(flerp(c4, c5, fx), flerp(c6, c7, fx), fy);
         float val = flerp(ly, hy, fz);
-        if constexpr (DT == BIRP_DTYPE_F16) val *= v.bscale[brick];
+        if constexpr (DT == BIRP_DTYPE_F16) val *= v.bscale[off >> 13];
         return val;
     }
 }
@@ -676,19 +724,33 @@ __device__ __forceinline__ void kahan_add(float& sum, float& c, float x) {
     sum = t;
 }
 
+// Work distribution: items (pose-major, tiles row-major) are split into 8 contiguous ranges, one
+// per XCD queue. Block b starts on queue b % 8 (dispatch places consecutive blocks on successive
+// XCDs, so a queue's tiles -- neighbouring rays of one image -- meet in one L2) and moves on to
+// the other queues when its own is drained. Every wave leaves after finding all 8 queues empty.
 template <int DT, int S, int U>
 __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __restrict__ vols, const FastPose* __restrict__ poses,
                                                             const int* __restrict__ prefix, int npose, int total,
-                                                            unsigned* counter, int tw_log2) {
+                                                            unsigned* counters, int tw_log2) {
+    __shared__ unsigned s_tab[BIRP_BLOCK / 64][BIRP_FAST_TAB];
     const int lane = lane_id();
+    const int wave = threadIdx.x >> 6;
+    unsigned* tab = s_tab[wave];
     const int ray = lane / S, sub = lane % S;
     const int tw = 1 << tw_log2;
     const double K = BIRP_SCALE_FOLDED;
-    while (true) {
+    int cached_vol = -1;
+    int queue = blockIdx.x & 7;
+    for (int tried = 0; tried < 8;) {
+        const int q_lo = (int)(((long long)total * queue) >> 3), q_hi = (int)(((long long)total * (queue + 1)) >> 3);
         int item = 0;
-        if (lane == 0) item = (int)atomicAdd(counter, 1u);
+        if (lane == 0) item = q_lo + (int)atomicAdd(&counters[queue * 32], 1u);
         item = __builtin_amdgcn_readfirstlane(item);
-        if (item >= total) break;
+        if (item >= q_hi) {
+            queue = (queue + 1) & 7;
+            tried++;
+            continue;
+        }
         // pose = last p with prefix[p] <= item
         int lo = 0, hi = npose - 1;
         while (lo < hi) {
@@ -698,6 +760,16 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
         const int pidx = __builtin_amdgcn_readfirstlane(lo);
         const FastPose& P = poses[pidx];
         const DevVolume& V = vols[P.vol];
+        if (P.vol != cached_vol) {
+            const unsigned* src = (DT == BIRP_DTYPE_F32) ? V.qtab : V.otab;
+            const int ntab = V.nx + V.ny + V.nz;
+            for (int q = lane; q < ntab; q += 64) tab[q] = src[q];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            cached_vol = P.vol;
+        }
+        const __amdgpu_buffer_rsrc_t rs = (DT == BIRP_DTYPE_F32) ? make_rsrc(V.quad, V.quad_bytes) : make_rsrc(V.oct, V.oct_bytes);
         const int tile = item - prefix[pidx];
         const int tx = tile % P.tiles_x, ty = tile / P.tiles_x;
         const int i = tx * tw + (ray & (tw - 1));
@@ -707,7 +779,7 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
         int kf = P.n, kl = -1;
         float ix0 = 0.f, iy0 = 0.f, iz0 = 0.f, dix = 0.f, diy = 0.f, diz = 0.f;
         if (valid) {
-            // FP64 ray (same formulas as the reference, in double)
+            // FP64 ray (the reference's formulas, in double)
             double sx = (2 * ((i + 0.5) / P.W) - 1) * P.tanx_aspect;
             double sy = (1 - 2 * ((j + 0.5) / P.H)) * P.tany;
             double nd = sqrt(sx * sx + sy * sy + 1.0);
@@ -746,7 +818,7 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
         const int cnt = (kl >= kf) ? (kl - kf + 1) : 0;
         const int my = (cnt > sub) ? (cnt - sub + S - 1) / S : 0;  // samples of this sub-lane
         const int trips = wave_max_i(my);
-        const float fmaxx = V.fnx - 1.0f, fmaxy = V.fny - 1.0f, fmaxz = V.fnz - 1.0f;
+        const float hx = V.fnx - 1.0f, hy = V.fny - 1.0f, hz = V.fnz - 1.0f;
         float sum = 0.f, comp = 0.f;
         for (int q = 0; q < trips; q += U) {
 #pragma unroll
@@ -755,10 +827,10 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
                 const bool act = js < cnt;
                 const float fj = (float)js;
                 // clamp into the cube so that every lane's gather stays in bounds
-                float xd = fminf(fmaxf(__builtin_fmaf(fj, dix, ix0), 0.f), fmaxx);
-                float yd = fminf(fmaxf(__builtin_fmaf(fj, diy, iy0), 0.f), fmaxy);
-                float zd = fminf(fmaxf(__builtin_fmaf(fj, diz, iz0), 0.f), fmaxz);
-                float val = fast_sample<DT>(V, xd, yd, zd);
+                float xd = __builtin_amdgcn_fmed3f(__builtin_fmaf(fj, dix, ix0), 0.f, hx);
+                float yd = __builtin_amdgcn_fmed3f(__builtin_fmaf(fj, diy, iy0), 0.f, hy);
+                float zd = __builtin_amdgcn_fmed3f(__builtin_fmaf(fj, diz, iz0), 0.f, hz);
+                float val = fast_sample<DT>(V, rs, tab, xd, yd, zd);
                 kahan_add(sum, comp, act ? val : 0.f);
             }
         }
@@ -788,13 +860,15 @@ static int launch_exact_s(const DevVolume& v, const ExactView& ev, hipStream_t s
     int wg_y = (ev.H + 2 * TH - 1) / (2 * TH);
     int nblocks = wg_x * wg_y;
     int grid = ((nblocks + 7) / 8) * 8;
-    size_t lds = (size_t)ev.n * sizeof(float);
     int mode;
     if (v.kind == SK_CMEM) mode = SM_CMEM;
     else if (v.interp == BIRP_INTERP_NEAREST) mode = SM_LIN_NEAREST;
-    else if (v.vclass == VC_POSITIVE) mode = SM_QUAD_POS;
-    else if (v.vclass == VC_FINITE) mode = SM_QUAD_GEN;
-    else mode = SM_LIN_TRI;
+    else if (v.vclass == VC_NONFINITE || !v.quad || !v.qtab) mode = SM_LIN_TRI;
+    else if (v.vclass == VC_POSITIVE && v.origin_far && ev.fast_guards) mode = SM_QUAD_POS;
+    else mode = SM_QUAD_GEN;
+    size_t lds = (size_t)((ev.n + 3) & ~3) * sizeof(float);
+    if (mode == SM_QUAD_POS || mode == SM_QUAD_GEN) lds += (size_t)(v.nx + v.ny + v.nz) * sizeof(unsigned);
+    if (lds > 64 * 1024) return BIRP_ERR_UNSUPPORTED;
     switch (mode) {
         case SM_QUAD_POS: hipLaunchKernelGGL((k_render_exact<S, SM_QUAD_POS>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks); break;
         case SM_QUAD_GEN: hipLaunchKernelGGL((k_render_exact<S, SM_QUAD_GEN>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks); break;
@@ -835,10 +909,10 @@ int launch_scan_class(const float* values, size_t n, unsigned* flags_dev, void* 
 }
 
 int launch_build_quad(const DevVolume& v, void* quad, void* stream) {
-    size_t nrec = (size_t)v.nbx * v.nby * v.nbz * 576u;
-    size_t blocks = (nrec + BIRP_BLOCK - 1) / BIRP_BLOCK;
+    size_t total = (size_t)(v.nx - 1) * (v.ny - 1) * v.nbz * 9u;
+    size_t blocks = (total + BIRP_BLOCK - 1) / BIRP_BLOCK;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(k_build_quad, dim3((unsigned)blocks), dim3(BIRP_BLOCK), 0, (hipStream_t)stream, v, (float4*)quad);
+    hipLaunchKernelGGL(k_build_quad, dim3((unsigned)blocks), dim3(BIRP_BLOCK), 0, (hipStream_t)stream, v, (unsigned char*)quad);
     return hip_status(hipGetLastError());
 }
 
@@ -849,18 +923,18 @@ int launch_build_oct(const DevVolume& v, void* oct, float* bscale, void* stream)
         hipLaunchKernelGGL(k_brick_scale, dim3(nb), dim3(BIRP_BLOCK), 0, st, v, bscale);
         if (hipGetLastError() != hipSuccess) return BIRP_ERR_DEVICE;
     }
-    size_t nrec = (size_t)nb * 512u;
-    size_t blocks = (nrec + BIRP_BLOCK - 1) / BIRP_BLOCK;
+    size_t total = (size_t)(v.nx - 1) * (v.ny - 1) * (v.nz - 1);
+    size_t blocks = (total + BIRP_BLOCK - 1) / BIRP_BLOCK;
     if (blocks > 65536) blocks = 65536;
     if (v.dtype == BIRP_DTYPE_F16)
-        hipLaunchKernelGGL((k_build_oct<true>), dim3((unsigned)blocks), dim3(BIRP_BLOCK), 0, st, v, (uint4*)oct, bscale);
+        hipLaunchKernelGGL((k_build_oct<true>), dim3((unsigned)blocks), dim3(BIRP_BLOCK), 0, st, v, (unsigned char*)oct, bscale);
     else
-        hipLaunchKernelGGL((k_build_oct<false>), dim3((unsigned)blocks), dim3(BIRP_BLOCK), 0, st, v, (uint4*)oct, bscale);
+        hipLaunchKernelGGL((k_build_oct<false>), dim3((unsigned)blocks), dim3(BIRP_BLOCK), 0, st, v, (unsigned char*)oct, bscale);
     return hip_status(hipGetLastError());
 }
 
 int launch_reset_counter(unsigned* counter, void* stream) {
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, (hipStream_t)stream, counter);
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(256), 0, (hipStream_t)stream, counter);
     return hip_status(hipGetLastError());
 }
 
@@ -894,12 +968,14 @@ static int launch_fast_d(int S, int unroll, int grid, hipStream_t st, const DevV
     return launch_fast_su<DT, 1>(unroll, grid, st, vols, poses, prefix, npose, total, counter, twl);
 }
 
+// counters: 256 unsigned (8 queues, 128 B apart), reset by launch_reset_counter before each launch.
 int launch_fast_batch(const DevVolume* vols_dev, int dtype, const FastPose* poses_dev, const int* tile_prefix_dev, int npose,
                       int total_tiles, unsigned* work_counter, int tile_w, int tile_h, int lanes_per_ray, int unroll,
                       int grid, void* stream) {
     int twl = 0;
     while ((1 << twl) < tile_w) twl++;
     if ((1 << twl) != tile_w || tile_w * tile_h * lanes_per_ray != 64) return BIRP_ERR_INVALID_ARGUMENT;
+    grid = ((grid + 7) / 8) * 8;
     hipStream_t st = (hipStream_t)stream;
     if (dtype == BIRP_DTYPE_F16)
         return launch_fast_d<BIRP_DTYPE_F16>(lanes_per_ray, unroll, grid, st, vols_dev, poses_dev, tile_prefix_dev, npose, total_tiles, work_counter, twl);

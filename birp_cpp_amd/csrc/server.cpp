@@ -143,9 +143,9 @@ int launch_slot(birp_server* S, Slot& s) {
     for (int g = 0; g < 3; g++) {
         if (!counts[g]) continue;
         int total_tiles = s.h_prefix[g * (nb + 1) + counts[g]];
-        if ((rc = launch_reset_counter(s.d_work + g * 16, s.stream))) return rc;
+        if ((rc = launch_reset_counter(s.d_work + g * 256, s.stream))) return rc;
         int grid = std::min(fast_grid_size(g, t.lanes_per_ray, t.unroll), (total_tiles + 3) / 4 + 1);
-        rc = launch_fast_batch(S->d_vols, g, s.d_poses + g * nb, s.d_prefix + g * (nb + 1), counts[g], total_tiles, s.d_work + g * 16,
+        rc = launch_fast_batch(S->d_vols, g, s.d_poses + g * nb, s.d_prefix + g * (nb + 1), counts[g], total_tiles, s.d_work + g * 256,
                                t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll, grid, s.stream);
         if (rc) return rc;
     }
@@ -275,7 +275,7 @@ extern "C" int birp_server_create(birp_device* dev, const birp_server_config* cf
         if ((e = hipMalloc(&s.d_poses, sizeof(FastPose) * 3 * nb)) != hipSuccess) { rc = hip_fail(e, "poses"); break; }
         if ((e = hipHostMalloc(&s.h_prefix, sizeof(int) * 3 * (nb + 1), hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
         if ((e = hipMalloc(&s.d_prefix, sizeof(int) * 3 * (nb + 1))) != hipSuccess) { rc = hip_fail(e, "prefix"); break; }
-        if ((e = hipMalloc(&s.d_work, sizeof(unsigned) * 48)) != hipSuccess) { rc = hip_fail(e, "counter"); break; }
+        if ((e = hipMalloc(&s.d_work, sizeof(unsigned) * 3 * 256)) != hipSuccess) { rc = hip_fail(e, "counter"); break; }
     }
     if (rc) {
         birp_server_destroy(S);

@@ -205,11 +205,19 @@ def test_count_samples_matches_render(cfg2):
     assert vol.count_samples(v) == st.samples_in_cube
 
 
-def test_singular_and_invalid_views(cfg1):
-    vol, _ = cfg1
-    v = W.view(16, 16, 0.5, pos=(10.0, 0.0, 0.0), aim=(10.0, 0.0, 0.0))  # camera at the aim point
-    with pytest.raises(capi.BirpError):
-        vol.render(v)
+def test_camera_at_aim_point_and_invalid_views(cfg1, oracle):
+    """A camera at its aim point: Orient's unit vector is 0/0, its Sun projection falls back to
+    xo = yo = 0 (Camera.cpp:499-504), so the roll is 270 deg and the basis stays finite; the
+    reference renders it, and so do we, identically."""
+    vol, oc = cfg1
+    v = W.view(16, 16, 0.5, pos=(10.0, 0.0, 0.0), aim=(10.0, 0.0, 0.0))
+    img, _ = vol.render(v)
+    ref, _ = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED)
+    assert_bitwise(img, ref)
+    nan_pose = W.view(16, 16, 0.5, pos=(float("nan"), 0.0, 0.0))
+    with pytest.raises(capi.BirpError) as e:
+        vol.render(nan_pose)
+    assert e.value.status == capi.ERR_INVALID_ARGUMENT
     bad = W.view(16, 16, 0.5)
     bad.samples = 1
     with pytest.raises(capi.BirpError) as e:
