@@ -235,6 +235,13 @@ def ray_dir(view: View, rinv, i, j):
     return d
 
 
+def ray_dir_f64(view: View, rinv, i, j):
+    r = np.ascontiguousarray(rinv, np.float64).reshape(9)
+    d = np.zeros(3, np.float64)
+    lib().or_ray_dir_f64(C.byref(view), _dptr(r), i, j, _dptr(d))
+    return d
+
+
 def scale_sample(v, w) -> float:
     return lib().or_scale_sample(v, w)
 

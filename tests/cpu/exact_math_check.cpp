@@ -1,6 +1,7 @@
 // Exhaustive CPU check of the exact-arithmetic shortcuts in birp_cpp_amd/csrc/exact_math.h.
 //   scale <lo_bits> <hi_bits>     every float bit pattern in [lo, hi): birp_scale_sample == chain
-//   div <b> <count>               every non-negative finite float a: birp_div_rn(a,b,RN(1/b)) == a/b
+//   div <b>                       every non-negative finite float a: birp_div_rn(a,b,RN(1/b)) == a/b
+//   divrange <b>                  the same for every float a in [2^-64, 2^64) (the kernels' range)
 // Prints "mismatches N checked M" and exits 0 when N == 0. Threads split the range.
 #include <cmath>
 #include <cstdint>
@@ -35,14 +36,16 @@ int main(int argc, char** argv) {
                 }
                 bad += b; checked += c;
             });
-    } else if (!strcmp(argv[1], "div")) {
+    } else if (!strcmp(argv[1], "div") || !strcmp(argv[1], "divrange")) {
         float bb = strtof(argv[2], 0);
         volatile float one = 1.0f;
         float rb = one / bb;
+        const bool ranged = !strcmp(argv[1], "divrange");
+        const uint64_t lo = ranged ? (uint64_t)(127 - 64) << 23 : 0, hi = ranged ? (uint64_t)(127 + 64) << 23 : 0x7F800000ull;
         for (unsigned t = 0; t < nt; t++)
             th.emplace_back([&, t] {
                 unsigned long long b = 0, c = 0;
-                for (uint64_t u = t; u < 0x7F800000ull; u += nt) {
+                for (uint64_t u = lo + t; u < hi; u += nt) {
                     float a = f_of((uint32_t)u);
                     volatile float q = a / bb;
                     float r = birp_div_rn(a, bb, rb);

@@ -1,0 +1,125 @@
+"""CPU tests of the C-ABI library (no GPU compute): it loads, it exports every entry point
+include/birp.h declares, its host-side pose math equals the oracle's bit for bit, its synthetic
+workloads are deterministic and have the stated geometry, and it reports the absence of a GPU
+as an error instead of crashing."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from birp_cpp_amd import capi
+from birp_cpp_amd import workloads as W
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "birp.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(birp_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = capi.lib()
+    names = declared_functions()
+    assert len(names) >= 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the ctypes signature table covers exactly the header
+    assert sorted(n for n, _, _ in capi.SIGNATURES) == names
+
+
+def test_version_and_strerror():
+    assert capi.lib().birp_version() >= 10000
+    assert capi.lib().birp_strerror(capi.ERR_SINGULAR) == b"Determinant is 0"
+    assert capi.lib().birp_strerror(0) == b"ok"
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is visible")
+def test_no_gpu_is_an_error_not_a_crash():
+    h = capi.C.c_void_p()
+    assert capi.lib().birp_device_open(0, capi.C.byref(h)) == capi.ERR_NO_DEVICE
+    with pytest.raises(capi.BirpError) as e:
+        capi.Device(0)
+    assert e.value.status == capi.ERR_NO_DEVICE
+
+
+def test_view_basis_matches_oracle_bitwise(oracle):
+    """The library's FP32 Orient/basis (Camera.cpp:85-137, 309-508) equals the oracle's."""
+    rng = np.random.default_rng(1)
+    for k in range(2000):
+        pos = rng.uniform(-30, 30, 3).astype(np.float32)
+        aim = (float(rng.uniform(-15, 25)), 0.0, 0.0)
+        if k % 50 == 0:
+            aim = (float(rng.uniform(-15, 25)), float(rng.uniform(-3, 3)), float(rng.uniform(-3, 3)))  # Orient uses all of aim
+        v = capi.make_view(144, 84, 26.5, 15.5, pos, aim, 80, 160)
+        st, a, rot, rinv = capi.view_basis(v)
+        ov = O.make_view(144, 84, 26.5, 15.5, list(map(float, pos)), list(aim), 80, 160)
+        rc, orot, orinv = O.view_basis(ov)
+        assert st == rc == 0
+        assert np.float32(a).tobytes() == np.float32(O.orient(pos, np.float32(aim))).tobytes()
+        assert rot.tobytes() == orot.tobytes()
+        assert rinv.tobytes() == orinv.tobytes()
+
+
+def test_view_basis_rejects_nonfinite_pose():
+    v = capi.make_view(16, 16, 26.5, 15.5, (float("nan"), 0, 0), (10, 0, 0), 80, 160)
+    st, *_ = capi.view_basis(v)
+    assert st == capi.ERR_INVALID_ARGUMENT
+
+
+def test_shell_generator_deterministic_and_shaped():
+    a, org, spc, r0 = capi.synth_shell(48, (8.0, 11.0), 0.05, seed=1)
+    b, *_ = capi.synth_shell(48, (8.0, 11.0), 0.05, seed=1)
+    c, *_ = capi.synth_shell(48, (8.0, 11.0), 0.05, seed=2)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    assert 8.0 <= r0 <= 11.0
+    assert org == (-10.0, -15.0, -15.0)
+    assert spc[0] == pytest.approx(30 / 47, rel=1e-6) and spc[1] == spc[2] == pytest.approx(30 / 47, rel=1e-6)
+    assert np.all(a > 0) and np.all(np.isfinite(a))
+    # shell vs background: the magnetopause at the subsolar point is r0; its value there is
+    # 1e-5 (r0/10)^-3, the background 1e-7 (times lognormal noise with sigma 0.05)
+    g = -10 + spc[0] * np.arange(48)
+    gy = -15 + spc[1] * np.arange(48)
+    iy = int(np.argmin(np.abs(gy)))
+    sub = a[iy, iy, :]  # the x axis (y = z ~ 0)
+    xin = [i for i in range(48) if r0 * 1.02 < g[i] < r0 * 1.28]
+    xout = [i for i in range(48) if g[i] > r0 * 1.35 or (0 < g[i] < r0 * 0.95)]
+    assert np.median(sub[xin]) > 30 * np.median(sub[xout])
+    assert np.median(sub[xout]) == pytest.approx(1e-7, rel=0.1)
+
+
+def test_shell_noise_is_lognormal():
+    a, *_ = capi.synth_shell(40, (100.0, 101.0), 0.1, seed=7)  # magnetopause beyond the cube: background only
+    z = np.log(a.astype(np.float64) / 1e-7)
+    assert abs(z.mean()) < 0.01 and z.std() == pytest.approx(0.1, rel=0.05)
+
+
+def test_orbit_geometry():
+    p = capi.synth_orbit(2000, seed=3).astype(np.float64)
+    r = np.linalg.norm(p, axis=1)
+    assert r.min() >= 1.8 - 1e-4 and r.max() <= 19.0 + 1e-4
+    assert r.min() < 2.5 and r.max() > 18.5
+    # the orbit plane contains the x axis and is tilted 70 degrees from the x-y plane
+    normal = np.array([0.0, -np.sin(np.radians(70)), np.cos(np.radians(70))])
+    assert np.abs(p @ normal).max() < 1e-4
+    assert np.array_equal(p, capi.synth_orbit(2000, seed=3).astype(np.float64))
+
+
+def test_workload_definitions():
+    v1, v2 = W.view_config1(), W.view_config2()
+    assert (v1.width, v1.height, v1.samples) == (144, 84, 160)
+    assert (v2.width, v2.height, v2.samples) == (576, 336, 800)
+    assert O.image_dims(0.1845, 15.5, 26.5) == (144, 84)  # the reference's FOV rule (Camera.cpp:38-39)
+    vs = W.views_config3(256)
+    assert len(vs) == 256 and {(v.width, v.height) for v in vs} == set(W.SIZES)
+    assert {v.samples for v in vs} == {800, 320}
+    assert all(v.math == capi.MATH_FAST for v in vs)
+    assert W.views_config4(4)[0].samples == 1600
+    cubes = W.cubes_config5()
+    assert len(cubes) == 8 and all(n in (128, 256, 384, 512) for n, _ in cubes)
+    t, which, views = W.requests_config5(1000)
+    rate = (len(t) - 1) / (t[-1] - t[0])
+    assert rate == pytest.approx(200, rel=0.15) and which.max() < 8

@@ -1,0 +1,329 @@
+"""Pins the oracle (oracle/birp_oracle.c) -- the CPU restatement every GPU parity test trusts.
+
+Each function of the oracle is checked against something that is not the oracle's own formula:
+  * the reference's own known answers (tests/CMEM_test.cpp:14-90 in the reference);
+  * golden vectors written by the reference's own DataCube/CMEM/Helper code compiled from its
+    sources (tests/golden/make_golden.py, oracle/_ref);
+  * libstdc++'s std::lerp compiled here;
+  * closed forms and invariants of the geometry (projected-Sun roll angle, optical axis,
+    rotation properties, the README's worked angles, constant and linear fields).
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+
+
+def f32(bits_list):
+    return np.asarray(bits_list, np.uint32).view(np.float32)
+
+
+@pytest.fixture(scope="module")
+def lib(oracle):
+    return oracle
+
+
+# ------------------------------------------------------------------------- the reference's own tests
+def test_cmem_known_answers_default(lib):
+    """tests/CMEM_test.cpp:14-45 (EXPECT_EQ on floats)."""
+    m = O.OracleCmem()
+    assert m.sample(8.0, 4.0, 4.0) == np.float32(0)
+    assert np.float32(m.sample(4.0, 12.0, 10.0)) == np.float32(4.46853346e-06)
+    assert m.sample(3.0, 1.5, 1.2) == np.float32(0)
+    for p in [(8.0, 10.0, 10.0), (8.0, -10.0, -10.0), (8.0, 10.0, -10.0)]:
+        assert np.float32(m.sample(*p)) == np.float32(8.15947317e-07)
+
+
+def test_cmem_known_answers_init_params(lib):
+    """tests/CMEM_test.cpp:48-90: Init(true, v, b, dipole, p1, p2, p3, B, alpha, beta) binds its
+    arguments one position off the signature (Cmem.h:21-39); the known answers pin that binding."""
+    m = O.OracleCmem(dict(v=[400, 0, 0], b=[0, 0, 4], dipole=0.0, p0=1, p1=4, p2=6, p3=3, B=3.6, alpha=-1.0))
+    assert m.sample(8.0, 4.0, 4.0) == np.float32(0)
+    assert np.float32(m.sample(12.0, 12.0, 10.0)) == np.float32(4.57952495e-07)
+    assert m.sample(3.0, 1.5, 1.2) == np.float32(0)
+    for p in [(12.0, 10.0, 10.0), (12.0, -10.0, -10.0), (12.0, 10.0, -10.0)]:
+        assert np.float32(m.sample(*p)) == np.float32(5.48567925e-07)
+
+
+def test_cmem_overwrites_B_and_p2_quirk(lib):
+    """CalcMagneticPressure stores |b|*1e-9 into B (Cmem.cpp:310); p2 takes p3 (Cmem.cpp:90)."""
+    st = O.OracleCmem().state()
+    assert st["B"] == pytest.approx(5e-9, rel=1e-6)
+    assert st["p2"] == st["p3"] == 4.0
+
+
+# ------------------------------------------------------------------------- golden vectors (reference code)
+def test_datacube_golden_bitwise(lib):
+    g = json.load(open(os.path.join(GOLDEN, "datacube_samples.json")))
+    vals = f32(g["values"]).reshape(g["nz"], g["ny"], g["nx"])
+    org, spc = f32(g["origin"]), f32(g["spacing"])
+    pts = f32(g["points"]).reshape(-1, 3)
+    assert g["size"] == [g["nx"], g["ny"], g["nz"]]
+    for key, tri in (("trilinear", True), ("nearest", False)):
+        oc = O.OracleCube(vals, list(map(float, org)), list(map(float, spc)), trilinear=tri)
+        got = np.array([oc.sample(*map(float, p)) for p in pts], np.float32)
+        want = f32(g[key])
+        bad = np.flatnonzero(got.view(np.uint32) != want.view(np.uint32))
+        assert bad.size == 0, f"{key}: {bad.size} samples differ, e.g. {pts[bad[:3]]} {got[bad[:3]]} {want[bad[:3]]}"
+        assert (want == -1).any() and (want != -1).any()
+
+
+def test_datacube_metadata_from_coordinates(lib):
+    """spacing = |c[0] - c[1]| and origin = c[0] per axis (DataCube.cpp:150-191), as loaded."""
+    g = json.load(open(os.path.join(GOLDEN, "datacube_samples.json")))
+    for axis, key in enumerate(("xs", "ys", "zs")):
+        c = f32(g[key])
+        assert f32(g["origin"])[axis] == c[0]
+        assert f32(g["spacing"])[axis] == np.float32(abs(c[0] - c[1]))
+
+
+def test_cmem_golden_bitwise(lib):
+    g = json.load(open(os.path.join(GOLDEN, "cmem_samples.json")))
+    pts = f32(g["points"]).reshape(-1, 3)
+    for s in g["sets"]:
+        m = O.OracleCmem(s["params"])
+        got = np.array([m.sample(*map(float, p)) for p in pts], np.float32)
+        want = f32(s["values"])
+        bad = np.flatnonzero(got.view(np.uint32) != want.view(np.uint32))
+        assert bad.size == 0, f"params {s['params']}: {bad.size} differ"
+
+
+def test_helper_matrix_golden_bitwise(lib):
+    g = json.load(open(os.path.join(GOLDEN, "helper_matrix.json")))
+    for p in g["products"]:
+        ab = O.matmul3(f32(p["rz2"]), f32(p["rx"]))
+        abc = O.matmul3(ab, f32(p["rz"]))
+        assert np.array_equal(ab.ravel().view(np.uint32), f32(p["ab"]).view(np.uint32))
+        assert np.array_equal(abc.ravel().view(np.uint32), f32(p["abc"]).view(np.uint32))
+        rc, inv = O.get_inverse3(abc)
+        assert rc == p["rc"] == 0
+        assert np.array_equal(inv.ravel().view(np.uint32), f32(p["inverse"]).view(np.uint32))
+    for p in g["general"]:
+        rc, inv = O.get_inverse3(f32(p["m"]))
+        assert rc == p["rc"]
+        if rc == 0:
+            assert np.array_equal(inv.ravel().view(np.uint32), f32(p["inverse"]).view(np.uint32))
+
+
+# ------------------------------------------------------------------------- std::lerp from libstdc++
+def test_lerp_matches_libstdcxx(tmp_path, lib):
+    src = tmp_path / "lerp.cpp"
+    src.write_text(
+        '#include <cmath>\n#include <cstdio>\n#include <cstdint>\n#include <cstring>\n'
+        'int main(){uint32_t a,b,t; while(scanf("%u %u %u",&a,&b,&t)==3){float fa,fb,ft; memcpy(&fa,&a,4);'
+        'memcpy(&fb,&b,4);memcpy(&ft,&t,4); float r=std::lerp(fa,fb,ft); uint32_t u; memcpy(&u,&r,4);'
+        'printf("%u\\n",u);} return 0;}\n')
+    exe = tmp_path / "lerp"
+    subprocess.run(["g++", "-O2", "-std=c++20", "-ffp-contract=off", str(src), "-o", str(exe)], check=True)
+    rng = np.random.default_rng(5)
+    n = 20000
+    a = rng.standard_normal(n).astype(np.float32)
+    b = rng.standard_normal(n).astype(np.float32)
+    t = rng.random(n).astype(np.float32)
+    a[::7] = 0
+    b[::11] = 0
+    b[::13] = a[::13]
+    t[::17] = 0
+    t[::19] = 1
+    t[::23] = rng.uniform(1, 2, len(t[::23])).astype(np.float32)
+    inp = "\n".join(f"{x} {y} {z}" for x, y, z in zip(a.view(np.uint32), b.view(np.uint32), t.view(np.uint32)))
+    out = subprocess.run([str(exe)], input=inp, capture_output=True, text=True, check=True).stdout.split()
+    want = np.array(out, np.uint64).astype(np.uint32)
+    got = np.array([O.lib().or_lerpf(float(x), float(y), float(z)) for x, y, z in zip(a, b, t)], np.float32)
+    assert np.array_equal(got.view(np.uint32), want)
+
+
+# ------------------------------------------------------------------------- geometry closed forms
+def _roll_closed_form(pos, aim):
+    """angle_sun from geometry: the Sun (23455,0,0) projected on the image plane, its signed angle
+    `ang` from the projected +z axis towards right = u x north; Orient returns (90 - ang) mod 360."""
+    P, A = np.asarray(pos, float), np.asarray(aim, float)
+    u = (A - P) / np.linalg.norm(A - P)
+    z = np.array([0.0, 0.0, 1.0])
+    north = z - u[2] * u
+    north /= np.linalg.norm(north)
+    right = np.cross(u, north)
+    s = np.array([23455.0, 0, 0]) - P
+    sp = s - s.dot(u) * u
+    ang = np.degrees(np.arctan2(sp.dot(right), sp.dot(north)))
+    return (90.0 - ang) % 360.0
+
+
+def test_orient_matches_projected_sun_closed_form(lib):
+    rng = np.random.default_rng(9)
+    checked = 0
+    for _ in range(400):
+        pos = rng.uniform(-25, 25, 3)
+        aim = (rng.uniform(-15, 25), 0.0, 0.0)
+        u = (np.asarray(aim) - pos) / np.linalg.norm(np.asarray(aim) - pos)
+        if abs(u[1]) > 0.999 * np.sqrt(1 - u[2] ** 2) or abs(u[2]) > 0.999:
+            continue  # Orient's degenerate branches (Camera.cpp:352-357) and a vertical view
+        want = _roll_closed_form(pos, aim)
+        d64 = (O.orient_f64(pos, aim) - want + 180) % 360 - 180
+        d32 = (O.orient(pos.astype(np.float32), np.float32(aim)) - want + 180) % 360 - 180
+        assert abs(d64) < 1e-7, (pos, aim)
+        assert abs(d32) < 2e-3, (pos, aim)
+        checked += 1
+    assert checked > 300
+
+
+def test_basis_optical_axis_and_rotation(lib):
+    """R^-1 (0,0,-1) = (A-P)/|A-P| for aim (a,0,0) whatever the roll; R is a rotation."""
+    rng = np.random.default_rng(10)
+    for _ in range(200):
+        pos = rng.uniform(-25, 25, 3).astype(np.float32)
+        aim = (float(rng.uniform(-15, 25)), 0.0, 0.0)
+        v = O.make_view(144, 84, 26.5, 15.5, list(map(float, pos)), list(aim), 80, 160)
+        rc, rot, rinv = O.view_basis_f64(v)
+        assert rc == 0
+        u = (np.asarray(aim) - pos.astype(float)) / np.linalg.norm(np.asarray(aim) - pos.astype(float))
+        assert np.allclose(rinv @ np.array([0, 0, -1.0]), u, atol=1e-12)
+        assert np.allclose(rot @ rot.T, np.eye(3), atol=1e-12)
+        assert np.linalg.det(rot) == pytest.approx(1.0, abs=1e-12)
+        rc, rot32, rinv32 = O.view_basis(v)
+        assert np.allclose(rinv32 @ rot32, np.eye(3), atol=2e-6)
+        assert np.allclose(rinv32 @ np.array([0, 0, -1.0]), u, atol=2e-6)
+
+
+def test_readme_worked_angles(lib):
+    """README.md:75-146: P = (5.9, 6.7, 17.7), aim 7.8 -> the x-rotation 0.37485342 rad and the
+    z-rotation magnitude atan(1.9/6.7) = 0.27632 rad (Camera.cpp:93 uses 2 pi - that)."""
+    v = O.make_view(144, 144, 36, 36, [5.9, 6.7, 17.7], [7.8, 0, 0], 44, 200)
+    rc, rot, _ = O.view_basis_f64(v)
+    # the third row of R = Rz2 Rx Rz is (sin phi sin theta, sin phi cos theta, cos phi)
+    phi = np.arccos(rot[2, 2])
+    theta = np.arctan2(rot[2, 0], rot[2, 1]) % (2 * np.pi)
+    assert phi == pytest.approx(0.37485342, abs=1e-7)
+    assert theta == pytest.approx(2 * np.pi - 0.27632, abs=1e-5)
+    rc, rot32, _ = O.view_basis(v)
+    assert np.arccos(np.clip(rot32[2, 2], -1, 1)) == pytest.approx(0.37485342, abs=2e-6)
+
+
+def test_pixel_rays_unit_and_mirror(lib):
+    v = O.make_view(144, 84, 26.5, 15.5, [0.0, -20.0, 10.0], [10.0, 0, 0], 80, 160)
+    _, rot, rinv = O.view_basis(v)
+    for i, j in [(0, 0), (143, 83), (71, 41), (10, 70)]:
+        d = O.ray_dir(v, rinv, i, j)
+        assert np.linalg.norm(d.astype(float)) == pytest.approx(1.0, abs=1e-6)
+        # camera frame: x = screen right with the aspect-scaled tangent (Camera.cpp:150, Q3)
+        c = rot.astype(float) @ d.astype(float)
+        tx = np.tan(np.radians(26.5 / 2)) * (144 / 84)
+        ty = np.tan(np.radians(15.5 / 2))
+        assert -c[0] / c[2] == pytest.approx(((2 * (i + 0.5) / 144) - 1) * tx, abs=1e-5)
+        assert -c[1] / c[2] == pytest.approx((1 - 2 * (j + 0.5) / 84) * ty, abs=1e-5)
+
+
+# ------------------------------------------------------------------------- renders: closed forms
+def _const_cube(value, n=64, lo=-60.0, hi=60.0):
+    vals = np.full((n, n, n), value, np.float32)
+    spc = (hi - lo) / (n - 1)
+    return O.OracleCube(vals, (lo, lo, lo), (spc, spc, spc))
+
+
+def test_cubefake_constant_field_closed_form(lib):
+    """tests/Camera_test.cpp:25-29's CubeFake returns 1 everywhere; a constant cube that holds every
+    sample of every ray is the same Space. Each pixel is then the FP32 sequential sum of n copies of
+    s1 = f32(f64(w) * 637810000 / (4/pi) / 1e4), w = t1 - t0 (Camera.cpp:73-78, 201, 223)."""
+    cube = _const_cube(1.0)
+    W_, H_ = O.image_dims(0.25, 36.0, 36.0)
+    v = O.make_view(W_, H_, 36.0, 36.0, [1.0, 1.0, 1.0], [1.0, 0, 0], 44, 200)
+    img, cnt = O.render_f32(cube, v, O.SEM_REFERENCE)
+    t = np.float32(44) * ((np.arange(2, dtype=np.float32) + np.float32(0.5)) / np.float32(200))
+    w = np.float32(t[1] - t[0])
+    s1 = np.float32(float(w) * 637810000.0 / (4 / np.pi) / 10000)
+    acc = np.float32(0)
+    for _ in range(200):
+        acc = np.float32(acc + s1)
+    assert cnt == W_ * H_ * 200
+    assert np.all(img == acc)
+    assert float(acc) == pytest.approx(200 * 0.22 * 6378.1 * 1e5 / (4 / np.pi) / 1e4, rel=1e-5)
+
+
+def test_linear_field_line_integral(lib):
+    """Trilinear interpolation reproduces a linear field exactly, so each pixel of the FP64 render
+    is the closed-form sum over its in-cube samples of (a.X(t_k) + e) * w * K."""
+    n = 48
+    lo = -15.0
+    spc = float(np.float32(30.0 / (n - 1)))  # the cube's metadata is float (DataCube.h:40-41)
+    g = lo + spc * np.arange(n)
+    a = np.array([2e-7, -1e-7, 3e-7])
+    e = 5e-6
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+    vals = (a[0] * X + a[1] * Y + a[2] * Z + e).astype(np.float32)
+    cube = O.OracleCube(vals, (lo, lo, lo), (spc, spc, spc))
+    pos = np.array([3.0, -4.0, 5.0])
+    v = O.make_view(24, 16, 26.5, 15.5, list(pos), [10.0, 0, 0], 30, 120)
+    img64, _ = O.render_f64(cube, v, O.SEM_EXTENDED)
+    _, _, rinv = O.view_basis_f64(v)
+    t = 30 * ((np.arange(120) + 0.5) / 120)
+    w = t[1] - t[0]
+    K = 637810000.0 / (4 / np.pi) / 10000
+    for (i, j) in [(0, 0), (23, 15), (12, 8), (5, 11)]:
+        d = O.ray_dir_f64(v, rinv, i, j)
+        Xk = pos + np.outer(t, d)
+        inside = np.all(Xk - lo >= 0, axis=1) & np.all((Xk - lo) / spc + 1 < n, axis=1)
+        want = np.sum((Xk[inside] @ a + e) * w * K)
+        assert inside.sum() > 20
+        assert img64[j, i] == pytest.approx(want, rel=2e-6)
+
+
+def test_incube_reference_equals_extended(lib):
+    vals = np.random.default_rng(2).lognormal(-14, 0.3, (40, 40, 40)).astype(np.float32)
+    cube = O.OracleCube(vals, (-10.0, -15.0, -15.0), (30 / 39, 30 / 39, 30 / 39))
+    v = O.make_view(48, 32, 26.5, 15.5, [5.0, 2.0, -3.0], [0.0, 0, 0], 44, 200)
+    ref, c1 = O.render_f32(cube, v, O.SEM_REFERENCE)
+    ext, c2 = O.render_f32(cube, v, O.SEM_EXTENDED)
+    assert c1 == c2 > 0
+    assert np.array_equal(ref.view(np.uint32), ext.view(np.uint32))
+    out = O.make_view(48, 32, 26.5, 15.5, [0.0, -20.0, 10.0], [10.0, 0, 0], 80, 160)
+    z, c3 = O.render_f32(cube, out, O.SEM_REFERENCE)
+    assert c3 == 0 and not z.any()
+
+
+def test_threads_do_not_change_results(lib):
+    vals = np.random.default_rng(3).lognormal(-14, 0.3, (32, 32, 32)).astype(np.float32)
+    cube = O.OracleCube(vals, (-10.0, -15.0, -15.0), (30 / 31,) * 3)
+    v = O.make_view(64, 40, 26.5, 15.5, [0.0, -20.0, 10.0], [10.0, 0, 0], 80, 400)
+    a, ca = O.render_f32(cube, v, O.SEM_EXTENDED, nthreads=1)
+    b, cb = O.render_f32(cube, v, O.SEM_EXTENDED, nthreads=7)
+    assert ca == cb and np.array_equal(a, b)
+
+
+def test_f32_and_f64_renders_agree(lib):
+    vals = np.random.default_rng(4).lognormal(-14, 0.3, (48, 48, 48)).astype(np.float32)
+    cube = O.OracleCube(vals, (-10.0, -15.0, -15.0), (30 / 47, 30 / 47, 30 / 47))
+    v = O.make_view(96, 56, 26.5, 15.5, [0.0, -20.0, 10.0], [10.0, 0, 0], 80, 400)
+    a, _ = O.render_f32(cube, v, O.SEM_EXTENDED)
+    b, _ = O.render_f64(cube, v, O.SEM_EXTENDED)
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-5
+
+
+# ------------------------------------------------------------------------- exact-arithmetic shortcuts
+@pytest.fixture(scope="module")
+def exact_checker(tmp_path_factory):
+    d = tmp_path_factory.mktemp("emc")
+    exe = d / "emc"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-pthread", os.path.join(HERE, "cpu", "exact_math_check.cpp"),
+                    "-o", str(exe)], check=True)
+    return str(exe)
+
+
+def test_scale_shortcut_exhaustive(exact_checker):
+    """birp_scale_sample == the reference's double chain for every non-NaN float (all 2^32 patterns)."""
+    r = subprocess.run([exact_checker, "scale", "0", "0x100000000"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout
+
+
+@pytest.mark.parametrize("spacing", ["0.236220479", "0.117647059", "0.0587084144", "0.99999994"])
+def test_division_shortcut_exhaustive(exact_checker, spacing):
+    """birp_div_rn(a, b, RN(1/b)) == a / b for every float a in [2^-64, 2^64) (the kernels use IEEE
+    division below 2^-64; ingress never exceeds 2^64)."""
+    r = subprocess.run([exact_checker, "divrange", spacing], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout
