@@ -1,0 +1,61 @@
+// Camera.h -- the SXI camera: ray generation, march and integration into an image
+// (reference include/Camera.h:20-63). Render runs on the GPU for DataCube and CMEM spaces.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "DataCube.h"
+
+class Camera {
+public:
+    // W = round(fov_w / pixel), H = round(fov_h / pixel) (Camera.cpp:38-39); 200 samples per ray
+    // over 44 RE (Camera.cpp:31, 65).
+    Camera(Space& cube, float pixel_size_deg, float plot_fov_h, float plot_fov_w);
+    // Addition: an explicit W x H (576x336 has no pixel size under the rule above).
+    Camera(Space& cube, int width, int height, float plot_fov_h, float plot_fov_w);
+    ~Camera();
+
+    void SetPosition(float&, float&, float&);
+    void SetAim(float, float, float);
+    // Adds one render into `image` (repeated calls accumulate, as Camera.cpp:223 does).
+    // Throws std::invalid_argument for a Space other than DataCube or CMEM, and
+    // std::runtime_error for a device failure.
+    void Render();
+    int ToDat(std::string filename);   // CSV, one row per line, trailing commas (Camera.cpp:235-250)
+    int ToFITS(std::string filename);  // FITS, BITPIX -32, row 0 first (Camera.cpp:252-307)
+
+    Space* dataCube;
+    std::vector<std::vector<float>> image;
+
+    // ---- additions ----
+    void SetRay(float length, int samples);  // default 44 RE, 200 samples
+    void SetSemantics(int semantics);        // birp.h BIRP_SEM_*, default BIRP_SEM_REFERENCE
+    void SetMath(int math);                  // birp.h BIRP_MATH_*, default BIRP_MATH_EXACT
+    int Width() const { return image_dimension_x; }
+    int Height() const { return image_dimension_y; }
+    long long LastSamplesInCube() const { return last_in_cube_; }
+    double LastKernelMs() const { return last_ms_; }
+
+private:
+    int image_dimension_x;
+    int image_dimension_y;
+    float fov_x;
+    float fov_y;
+    struct {
+        float x;
+        float y;
+        float z;
+    } position;
+    struct {
+        float x;
+        float y;
+        float z;
+    } aim;
+    float pixel_size_deg;
+    float ray_length_ = 44.f;
+    int ray_samples = 200;
+    int semantics_ = 0;
+    int math_ = 0;
+    long long last_in_cube_ = 0;
+    double last_ms_ = 0.0;
+};

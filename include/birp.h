@@ -73,7 +73,7 @@ typedef struct birp_server birp_server;
 
 /* Grid description (reference DataCube public members, DataCube.h:30-41). spacing is the
  * per-axis |coords[0]-coords[1]| (DataCube.cpp:150-183); origin is coords[0] (:185-191). */
-typedef struct {
+typedef struct birp_volume_desc {
     int nx, ny, nz;
     float origin[3];
     float spacing[3];
@@ -83,7 +83,7 @@ typedef struct {
 
 /* CMEM post-Init state (reference include/Cmem.h:43-76): the values CMEM::Init leaves in the
  * object, including B as overwritten by CalcMagneticPressure (Cmem.cpp:310). */
-typedef struct {
+typedef struct birp_cmem_state {
     float p0, p1, p2, p3;
     float bs, A1, A2, B, alpha, beta, ay_bs, az_bs;
     float a5, a21;
@@ -92,7 +92,7 @@ typedef struct {
 } birp_cmem_state;
 
 /* One camera pose and ray grid (reference Camera members, Camera.h:23-53). */
-typedef struct {
+typedef struct birp_view {
     int width, height;   /* W x H pixels; the reference derives them from the FOV (Camera.cpp:38-39) */
     float fov_w, fov_h;  /* degrees */
     float position[3];   /* camera position, RE (Camera::SetPosition) */
@@ -104,7 +104,7 @@ typedef struct {
     int flags;           /* BIRP_RENDER_* */
 } birp_view;
 
-typedef struct {
+typedef struct birp_render_stats {
     long long samples_in_cube; /* samples added into the image (cube renders), or n per ray (CMEM) */
     long long samples_nominal; /* W * H * n */
     double kernel_ms;          /* device time of the render kernel(s), from HIP events */
@@ -168,7 +168,7 @@ int birp_view_basis(const birp_view* view, float* angle_sun, float rot[9], float
  * captured in a HIP graph and replayable. Images are written into one device buffer; image i
  * starts at offset[i] floats. No reference counterpart: the reference renders one pose per
  * Camera (Camera.cpp:63) and batches cubes serially (main.cpp:242-264). */
-typedef struct {
+typedef struct birp_tuning {
     int tile_w, tile_h; /* rays per wave: tile_w * tile_h == 64 */
     int lanes_per_ray;  /* 1, 2 or 4 lanes share one ray's samples */
     int unroll;         /* samples per lane per loop trip: 1, 2 or 4 */
@@ -193,19 +193,19 @@ int birp_autotune_load(const char* path);
 int birp_autotune_lookup(int dtype, int width, int height, int samples, birp_tuning* out);
 
 /* ------------------------------------------------------------------ serving */
-typedef struct {
+typedef struct birp_server_config {
     size_t memory_budget;  /* device bytes for resident volumes + in-flight images */
     int max_batch;         /* most requests one launch may carry */
     int max_inflight;      /* launches in flight at once (each on its own stream) */
     double batch_window_ms;/* wait this long for more arrivals before launching a non-full batch */
 } birp_server_config;
 
-typedef struct {
+typedef struct birp_request {
     int volume;     /* index returned by birp_server_add_volume */
     birp_view view; /* math is forced to BIRP_MATH_FAST */
 } birp_request;
 
-typedef struct {
+typedef struct birp_result_info {
     int status;          /* BIRP_OK or the error of this request */
     double submit_s;     /* host clock at submission (seconds, steady clock) */
     double start_s;      /* launch time of the batch that carried it */
@@ -230,7 +230,7 @@ double birp_clock_s(void);
  * r_mp = r0 (2/(1+cos th))^0.6 with r0 ~ U(r0_lo, r0_hi), r_bs = 1.3 r_mp, shell value
  * amp (r/10)^-3, background bg elsewhere, times exp(sigma N(0,1)) per voxel (SplitMix64 +
  * Box-Muller, one normal per voxel in [z][y][x] order after the r0 draw). Host, threaded. */
-typedef struct {
+typedef struct birp_shell_params {
     int n;            /* voxels per axis */
     float x0, x1, y0, y1;
     float r0_lo, r0_hi;

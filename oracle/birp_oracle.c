@@ -271,7 +271,13 @@ float or_cmem_sample(const or_cmem* m, float x, float y, float z) {
 }
 
 /* The same model with every operation in double (FP64 oracle for CMEM renders). */
-double or_cmem_sample_f64(const or_cmem* m, double x, double y, double z) {
+/* The FP64 model's zone radii and inner/outer emissivities at one point (Cmem.cpp:188-288). */
+typedef struct {
+    double r, rmp, rbs, inner, outer;
+} cmem_zones64;
+
+static cmem_zones64 cmem_zones_f64(const or_cmem* m, double x, double y, double z) {
+    cmem_zones64 o;
     double r = sqrt(x * x + y * y + z * z);
     double theta = (x != 0) ? acos(x / r) : 0.0;
     double phi = (y != 0) ? acos(y / sqrt(y * y + z * z)) : 0.0;
@@ -283,19 +289,36 @@ double or_cmem_sample_f64(const or_cmem* m, double x, double y, double z) {
                    m->p1 * (m->beta_c[0] + m->beta_c[1] * cos(phi) + (double)m->beta_c[3] * (sp * sp)));
     double q = (double)m->p2 * m->charlie * exp((double)m->p3 * m->dn * pow(phi_n, (double)A[21])) +
                (double)m->p2 * m->charlie * exp((double)m->p3 * m->ds * pow(phi_s, (double)A[21]));
-    double rmp = (double)m->p0 * m->r0_lin * f + q;
     double ry = m->bs * pow(2 / (1 + cos(theta)), (double)m->ay_bs);
     double rz = m->bs * pow(2 / (1 + cos(theta)), (double)m->az_bs);
-    double rbs = (ry * rz) / sqrt((rz * cos(phi)) * (rz * cos(phi)) + (ry * sin(phi)) * (ry * sin(phi)));
-    double h = 0.025;
     double hs = theta / 2, ss = sin(theta);
-    double inner = (double)m->A1 * exp(-(double)m->B * (hs * hs * hs * hs)) * pow(r / 10, -(double)m->alpha - (double)m->beta * (ss * ss));
-    double outer = (double)m->A2 / ((r / 10) * (r / 10) * (r / 10));
-    if (r < rmp - h) return 0.0;
-    if (r <= rmp + h) return or_lerp(0.0, inner, (r - (rmp - h)) / (rmp + h));
-    if (r < rbs - h) return inner;
-    if (r <= rbs + h) return or_lerp(inner, outer, (r - (rbs - h)) / (rbs + h));
-    return outer;
+    o.r = r;
+    o.rmp = (double)m->p0 * m->r0_lin * f + q;
+    o.rbs = (ry * rz) / sqrt((rz * cos(phi)) * (rz * cos(phi)) + (ry * sin(phi)) * (ry * sin(phi)));
+    o.inner = (double)m->A1 * exp(-(double)m->B * (hs * hs * hs * hs)) * pow(r / 10, -(double)m->alpha - (double)m->beta * (ss * ss));
+    o.outer = (double)m->A2 / ((r / 10) * (r / 10) * (r / 10));
+    return o;
+}
+
+double or_cmem_sample_f64(const or_cmem* m, double x, double y, double z) {
+    cmem_zones64 o = cmem_zones_f64(m, x, y, z);
+    const double h = 0.025, r = o.r;
+    if (r < o.rmp - h) return 0.0;
+    if (r <= o.rmp + h) return or_lerp(0.0, o.inner, (r - (o.rmp - h)) / (o.rmp + h));
+    if (r < o.rbs - h) return o.inner;
+    if (r <= o.rbs + h) return or_lerp(o.inner, o.outer, (r - (o.rbs - h)) / (o.rbs + h));
+    return o.outer;
+}
+
+double or_cmem_edge_distance_f64(const or_cmem* m, double x, double y, double z) {
+    cmem_zones64 o = cmem_zones_f64(m, x, y, z);
+    const double h = 0.025;
+    double e[4] = {o.rmp - h, o.rmp + h, o.rbs - h, o.rbs + h}, best = INFINITY;
+    for (int k = 0; k < 4; k++) {
+        double d = fabs(o.r - e[k]);
+        if (d < best) best = d;
+    }
+    return o.r > 0 ? best / o.r : best;
 }
 
 /* ---------------------------------------------------------------- Camera geometry */
@@ -793,4 +816,26 @@ int or_render_f64(const or_space* s, const or_view* v, int semantics, int row0, 
     if (n_incube) *n_incube = jb.count;
     free(t);
     return 0;
+}
+
+int or_cmem_edge_pixels(const or_cmem* m, const or_view* v, double rel_band, unsigned char* mask) {
+    float rot[9], rinv[9], w;
+    if (or_view_basis(v, rot, rinv) != 0) return -1;
+    float* t = (float*)malloc(sizeof(float) * (size_t)(v->n > 0 ? v->n : 1));
+    or_ray_dist(v->L, v->n, t, &w);
+    int count = 0;
+    for (int j = 0; j < v->H; ++j)
+        for (int i = 0; i < v->W; ++i) {
+            float d[3];
+            or_ray_dir(v, rinv, i, j, d);
+            unsigned char hit = 0;
+            for (int k = 0; k < v->n && !hit; ++k) {
+                float x = d[0] * t[k] + v->pos[0], y = d[1] * t[k] + v->pos[1], z = d[2] * t[k] + v->pos[2];
+                if (or_cmem_edge_distance_f64(m, x, y, z) < rel_band) hit = 1;
+            }
+            mask[(size_t)j * v->W + i] = hit;
+            count += hit;
+        }
+    free(t);
+    return count;
 }

@@ -99,6 +99,13 @@ void or_cmem_init(or_cmem* m, const int* v, const int* b, float dipole, float p0
 void or_cmem_init_default(or_cmem* m);
 float or_cmem_sample(const or_cmem* m, float x, float y, float z);
 double or_cmem_sample_f64(const or_cmem* m, double x, double y, double z);
+/* |r - e| / r for the nearest zone edge e of r = rmp +- h, r = rbs +- h (Cmem.cpp:214-288), in
+ * FP64. The model jumps at these edges (the ramps are normalised by rmp + h, not 2h), so two
+ * FP32 evaluations whose libm results differ by an ulp can fall on opposite sides of one. */
+double or_cmem_edge_distance_f64(const or_cmem* m, double x, double y, double z);
+/* mask[j*W+i] = 1 when pixel (i,j)'s FP32 ray (traced as or_render_f32 traces it) has a sample
+ * within rel_band of a zone edge; returns the number of such pixels, -1 for a singular basis. */
+int or_cmem_edge_pixels(const or_cmem* m, const or_view* v, double rel_band, unsigned char* mask);
 
 /* Camera::Camera image size (Camera.cpp:38-39). */
 void or_image_dims(float pixel_deg, float fov_h, float fov_w, int* W, int* H);

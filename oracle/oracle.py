@@ -78,6 +78,10 @@ def lib():
         L.or_cmem_sample.argtypes = [C.POINTER(Cmem), _f, _f, _f]
         L.or_cmem_sample_f64.restype = _d
         L.or_cmem_sample_f64.argtypes = [C.POINTER(Cmem), _d, _d, _d]
+        L.or_cmem_edge_distance_f64.restype = _d
+        L.or_cmem_edge_distance_f64.argtypes = [C.POINTER(Cmem), _d, _d, _d]
+        L.or_cmem_edge_pixels.restype = _i
+        L.or_cmem_edge_pixels.argtypes = [C.POINTER(Cmem), C.POINTER(View), _d, C.POINTER(C.c_ubyte)]
         L.or_image_dims.argtypes = [_f, _f, _f, _ip, _ip]
         L.or_ray_dist.argtypes = [_f, _i, _fp, _fp]
         L.or_orient.restype = _f
@@ -151,6 +155,19 @@ class OracleCmem:
 
     def sample_f64(self, x, y, z) -> float:
         return lib().or_cmem_sample_f64(C.byref(self.m), x, y, z)
+
+    def edge_distance(self, x, y, z) -> float:
+        """Relative distance |r - e| / r of a point from the nearest zone edge (FP64)."""
+        return lib().or_cmem_edge_distance_f64(C.byref(self.m), x, y, z)
+
+    def edge_pixels(self, view: "View", rel_band: float) -> np.ndarray:
+        """Boolean HxW mask of pixels whose FP32 ray has a sample within rel_band of a zone edge."""
+        mask = np.zeros((view.H, view.W), np.uint8)
+        rc = lib().or_cmem_edge_pixels(C.byref(self.m), C.byref(view), rel_band,
+                                       mask.ctypes.data_as(C.POINTER(C.c_ubyte)))
+        if rc < 0:
+            raise ValueError("singular view basis")
+        return mask.astype(bool)
 
     def state(self) -> dict:
         return {name: (list(getattr(self.m, name)) if isinstance(getattr(self.m, name), C.Array)

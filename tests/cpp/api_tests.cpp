@@ -1,0 +1,206 @@
+// api_tests.cpp -- drives the C++ API (include/DataCube.h, Cmem.h, Camera.h, Helper.h; built into
+// libbirp_api.so) for tests/test_api_cpp.py. Each subcommand reads its inputs from argv/stdin and
+// prints results (floats as IEEE bit patterns) for the Python side to compare with the golden
+// vectors and the oracle. Host-only subcommands: inverse, cmem_state, load, write_outputs,
+// save_load. GPU subcommands: sample, render, cmem_sample, cmem_render, render_twice,
+// unsupported_space.
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "Camera.h"
+#include "Cmem.h"
+#include "DataCube.h"
+#include "Helper.h"
+#include "birp.h"
+
+static uint32_t bits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+static float from_bits(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// A Space that is neither DataCube nor CMEM, like the reference test's CubeFake
+// (tests/Camera_test.cpp:25-29).
+class CubeFake : public Space {
+public:
+    void Init() override {}
+    float GetSample(float, float, float) override { return 1.f; }
+};
+
+static void print_image(const Camera& cam) {
+    std::printf("%d %d\n", cam.Width(), cam.Height());
+    for (const auto& row : cam.image)
+        for (float v : row) std::printf("%u\n", bits(v));
+}
+
+static void init_cmem(CMEM& m, const std::string& which) {
+    if (which == "default") {
+        m.Init();
+    } else if (which == "test_params") {
+        // tests/CMEM_test.cpp:51-63, arguments bound exactly as that call binds them
+        std::vector<int> v = {400, 0, 0}, b = {0, 0, 4};
+        float dipole = 0.f;
+        int p1 = 1, p2 = 4, p3 = 6;
+        float B = 3.f, alpha = 3.6f, beta = -1.0f;
+        m.Init(true, v, b, dipole, p1, p2, p3, B, alpha, beta);
+    } else if (which == "cli_params") {
+        // main.cpp:285-294 with its defaults: p = {1,1,3,4}, the NULL floats 0
+        std::vector<float> p = {1.f, 1.f, 3.f, 4.f};
+        m.Init(true, {}, {}, 0.f, p[0], p[1], p[2], p[3], 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f);
+    } else {
+        throw std::invalid_argument("unknown cmem set " + which);
+    }
+}
+
+int main(int argc, char** argv) {
+    if (argc < 2) return 2;
+    const std::string cmd = argv[1];
+    try {
+        if (cmd == "inverse") {  // 9 uint32 per line -> inverse bits or "throw"
+            uint32_t u[9];
+            while (std::scanf("%u %u %u %u %u %u %u %u %u", &u[0], &u[1], &u[2], &u[3], &u[4], &u[5], &u[6], &u[7], &u[8]) == 9) {
+                std::vector<std::vector<float>> m(3, std::vector<float>(3));
+                for (int k = 0; k < 9; k++) m[k / 3][k % 3] = from_bits(u[k]);
+                try {
+                    auto inv = Helper::GetInverse(m);
+                    for (int k = 0; k < 9; k++) std::printf("%u%c", bits(inv[k / 3][k % 3]), k == 8 ? '\n' : ' ');
+                } catch (const std::runtime_error& e) {
+                    std::printf("throw %s\n", e.what());
+                }
+            }
+            return 0;
+        }
+        if (cmd == "cmem_state") {  // post-Init members as bits
+            CMEM m;
+            init_cmem(m, argv[2]);
+            birp_cmem_state s;
+            m.State(&s);
+            const float* f = &s.p0;
+            for (size_t k = 0; k < sizeof(s) / sizeof(float); k++) std::printf("%u\n", bits(f[k]));
+            return 0;
+        }
+        if (cmd == "load") {  // DataCube::Load: size, spacing, origin, then the values
+            DataCube c;
+            std::string path = argv[2];
+            c.Init(path, false);
+            std::printf("%d %d %d\n", c.size.x, c.size.y, c.size.z);
+            std::printf("%u %u %u\n", bits(c.spacing[0]), bits(c.spacing[1]), bits(c.spacing[2]));
+            std::printf("%u %u %u\n", bits(c.origin_coords[0]), bits(c.origin_coords[1]), bits(c.origin_coords[2]));
+            for (const auto& s : c.slices)
+                for (const auto& r : s)
+                    for (float v : r) std::printf("%u\n", bits(v));
+            return 0;
+        }
+        if (cmd == "save_load") {  // Load, Save, Load again: identical cube
+            DataCube a, b;
+            std::string path = argv[2], out = argv[3];
+            a.Init(path, false);
+            a.Save(out);
+            b.Init(out, false);
+            bool same = a.coords_x == b.coords_x && a.coords_y == b.coords_y && a.coords_z == b.coords_z && a.slices == b.slices;
+            std::printf("%s\n", same ? "same" : "different");
+            return same ? 0 : 1;
+        }
+        if (cmd == "bad_load") {  // a missing or truncated file throws std::invalid_argument
+            DataCube c;
+            std::string path = argv[2];
+            try {
+                c.Init(path, false);
+                std::printf("no throw\n");
+                return 1;
+            } catch (const std::invalid_argument& e) {
+                std::printf("invalid_argument %s\n", e.what());
+                return 0;
+            }
+        }
+        if (cmd == "write_outputs") {  // ToDat / ToFITS of a known image (no render)
+            CubeFake fake;
+            Camera cam(fake, 0.25f, 2.0f, 3.0f);  // 12 x 8
+            float x = 1.5f, y = -2.f, z = 3.25f;
+            cam.SetPosition(x, y, z);
+            cam.SetAim(7.f, 0.f, 0.f);
+            for (int j = 0; j < cam.Height(); j++)
+                for (int i = 0; i < cam.Width(); i++) cam.image[j][i] = 0.125f * i - 3.5f * j + 1e-3f * i * j;
+            std::string base = argv[2];
+            if (cam.ToDat(base) != 1 || cam.ToFITS(base) != 1) return 1;
+            print_image(cam);
+            return 0;
+        }
+        // ------------------------------------------------------------------ GPU subcommands
+        if (cmd == "sample") {  // DataCube::GetSample at points read as "x y z" bits
+            DataCube c;
+            std::string path = argv[2];
+            c.Init(path, false);
+            c.SetTrilinear(std::atoi(argv[3]) != 0);
+            uint32_t u[3];
+            while (std::scanf("%u %u %u", &u[0], &u[1], &u[2]) == 3)
+                std::printf("%u\n", bits(c.GetSample(from_bits(u[0]), from_bits(u[1]), from_bits(u[2]))));
+            return 0;
+        }
+        if (cmd == "render" || cmd == "render_twice") {
+            // render <dat> <trilinear> <pixel|W> <H or 0> <fov_h> <fov_w> <px> <py> <pz> <ax> <L> <n> <semantics>
+            DataCube c;
+            std::string path = argv[2];
+            c.Init(path, false);
+            c.SetTrilinear(std::atoi(argv[3]) != 0);
+            int H = std::atoi(argv[5]);
+            float fov_h = std::atof(argv[6]), fov_w = std::atof(argv[7]);
+            Camera cam = H > 0 ? Camera(c, std::atoi(argv[4]), H, fov_h, fov_w) : Camera(c, (float)std::atof(argv[4]), fov_h, fov_w);
+            float px = std::atof(argv[8]), py = std::atof(argv[9]), pz = std::atof(argv[10]);
+            cam.SetPosition(px, py, pz);
+            cam.SetAim(std::atof(argv[11]), 0.f, 0.f);
+            cam.SetRay(std::atof(argv[12]), std::atoi(argv[13]));
+            cam.SetSemantics(std::atoi(argv[14]));
+            cam.Render();
+            if (cmd == "render_twice") cam.Render();
+            print_image(cam);
+            std::fprintf(stderr, "in_cube %lld kernel_ms %.4f\n", cam.LastSamplesInCube(), cam.LastKernelMs());
+            return 0;
+        }
+        if (cmd == "cmem_sample") {
+            CMEM m;
+            init_cmem(m, argv[2]);
+            uint32_t u[3];
+            while (std::scanf("%u %u %u", &u[0], &u[1], &u[2]) == 3)
+                std::printf("%u\n", bits(m.GetSample(from_bits(u[0]), from_bits(u[1]), from_bits(u[2]))));
+            return 0;
+        }
+        if (cmd == "cmem_render") {  // the reference's commented-out CMEM render test pose (Camera_test.cpp:122-158)
+            CMEM m;
+            init_cmem(m, argv[2]);
+            Camera cam(m, 0.25f, 36.f, 36.f);
+            float x = 1.f, y = 1.f, z = 1.f;
+            cam.SetPosition(x, y, z);
+            cam.SetAim(1, 0, 0);
+            cam.Render();
+            print_image(cam);
+            return 0;
+        }
+        if (cmd == "unsupported_space") {
+            CubeFake fake;
+            Camera cam(fake, 0.25f, 36.f, 36.f);
+            try {
+                cam.Render();
+            } catch (const std::invalid_argument& e) {
+                std::printf("invalid_argument %s\n", e.what());
+                return 0;
+            }
+            return 1;
+        }
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "exception: %s\n", e.what());
+        return 3;
+    }
+    return 2;
+}

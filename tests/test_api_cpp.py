@@ -1,0 +1,211 @@
+"""The C++ API (include/DataCube.h, Cmem.h, Camera.h, Helper.h -> libbirp_api.so), driven through
+tests/cpp/api_tests.cpp: host-side parts against the reference's golden vectors and the oracle,
+GPU renders against the oracle bit for bit (CMEM renders by the rule in tests/parity.py)."""
+import json
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from parity import CMEM_EDGE_BAND, cmem_render_parity
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+GOLDEN = os.path.join(HERE, "golden")
+LIB = os.path.join(ROOT, "birp_cpp_amd", "lib")
+
+
+def f32(b):
+    return np.asarray(b, np.uint32).view(np.float32)
+
+
+@pytest.fixture(scope="module")
+def exe(tmp_path_factory):
+    out = tmp_path_factory.mktemp("api") / "api_tests"
+    if not os.path.exists(os.path.join(LIB, "libbirp_api.so")):
+        pytest.fail("birp_cpp_amd/lib/libbirp_api.so is missing: run `make -C birp_cpp_amd`")
+    subprocess.run(["g++", "-O2", "-std=c++20", "-I", os.path.join(ROOT, "include"), os.path.join(HERE, "cpp", "api_tests.cpp"),
+                    "-o", str(out), "-L", LIB, "-lbirp_api", "-lbirp_hip", f"-Wl,-rpath,{LIB}"], check=True)
+    return str(out)
+
+
+def run(exe, *args, stdin=""):
+    r = subprocess.run([exe, *map(str, args)], input=stdin, capture_output=True, text=True)
+    return r.returncode, r.stdout, r.stderr
+
+
+@pytest.fixture(scope="module")
+def golden_dat(tmp_path_factory):
+    g = json.load(open(os.path.join(GOLDEN, "datacube_samples.json")))
+    p = tmp_path_factory.mktemp("dat") / "golden.dat"
+    p.write_text(g["dat_text"])
+    return str(p), g
+
+
+# ---------------------------------------------------------------------------------- host side
+def test_helper_inverse_matches_reference(exe):
+    g = json.load(open(os.path.join(GOLDEN, "helper_matrix.json")))
+    mats = [p["abc"] for p in g["products"]] + [p["m"] for p in g["general"]]
+    want = [p["inverse"] for p in g["products"]] + [p["inverse"] if p["rc"] == 0 else None for p in g["general"]]
+    rc, out, err = run(exe, "inverse", stdin="\n".join(" ".join(map(str, m)) for m in mats) + "\n")
+    assert rc == 0, err
+    lines = out.strip().split("\n")
+    assert len(lines) == len(mats)
+    for line, w in zip(lines, want):
+        if w is None:
+            assert line.startswith("throw Determinant is 0")
+        else:
+            assert [int(x) for x in line.split()] == w
+
+
+@pytest.mark.parametrize("which,params", [("default", None),
+                                          ("test_params", dict(v=[400, 0, 0], b=[0, 0, 4], dipole=0.0, p0=1, p1=4, p2=6, p3=3,
+                                                               B=3.6, alpha=-1.0)),
+                                          ("cli_params", dict(dipole=0.0, p0=1.0, p1=1, p2=3, p3=4, B=0.0, alpha=0.0, beta=0.0,
+                                                              bs=0.0, A1=0.0, A2=0.0))])
+def test_cmem_init_state_matches_oracle(exe, oracle, which, params):
+    rc, out, err = run(exe, "cmem_state", which)
+    assert rc == 0, err
+    got = [int(x) for x in out.split()]
+    st = O.OracleCmem(params).state()
+    names = ["p0", "p1", "p2", "p3", "bs", "A1", "A2", "B", "alpha", "beta", "ay_bs", "az_bs"]
+    want = [st[n] for n in names] + [st["a"][5], st["a"][21]] + list(st["beta_c"]) + \
+           [st[n] for n in ("charlie", "dn", "ds", "theta_n", "theta_s", "r0_lin")]
+    assert got == np.asarray(want, np.float32).view(np.uint32).tolist()
+
+
+def test_load_matches_reference_loader(exe, golden_dat):
+    path, g = golden_dat
+    rc, out, err = run(exe, "load", path)
+    assert rc == 0, err
+    lines = out.split("\n")
+    assert [int(x) for x in lines[0].split()] == g["size"]
+    assert [int(x) for x in lines[1].split()] == g["spacing"]
+    assert [int(x) for x in lines[2].split()] == g["origin"]
+    vals = [int(x) for x in lines[3:] if x]
+    assert vals == np.asarray(g["values"]).ravel().tolist()
+
+
+def test_save_then_load_roundtrip(exe, golden_dat, tmp_path):
+    rc, out, err = run(exe, "save_load", golden_dat[0], tmp_path / "copy.dat")
+    assert rc == 0 and out.strip() == "same", err
+
+
+def test_bad_files_throw_invalid_argument(exe, golden_dat, tmp_path):
+    rc, out, _ = run(exe, "bad_load", tmp_path / "missing.dat")
+    assert rc == 0 and out.startswith("invalid_argument")
+    trunc = tmp_path / "trunc.dat"
+    trunc.write_text(golden_dat[1]["dat_text"][: len(golden_dat[1]["dat_text"]) // 2])
+    rc, out, _ = run(exe, "bad_load", trunc)
+    assert rc == 0 and out.startswith("invalid_argument")
+
+
+def test_todat_and_tofits_formats(exe, tmp_path):
+    base = tmp_path / "img"
+    rc, out, err = run(exe, "write_outputs", base)
+    assert rc == 0, err
+    vals = out.split()
+    W, H = int(vals[0]), int(vals[1])
+    img = f32([int(x) for x in vals[2:]]).reshape(H, W)
+    # ToDat: std::to_string (6 decimals) then ',' per value, one row per line (Camera.cpp:235-250)
+    rows = open(str(base) + ".dat").read().split("\n")
+    assert rows[-1] == "" and len(rows) == H + 1
+    for j in range(H):
+        assert rows[j] == "".join("%f," % v for v in img[j])
+    # ToFITS: primary HDU, BITPIX -32, NAXIS1 = W, NAXIS2 = H, header cards, row 0 first, big-endian
+    raw = open(str(base) + ".fits", "rb").read()
+    assert len(raw) % 2880 == 0
+    cards = {}
+    k = 0
+    while True:
+        c = raw[k:k + 80].decode()
+        k += 80
+        if c.startswith("END"):
+            break
+        key, _, rest = c.partition("=")
+        cards[key.strip()] = rest.split("/")[0].strip()
+    hdr_len = (k + 2879) // 2880 * 2880
+    assert cards["SIMPLE"] == "T" and cards["BITPIX"] == "-32" and cards["NAXIS"] == "2"
+    assert int(cards["NAXIS1"]) == W and int(cards["NAXIS2"]) == H
+    assert float(cards["CRVAL1"]) == -1.5 and float(cards["CRVAL2"]) == -1.0
+    assert float(cards["CDELT1"]) == 0.25 and float(cards["POS_Z"]) == 3.25 and float(cards["AIM_X"]) == 7.0
+    assert cards["RHO"] == "'0       '"
+    data = np.frombuffer(raw[hdr_len:hdr_len + 4 * W * H], dtype=">f4").reshape(H, W)
+    assert np.array_equal(data.astype(np.float32), img)
+
+
+# ---------------------------------------------------------------------------------- GPU
+@pytest.mark.gpu
+def test_getsample_matches_reference_golden(exe, golden_dat, device):
+    path, g = golden_dat
+    pts = np.asarray(g["points"], np.uint32).reshape(-1, 3)
+    stdin = "\n".join(" ".join(map(str, p)) for p in pts) + "\n"
+    for tri, key in ((1, "trilinear"), (0, "nearest")):
+        rc, out, err = run(exe, "sample", path, tri, stdin=stdin)
+        assert rc == 0, err
+        assert [int(x) for x in out.split()] == g[key]
+
+
+def _read_image(out):
+    vals = out.split()
+    W, H = int(vals[0]), int(vals[1])
+    return f32([int(x) for x in vals[2:]]).reshape(H, W)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("semantics", [0, 1])
+def test_camera_render_bitwise(exe, device, tmp_path, semantics):
+    """Camera::Render through the C++ API on a .dat cube equals the oracle bit for bit."""
+    from birp_cpp_amd import capi
+    vals, org, spc, _ = capi.synth_shell(64, (8.0, 11.0), 0.05, seed=11)
+    xs = (np.float32(org[0]) + np.float32(spc[0]) * np.arange(64, dtype=np.float32)).astype(np.float32)
+    ys = (np.float32(org[1]) + np.float32(spc[1]) * np.arange(64, dtype=np.float32)).astype(np.float32)
+    p = tmp_path / "c.dat"
+    with open(p, "w") as f:
+        f.write("cube\n" + "#\n" * 8 + "64 64 64\n")
+        for c in (xs, ys, ys):
+            f.write(" ".join("%.9g" % v for v in c) + "\n")
+        for z in range(64):
+            for y in range(64):
+                f.write(" ".join("%.9g" % v for v in vals[z, y]) + "\n")
+    pos, aim = (12.0, 4.0, 6.0), 1.0
+    rc, out, err = run(exe, "render", p, 1, 0.25, 0, 36.0, 36.0, *pos, aim, 44, 200, semantics)
+    assert rc == 0, err
+    img = _read_image(out)
+    # the coordinates the loader reads back define the cube's metadata
+    oc = O.OracleCube(vals, (float(xs[0]), float(ys[0]), float(ys[0])),
+                      (float(abs(xs[0] - xs[1])), float(abs(ys[0] - ys[1])), float(abs(ys[0] - ys[1]))))
+    v = O.make_view(144, 144, 36.0, 36.0, list(pos), [aim, 0, 0], 44, 200)
+    ref, _ = O.render_f32(oc, v, semantics)
+    assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+    rc, out, err = run(exe, "render_twice", p, 1, 0.25, 0, 36.0, 36.0, *pos, aim, 44, 200, semantics)
+    twice = _read_image(out)
+    ref2, _ = O.render_f32(oc, v, semantics, init=ref)
+    assert np.array_equal(twice.view(np.uint32), ref2.view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_cmem_api_known_answers_and_render(exe, device):
+    pts = np.array([[8, 4, 4], [4, 12, 10], [3, 1.5, 1.2], [8, 10, 10], [8, -10, -10], [8, 10, -10]], np.float32)
+    rc, out, err = run(exe, "cmem_sample", "default", stdin="\n".join(" ".join(map(str, p.view(np.uint32))) for p in pts) + "\n")
+    assert rc == 0, err
+    got = f32([int(x) for x in out.split()])
+    want = np.array([0, 4.46853346e-06, 0, 8.15947317e-07, 8.15947317e-07, 8.15947317e-07], np.float32)
+    np.testing.assert_allclose(got, want, rtol=4e-6, atol=0)  # device libm vs glibc: a few ulp
+    rc, out, err = run(exe, "cmem_render", "default")
+    assert rc == 0, err
+    img = _read_image(out)
+    v = O.make_view(144, 144, 36.0, 36.0, [1.0, 1.0, 1.0], [1.0, 0, 0], 44, 200)
+    oc = O.OracleCmem()
+    ref, _ = O.render_f32(oc, v, O.SEM_REFERENCE, nthreads=16)
+    ref64, _ = O.render_f64(oc, v, O.SEM_REFERENCE, nthreads=16)
+    cmem_render_parity(img, ref, ref64, oc.edge_pixels(v, CMEM_EDGE_BAND))
+
+
+@pytest.mark.gpu
+def test_render_of_other_space_types_is_refused(exe, device):
+    rc, out, _ = run(exe, "unsupported_space")
+    assert rc == 0 and out.startswith("invalid_argument")

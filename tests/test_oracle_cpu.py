@@ -59,6 +59,45 @@ def test_cmem_overwrites_B_and_p2_quirk(lib):
     assert st["p2"] == st["p3"] == 4.0
 
 
+def test_cmem_f64_model_and_zone_edges(lib):
+    """The FP64 CMEM model reproduces the reference's known answers to FP32 rounding, and the
+    zone-edge distance vanishes where the model switches zone (found by bisection along the
+    Sun-Earth line, where the sample turns from 0 to non-zero at r = rmp - h)."""
+    m = O.OracleCmem()
+    for p, want in [((4.0, 12.0, 10.0), 4.46853346e-06), ((8.0, 10.0, 10.0), 8.15947317e-07), ((8.0, -10.0, -10.0), 8.15947317e-07)]:
+        assert m.sample_f64(*p) == pytest.approx(want, rel=2e-6)
+    assert m.sample_f64(8.0, 4.0, 4.0) == 0.0
+    lo, hi = 5.0, 15.0
+    assert m.sample_f64(lo, 0, 0) == 0.0 and m.sample_f64(hi, 0, 0) > 0.0
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if m.sample_f64(mid, 0, 0) == 0.0 else (lo, mid)
+    assert m.edge_distance(hi, 0, 0) < 1e-12
+    assert m.edge_distance(hi + 0.05, 0, 0) < 1e-12  # rmp + h: rmp depends on direction only
+    assert m.edge_distance(hi + 0.025, 0, 0) == pytest.approx(0.025 / (hi + 0.025), rel=1e-9)
+    assert m.edge_distance(8.0, 10.0, 10.0) > 1e-3 and m.edge_distance(3.0, 1.5, 1.2) > 1e-3
+
+
+def test_cmem_parity_rule_is_not_vacuous(lib):
+    """The CMEM render rule of the GPU tests accepts the FP32 reference against itself and
+    rejects an image with one pixel 1% off or an error hidden in the edge exclusion."""
+    from parity import CMEM_EDGE_BAND, cmem_render_parity
+    m = O.OracleCmem()
+    v = O.make_view(72, 72, 36.0, 36.0, [1.0, 1.0, 1.0], [1.0, 0, 0], 44, 200)
+    a, _ = O.render_f32(m, v, O.SEM_REFERENCE, nthreads=8)
+    b, _ = O.render_f64(m, v, O.SEM_REFERENCE, nthreads=8)
+    mask = m.edge_pixels(v, CMEM_EDGE_BAND)
+    assert 0 < mask.sum() < 0.02 * mask.size
+    cmem_render_parity(a, a, b, mask)
+    j, i = np.unravel_index(np.argmax(np.where(mask, 0, a)), a.shape)
+    bad = a.copy()
+    bad[j, i] *= 1.01
+    with pytest.raises(AssertionError):
+        cmem_render_parity(bad, a, b, mask)
+    with pytest.raises(AssertionError):
+        cmem_render_parity(a, a, b, np.ones_like(mask))
+
+
 # ------------------------------------------------------------------------- golden vectors (reference code)
 def test_datacube_golden_bitwise(lib):
     g = json.load(open(os.path.join(GOLDEN, "datacube_samples.json")))
