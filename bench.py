@@ -11,7 +11,8 @@ Also measured in the same run (keys under "forms"):
                         renders/s of the same 256 views for the ratio
   precision             512^3 cube stored FP32/BF16/F16 (per-brick scale), 576x336, step 0.05 RE:
                         relative L2 and max relative error of every image against the FP64 render
-  serving               Poisson requests at 200/s over 8 resident cubes: p50/p99 latency, renders/s
+  serving               Poisson requests at 200/s over 8 resident cubes: p50/p99 latency, renders/s;
+                        the same requests as one burst (saturated server) and through the plain path
 The CPU baseline is the oracle's FP32 restatement of the reference render (a port: the reference's
 own Camera.cpp cannot be compiled here) on the same 576x336 image, on the host's cores.
 """
@@ -123,11 +124,12 @@ def bench_batched(capi, W, dev, vals, org, spc, fp32_vol):
     # plain path on the same 256 views: one exact render, wait and host copy each
     ex = [capi.make_view(x.width, x.height, x.fov_w, x.fov_h, list(x.position), list(x.aim), x.ray_length, x.samples,
                          capi.SEM_EXTENDED, capi.MATH_EXACT) for x in views]
-    for x in ex[:8]:
-        fp32_vol.render(x)
+    bufs = _result_buffers(ex)
+    for k in range(8):
+        fp32_vol.render(ex[k], out=bufs[k])
     t = time.perf_counter()
-    for x in ex:
-        fp32_vol.render(x)
+    for k, x in enumerate(ex):
+        fp32_vol.render(x, out=bufs[k])
     plain_el = time.perf_counter() - t
     # accuracy of the batch against the FP64 render (a sample of images)
     errs = [rel_l2(b.image(k), fp32_vol.render_f64(views[k])) for k in range(0, 256, 16)]
@@ -180,23 +182,22 @@ def bench_precision(capi, W, dev, n=512, count=64):
     return out
 
 
-def bench_serving(capi, W, dev, num=2000, rate=200.0):
-    cubes = []
-    for k, (n, dt) in enumerate(W.cubes_config5()):
-        vals, org, spc, _ = capi.synth_shell(n, (7.0, 12.0), 0.1, seed=500 + k)
-        cubes.append(capi.Volume.upload(dev, vals, org, spc, dtype=dt))
-        del vals
-    budget = sum(c.nbytes() for c in cubes) + (256 << 20)
-    srv = capi.Server(dev, memory_budget=budget, max_batch=64, max_inflight=3, batch_window_ms=0.2)
-    idx = [srv.add_volume(c) for c in cubes]
-    t_arr, which, views = W.requests_config5(num, rate)
+def _result_buffers(views):
+    """One written-to (so already paged-in) result image per request, as a serving client that
+    reuses its response buffers holds them."""
+    return [np.ones((v.height, v.width), np.float32) for v in views]
+
+
+def _serve(capi, srv, idx, which, views, t_arr, bufs):
+    """Submit requests at their arrival offsets (all at once when t_arr is None); wait for all."""
     ids = []
     start = capi.clock_s() + 0.05
-    for k in range(num):
-        target = start + t_arr[k]
-        while capi.clock_s() < target:
-            pass
-        ids.append(srv.submit(idx[which[k]], views[k]))
+    for k in range(len(views)):
+        if t_arr is not None:
+            target = start + t_arr[k]
+            while capi.clock_s() < target:
+                pass
+        ids.append(srv.submit(idx[which[k]], views[k], out=bufs[k]))
     lat, done_max, fails, bsz = [], 0.0, 0, []
     for rid in ids:
         st, _, info = srv.wait(rid, 60000.0)
@@ -206,14 +207,40 @@ def bench_serving(capi, W, dev, num=2000, rate=200.0):
         lat.append(info.done_s - info.submit_s)
         done_max = max(done_max, info.done_s)
         bsz.append(info.batch_size)
+    lat = np.array(lat) * 1e3
+    first = start if t_arr is not None else start - 0.05
+    return {"requests": len(views), "failed": fails, "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3), "renders_per_s": round(len(lat) / (done_max - first), 1),
+            "mean_batch": round(float(np.mean(bsz)), 2)}
+
+
+def bench_serving(capi, W, dev, num=2000, rate=200.0):
+    """Config-5 serving: 8 resident cubes, Poisson arrivals at `rate`, then the same requests as
+    one burst (the server saturated), then the same requests through the plain path (one
+    birp_render and host sync per request) for the renders/s comparison."""
+    cubes = []
+    for k, (n, dt) in enumerate(W.cubes_config5()):
+        vals, org, spc, _ = capi.synth_shell(n, (7.0, 12.0), 0.1, seed=500 + k)
+        cubes.append(capi.Volume.upload(dev, vals, org, spc, dtype=dt))
+        del vals
+    budget = sum(c.nbytes() for c in cubes) + (256 << 20)
+    srv = capi.Server(dev, memory_budget=budget, max_batch=64, max_inflight=3, batch_window_ms=0.2)
+    idx = [srv.add_volume(c) for c in cubes]
+    t_arr, which, views = W.requests_config5(num, rate)
+    bufs = _result_buffers(views)
+    out = {"rate_hz": rate, "budget_bytes": budget}
+    out["poisson"] = _serve(capi, srv, idx, which, views, t_arr, bufs)
+    out["saturated"] = _serve(capi, srv, idx, which, views, None, bufs)
     srv.destroy()
+    t0 = time.perf_counter()
+    for k in range(num):
+        cubes[which[k]].render(views[k], out=bufs[k])
+    plain_rps = num / (time.perf_counter() - t0)
     for c in cubes:
         c.free()
-    lat = np.array(lat) * 1e3
-    span = done_max - start
-    return {"requests": num, "rate_hz": rate, "failed": fails, "p50_ms": round(float(np.percentile(lat, 50)), 3),
-            "p99_ms": round(float(np.percentile(lat, 99)), 3), "renders_per_s": round(len(lat) / span, 1),
-            "mean_batch": round(float(np.mean(bsz)), 2), "budget_bytes": budget}
+    out["plain_renders_per_s_same_requests"] = round(plain_rps, 1)
+    out["saturated_speedup_vs_plain"] = round(out["saturated"]["renders_per_s"] / plain_rps, 2)
+    return out
 
 
 def main():

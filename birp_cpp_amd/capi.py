@@ -238,8 +238,17 @@ class Volume:
         check(lib().birp_volume_sample(self.h, fptr(p), p.shape[0], fptr(out)), "birp_volume_sample")
         return out
 
-    def render(self, view: View, init: np.ndarray | None = None):
-        img = np.zeros((view.height, view.width), np.float32) if init is None else np.ascontiguousarray(init, np.float32).copy()
+    def render(self, view: View, init: np.ndarray | None = None, out: np.ndarray | None = None):
+        """Render into a new image, into `out` (H x W float32, C-contiguous; overwritten unless the
+        view accumulates), or into a copy of `init` (the accumulate contract of Camera::Render)."""
+        if out is not None:
+            if out.dtype != np.float32 or out.shape != (view.height, view.width) or not out.flags.c_contiguous:
+                raise ValueError("out must be a C-contiguous float32 array of shape (height, width)")
+            img = out
+            if init is not None:
+                img[...] = init
+        else:
+            img = np.zeros((view.height, view.width), np.float32) if init is None else np.ascontiguousarray(init, np.float32).copy()
         st = RenderStats()
         check(lib().birp_render(self.h, C.byref(view), fptr(img), C.byref(st)), "birp_render")
         return img, st
@@ -331,8 +340,13 @@ class Server:
         check(lib().birp_server_add_volume(self.h, vol.h, C.byref(idx)), "birp_server_add_volume")
         return idx.value
 
-    def submit(self, volume: int, view: View) -> int:
-        out = np.zeros((view.height, view.width), np.float32)
+    def submit(self, volume: int, view: View, out: np.ndarray | None = None) -> int:
+        """Queue a render; the image is written into `out` (or a new array) by the time wait()
+        returns for this id."""
+        if out is None:
+            out = np.zeros((view.height, view.width), np.float32)
+        elif out.dtype != np.float32 or out.shape != (view.height, view.width) or not out.flags.c_contiguous:
+            raise ValueError("out must be a C-contiguous float32 array of shape (height, width)")
         rid = C.c_uint64()
         req = Request(volume, view)
         check(lib().birp_server_submit(self.h, C.byref(req), fptr(out), C.byref(rid)), "birp_server_submit")

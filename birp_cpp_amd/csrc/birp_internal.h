@@ -46,6 +46,7 @@ struct DevVolume {
     float ox, oy, oz;     // origin (coords[0])
     float sx, sy, sz;     // spacing
     float rsx, rsy, rsz;  // RN(1/spacing), host-computed
+    double rdx, rdy, rdz; // RN64(1/spacing), host-computed (exact-path divisions)
     float fnx, fny, fnz;  // (float) n per axis, as the reference's bound test compares
     int interp;           // 0 nearest, 1 trilinear
     int dtype;            // BIRP_DTYPE_*

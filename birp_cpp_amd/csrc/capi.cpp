@@ -386,6 +386,10 @@ extern "C" int birp_volume_upload(birp_device* dev, const birp_volume_desc* desc
         d.rsx = one / d.sx;
         d.rsy = one / d.sy;
         d.rsz = one / d.sz;
+        volatile double done = 1.0;
+        d.rdx = done / (double)d.sx;
+        d.rdy = done / (double)d.sy;
+        d.rdz = done / (double)d.sz;
     }
     d.fnx = (float)nx;
     d.fny = (float)ny;

@@ -6,6 +6,14 @@
 //                           second correction correctly rounded). Replaces the per-sample
 //                           divisions of DataCube.cpp:206,209,212 and the ~12-instruction IEEE
 //                           division sequence with 1 multiply + 4 FMAs.
+//  birp_div_rn64(a, rb64):  RN(a / b) for rb64 = RN64(1/b): the float of the double product
+//                           a * rb64. That product is within 2^-52 (relative) of a/b, while a
+//                           quotient of two normal floats is never a float rounding midpoint and
+//                           lies at least 2^-49 (relative) from every one (for a midpoint m,
+//                           a - b m is a non-zero multiple of ulp(b) times half an ulp of the
+//                           quotient, and b < 2^24 ulp(b)), so rounding the
+//                           product to float gives RN(a/b) for every normal quotient. 3 VALU
+//                           operations (two conversions and a double multiply).
 //  birp_scale_sample(vw):   f32( f64(vw) * 637810000 / (4/pi) / 10000 ), the chain of
 //                           Camera.cpp:201, as one double multiply by the chain's constant plus a
 //                           guard: the two results differ by < 8 double ulps, so their float
@@ -52,6 +60,8 @@ BIRP_HD float birp_div_rn(float a, float b, float rb) {
     float e1 = birp_fmaf(-q1, b, a);
     return birp_fmaf(e1, rb, q1);
 }
+
+BIRP_HD float birp_div_rn64(float a, double rb64) { return (float)((double)a * rb64); }
 
 // The reference chain, operation by operation (Camera.cpp:201).
 BIRP_HD float birp_scale_chain(float vw) {

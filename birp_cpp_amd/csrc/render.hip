@@ -129,8 +129,9 @@ struct ExactIdx {
     float xd, yd, zd;  // index
 };
 
-// GUARD: the residual-corrected division is proven exact for |ingress| >= 2^-64; below that
-// (possible only when an origin component is within 2^-30 of 0) the IEEE division is used.
+// GUARD: the double-product division is exact while the quotient is a normal float, which
+// holds for |ingress| >= 2^-64 (spacing <= 2^30); below that (possible only when an origin
+// component is within 2^-30 of 0) the IEEE division is used.
 template <bool GUARD>
 __device__ __forceinline__ ExactIdx exact_index(const DevVolume& v, float dx, float dy, float dz, float px, float py,
                                                 float pz, float t) {
@@ -150,9 +151,9 @@ __device__ __forceinline__ ExactIdx exact_index(const DevVolume& v, float dx, fl
             return r;
         }
     }
-    r.xd = birp_div_rn(r.xi, v.sx, v.rsx);
-    r.yd = birp_div_rn(r.yi, v.sy, v.rsy);
-    r.zd = birp_div_rn(r.zi, v.sz, v.rsz);
+    r.xd = birp_div_rn64(r.xi, v.rdx);
+    r.yd = birp_div_rn64(r.yi, v.rdy);
+    r.zd = birp_div_rn64(r.zi, v.rdz);
     return r;
 }
 
@@ -174,11 +175,14 @@ struct QuadCtx {
     float hx, hy, hz;  // n - 2 per axis
 };
 
+// For an in-cube sample the index is >= 0, so the lower corner floor(x) is the truncating
+// conversion of the clamped index (the clamp only matters to lanes outside the cube, whose
+// result is discarded) and the weight x - floor(x) is v_fract_f32 (exact: the subtraction of
+// floor(x) from x >= 0 is exact, so fract's clamp below 1.0 never applies).
 template <bool POS>
 __device__ __forceinline__ float sample_quad(const QuadCtx& c, float xd, float yd, float zd) {
-    float x0f = floorf(xd), y0f = floorf(yd), z0f = floorf(zd);
-    float fx = xd - x0f, fy = yd - y0f, fz = zd - z0f;
-    unsigned off = c.tx[to_index(x0f, c.hx)] + c.ty[to_index(y0f, c.hy)] + c.tz[to_index(z0f, c.hz)];
+    float fx = __builtin_amdgcn_fractf(xd), fy = __builtin_amdgcn_fractf(yd), fz = __builtin_amdgcn_fractf(zd);
+    unsigned off = c.tx[to_index(xd, c.hx)] + c.ty[to_index(yd, c.hy)] + c.tz[to_index(zd, c.hz)];
     float4 a = load16(c.rs, off);
     float4 b = load16(c.rs, off + 1024u);
     if constexpr (POS) {
@@ -359,7 +363,12 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact(DevVolume v, ExactV
             if (ev.semantics == BIRP_SEM_REFERENCE && kf != 0) kl = -1;  // first sample outside: ray dead
         }
     }
-    const int kmin = wave_min_i(kf), kmax = wave_max_i(kl);
+    // Every ray walks its own range kf..kl: sub-lane `sub` takes k = kf + kb + sub, and the wave
+    // makes as many trips as its longest ray needs. t is read at a clamped LDS byte offset.
+    const int len = (kl >= kf) ? kl - kf + 1 : 0;
+    const int trips = wave_max_i(len);
+    const int lim = len - sub;  // sample kb + sub of this ray exists iff kb < lim
+    const unsigned t_first = (unsigned)(kf + sub) * 4u, t_last = (unsigned)(n - 1) * 4u;
 
     float acc = 0.f;
     if (valid && ev.accumulate) acc = ev.out[(size_t)j * ev.W + i];
@@ -368,10 +377,9 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact(DevVolume v, ExactV
     const float w = ev.w;
     const float px = ev.px, py = ev.py, pz = ev.pz;
 
-    for (int kb = kmin; kb <= kmax; kb += S) {
-        const int k = kb + sub;
-        const float t = s_t[min(k, n - 1)];
-        const bool act = (k >= kf) && (k <= kl);
+    for (int kb = 0; kb < trips; kb += S) {
+        const float t = *(const float*)((const char*)s_t + min(t_first + (unsigned)kb * 4u, t_last));
+        const bool act = kb < lim;
         float val;
         if constexpr (MODE == SM_CMEM) {
             val = cmem_sample(v.cmem, dx * t + px, dy * t + py, dz * t + pz);
@@ -432,7 +440,7 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact(DevVolume v, ExactV
             }
         }
     }
-    if constexpr (POS) added = (kl >= kf) ? (unsigned)(kl - kf + 1) : 0u;
+    if constexpr (POS) added = (unsigned)len;
     if (valid && sub == 0) ev.out[(size_t)j * ev.W + i] = acc;
     if (ev.counter) {
         unsigned tot = wave_sum_u((valid && sub == 0) ? added : 0u);

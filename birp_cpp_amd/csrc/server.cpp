@@ -6,12 +6,16 @@
 // footprint (device image + pinned staging) fits the memory budget left after the resident
 // volumes, and launches them as one persistent-kernel batch (one launch per storage dtype). A
 // request that arrives while other batches are on the GPU is launched in the next free slot, so
-// its work overlaps theirs on the device. Every request is rendered by the same kernel tuning and
-// the same per-tile arithmetic whatever it is batched with, so results do not depend on the mix.
+// its work overlaps theirs on the device. When a slot's device work is done, a completion thread
+// copies its images from pinned staging into the callers' buffers (split over a few threads for
+// large batches) and releases the slot, so the dispatcher keeps launching meanwhile. Every
+// request is rendered by the same kernel tuning and the same per-tile arithmetic whatever it is
+// batched with, so results do not depend on the mix.
 // No reference counterpart: the reference renders one pose per process run (main.cpp:266-316).
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -60,7 +64,8 @@ struct Slot {
     std::vector<Request*> reqs;
     std::vector<size_t> offsets;
     size_t bytes = 0;
-    bool busy = false;
+    std::atomic<bool> busy{false};     // launched and not yet released
+    std::atomic<bool> copying{false};  // device work done; handed to the completion thread
 };
 
 }  // namespace
@@ -79,9 +84,13 @@ struct birp_server {
     std::deque<Request*> pending;
     std::unordered_map<uint64_t, Request*> reqs;
     uint64_t next_id = 1;
-    std::vector<Slot> slots;
+    std::deque<Slot> slots;  // deque: slots hold atomics and never move
     std::thread worker;
+    std::thread completer;
+    std::deque<Slot*> to_copy;  // guarded by mu
+    std::condition_variable cv_copy;
     bool stop = false;
+    bool stop_copy = false;
     int setup_error = BIRP_OK;
 };
 
@@ -94,7 +103,7 @@ int slot_reserve_images(Slot& s, size_t pixels) {
     s.d_images = nullptr;
     s.h_images = nullptr;
     s.image_cap = 0;
-    size_t cap = std::max(pixels, (size_t)1 << 20);
+    size_t cap = std::max(pixels + pixels / 2, (size_t)1 << 22);  // headroom: hipFree synchronises the device
     hipError_t e = hipMalloc(&s.d_images, cap * 4);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc(server images)");
     e = hipHostMalloc(&s.h_images, cap * 4, hipHostMallocDefault);
@@ -155,12 +164,38 @@ int launch_slot(birp_server* S, Slot& s) {
     return BIRP_OK;
 }
 
+// Copy the slot's images out of pinned staging; batches above 8 MB are split over up to 4 threads.
+void copy_out(Slot& s) {
+    size_t total = 0;
+    for (Request* r : s.reqs) total += r->pixels * 4;
+    auto run = [&s](size_t a, size_t b) {
+        for (size_t k = a; k < b; k++) memcpy(s.reqs[k]->result, s.h_images + s.offsets[k], s.reqs[k]->pixels * 4);
+    };
+    const size_t n = s.reqs.size();
+    const size_t parts = std::min<size_t>(n, total > ((size_t)8 << 20) ? 4 : 1);
+    if (parts <= 1) {
+        run(0, n);
+        return;
+    }
+    // split by bytes, not by count, so the threads get similar amounts of copying
+    std::vector<size_t> cut(1, 0);
+    size_t acc = 0, next = total / parts;
+    for (size_t k = 0; k < n && cut.size() < parts; k++) {
+        acc += s.reqs[k]->pixels * 4;
+        if (acc >= next) {
+            cut.push_back(k + 1);
+            next += total / parts;
+        }
+    }
+    cut.push_back(n);
+    std::vector<std::thread> th;
+    for (size_t p = 1; p + 1 < cut.size(); p++) th.emplace_back(run, cut[p], cut[p + 1]);
+    run(cut[0], cut[1]);
+    for (auto& t : th) t.join();
+}
+
 void finish_slot(birp_server* S, Slot& s, int status) {
     double now = birp_clock_s();
-    if (status == BIRP_OK) {
-        for (size_t k = 0; k < s.reqs.size(); k++)
-            memcpy(s.reqs[k]->result, s.h_images + s.offsets[k], s.reqs[k]->pixels * 4);
-    }
     std::lock_guard<std::mutex> lk(S->mu);
     for (Request* r : s.reqs) {
         r->status = status;
@@ -170,22 +205,49 @@ void finish_slot(birp_server* S, Slot& s, int status) {
     S->inflight_bytes -= s.bytes;
     s.bytes = 0;
     s.reqs.clear();
+    s.copying = false;
     s.busy = false;
     S->cv_done.notify_all();
+}
+
+void completer(birp_server* S) {
+    while (true) {
+        Slot* s = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_copy.wait(lk, [&] { return S->stop_copy || !S->to_copy.empty(); });
+            if (S->to_copy.empty()) return;  // stop_copy with nothing left
+            s = S->to_copy.front();
+            S->to_copy.pop_front();
+        }
+        copy_out(*s);
+        finish_slot(S, *s, BIRP_OK);
+    }
 }
 
 void dispatcher(birp_server* S) {
     (void)hipSetDevice(S->dev->ordinal);
     const size_t budget = S->cfg.memory_budget;
     while (true) {
-        // 1. retire finished launches
-        bool any_busy = false;
+        // 1. hand finished launches to the completion thread
+        bool any_busy = false, any_held = false;
         for (Slot& s : S->slots) {
             if (!s.busy) continue;
+            any_held = true;
+            if (s.copying) continue;
             hipError_t q = hipEventQuery(s.done);
-            if (q == hipSuccess) finish_slot(S, s, BIRP_OK);
-            else if (q != hipErrorNotReady) finish_slot(S, s, hip_fail(q, "server batch"));
-            else any_busy = true;
+            if (q == hipSuccess) {
+                s.copying = true;
+                {
+                    std::lock_guard<std::mutex> lk(S->mu);
+                    S->to_copy.push_back(&s);
+                }
+                S->cv_copy.notify_one();
+            } else if (q != hipErrorNotReady) {
+                finish_slot(S, s, hip_fail(q, "server batch"));
+            } else {
+                any_busy = true;
+            }
         }
         // 2. admit into a free slot
         Slot* free_slot = nullptr;
@@ -194,7 +256,7 @@ void dispatcher(birp_server* S) {
         std::vector<Request*> failed;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            if (S->stop && S->pending.empty() && !any_busy) break;
+            if (S->stop && S->pending.empty() && !any_held) break;
             if (free_slot && !S->pending.empty()) {
                 // a short window lets a trickle of arrivals share a launch while the GPU is busy
                 if (any_busy && (int)S->pending.size() < S->cfg.max_batch && S->cfg.batch_window_ms > 0) {
@@ -235,7 +297,7 @@ void dispatcher(birp_server* S) {
                     }
                     S->cv_done.notify_all();
                 }
-            } else if (!any_busy && S->pending.empty()) {
+            } else if (!any_held && S->pending.empty()) {
                 S->cv_work.wait(lk, [&] { return S->stop || !S->pending.empty(); });
                 continue;
             }
@@ -246,7 +308,7 @@ void dispatcher(birp_server* S) {
             if (rc) finish_slot(S, *free_slot, rc);
             continue;
         }
-        if (any_busy) std::this_thread::sleep_for(std::chrono::microseconds(20));
+        if (any_held) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
 }
 
@@ -265,7 +327,7 @@ extern "C" int birp_server_create(birp_device* dev, const birp_server_config* cf
     hipError_t e = hipMalloc(&S->d_vols, sizeof(DevVolume) * S->vol_cap);
     int rc = BIRP_OK;
     if (e != hipSuccess) rc = hip_fail(e, "hipMalloc(server volumes)");
-    S->slots.resize(cfg->max_inflight);
+    for (int k = 0; k < cfg->max_inflight; k++) S->slots.emplace_back();
     const int nb = cfg->max_batch;
     for (Slot& s : S->slots) {
         if (rc) break;
@@ -281,6 +343,7 @@ extern "C" int birp_server_create(birp_device* dev, const birp_server_config* cf
         birp_server_destroy(S);
         return rc;
     }
+    S->completer = std::thread(completer, S);
     S->worker = std::thread(dispatcher, S);
     *out = S;
     return BIRP_OK;
@@ -295,6 +358,14 @@ extern "C" int birp_server_destroy(birp_server* S) {
         }
         S->cv_work.notify_all();
         S->worker.join();
+    }
+    if (S->completer.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->stop_copy = true;
+        }
+        S->cv_copy.notify_all();
+        S->completer.join();
     }
     for (Slot& s : S->slots) {
         if (s.stream) (void)hipStreamSynchronize(s.stream);

@@ -2,6 +2,7 @@
 //   scale <lo_bits> <hi_bits>     every float bit pattern in [lo, hi): birp_scale_sample == chain
 //   div <b>                       every non-negative finite float a: birp_div_rn(a,b,RN(1/b)) == a/b
 //   divrange <b>                  the same for every float a in [2^-64, 2^64) (the kernels' range)
+//   div64range <b>                birp_div_rn64(a, RN64(1/b)) == a/b for every float a in [2^-64, 2^64)
 // Prints "mismatches N checked M" and exits 0 when N == 0. Threads split the range.
 #include <cmath>
 #include <cstdint>
@@ -36,11 +37,14 @@ int main(int argc, char** argv) {
                 }
                 bad += b; checked += c;
             });
-    } else if (!strcmp(argv[1], "div") || !strcmp(argv[1], "divrange")) {
+    } else if (!strcmp(argv[1], "div") || !strcmp(argv[1], "divrange") || !strcmp(argv[1], "div64range")) {
         float bb = strtof(argv[2], 0);
         volatile float one = 1.0f;
+        volatile double done = 1.0;
         float rb = one / bb;
-        const bool ranged = !strcmp(argv[1], "divrange");
+        double rb64 = done / (double)bb;
+        const bool ranged = strcmp(argv[1], "div") != 0;
+        const bool d64 = !strcmp(argv[1], "div64range");
         const uint64_t lo = ranged ? (uint64_t)(127 - 64) << 23 : 0, hi = ranged ? (uint64_t)(127 + 64) << 23 : 0x7F800000ull;
         for (unsigned t = 0; t < nt; t++)
             th.emplace_back([&, t] {
@@ -48,7 +52,7 @@ int main(int argc, char** argv) {
                 for (uint64_t u = lo + t; u < hi; u += nt) {
                     float a = f_of((uint32_t)u);
                     volatile float q = a / bb;
-                    float r = birp_div_rn(a, bb, rb);
+                    float r = d64 ? birp_div_rn64(a, rb64) : birp_div_rn(a, bb, rb);
                     if (u_of(r) != u_of(q)) b++;
                     c++;
                 }

@@ -116,25 +116,14 @@ def test_todat_and_tofits_formats(exe, tmp_path):
     for j in range(H):
         assert rows[j] == "".join("%f," % v for v in img[j])
     # ToFITS: primary HDU, BITPIX -32, NAXIS1 = W, NAXIS2 = H, header cards, row 0 first, big-endian
-    raw = open(str(base) + ".fits", "rb").read()
-    assert len(raw) % 2880 == 0
-    cards = {}
-    k = 0
-    while True:
-        c = raw[k:k + 80].decode()
-        k += 80
-        if c.startswith("END"):
-            break
-        key, _, rest = c.partition("=")
-        cards[key.strip()] = rest.split("/")[0].strip()
-    hdr_len = (k + 2879) // 2880 * 2880
+    assert os.path.getsize(str(base) + ".fits") % 2880 == 0
+    cards, data = _read_fits(str(base) + ".fits")
     assert cards["SIMPLE"] == "T" and cards["BITPIX"] == "-32" and cards["NAXIS"] == "2"
     assert int(cards["NAXIS1"]) == W and int(cards["NAXIS2"]) == H
     assert float(cards["CRVAL1"]) == -1.5 and float(cards["CRVAL2"]) == -1.0
     assert float(cards["CDELT1"]) == 0.25 and float(cards["POS_Z"]) == 3.25 and float(cards["AIM_X"]) == 7.0
     assert cards["RHO"] == "'0       '"
-    data = np.frombuffer(raw[hdr_len:hdr_len + 4 * W * H], dtype=">f4").reshape(H, W)
-    assert np.array_equal(data.astype(np.float32), img)
+    assert np.array_equal(data, img)
 
 
 # ---------------------------------------------------------------------------------- GPU
@@ -155,29 +144,56 @@ def _read_image(out):
     return f32([int(x) for x in vals[2:]]).reshape(H, W)
 
 
+def _write_dat(path, vals, xs, ys, zs):
+    """The .dat cube format DataCube::Load reads (DataCube.cpp:28-138): a title line, 8 comment
+    lines, "nx ny nz", the x, y and z coordinates, then the values x fastest, one row per line."""
+    nz, ny, nx = vals.shape
+    with open(path, "w") as f:
+        f.write("cube\n" + "#\n" * 8 + f"{nx} {ny} {nz}\n")
+        for c in (xs, ys, zs):
+            f.write(" ".join("%.9g" % v for v in c) + "\n")
+        for z in range(nz):
+            for y in range(ny):
+                f.write(" ".join("%.9g" % v for v in vals[z, y]) + "\n")
+
+
+def _read_fits(path):
+    """(header cards, H x W float32 image) of a primary-HDU FITS file, rows in file order."""
+    raw = open(path, "rb").read()
+    cards, k = {}, 0
+    while True:
+        c = raw[k:k + 80].decode()
+        k += 80
+        if c.startswith("END"):
+            break
+        key, _, rest = c.partition("=")
+        cards[key.strip()] = rest.split("/")[0].strip()
+    hdr = (k + 2879) // 2880 * 2880
+    W, H = int(cards["NAXIS1"]), int(cards["NAXIS2"])
+    return cards, np.frombuffer(raw[hdr:hdr + 4 * W * H], dtype=">f4").reshape(H, W).astype(np.float32)
+
+
+def _shell_dat(path, n, seed):
+    from birp_cpp_amd import capi
+    vals, org, spc, _ = capi.synth_shell(n, (8.0, 11.0), 0.05, seed=seed)
+    xs = (np.float32(org[0]) + np.float32(spc[0]) * np.arange(n, dtype=np.float32)).astype(np.float32)
+    ys = (np.float32(org[1]) + np.float32(spc[1]) * np.arange(n, dtype=np.float32)).astype(np.float32)
+    _write_dat(path, vals, xs, ys, ys)
+    # the coordinates the loader reads back define the cube's metadata
+    return O.OracleCube(vals, (float(xs[0]), float(ys[0]), float(ys[0])),
+                        (float(abs(xs[0] - xs[1])), float(abs(ys[0] - ys[1])), float(abs(ys[0] - ys[1]))))
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("semantics", [0, 1])
 def test_camera_render_bitwise(exe, device, tmp_path, semantics):
     """Camera::Render through the C++ API on a .dat cube equals the oracle bit for bit."""
-    from birp_cpp_amd import capi
-    vals, org, spc, _ = capi.synth_shell(64, (8.0, 11.0), 0.05, seed=11)
-    xs = (np.float32(org[0]) + np.float32(spc[0]) * np.arange(64, dtype=np.float32)).astype(np.float32)
-    ys = (np.float32(org[1]) + np.float32(spc[1]) * np.arange(64, dtype=np.float32)).astype(np.float32)
     p = tmp_path / "c.dat"
-    with open(p, "w") as f:
-        f.write("cube\n" + "#\n" * 8 + "64 64 64\n")
-        for c in (xs, ys, ys):
-            f.write(" ".join("%.9g" % v for v in c) + "\n")
-        for z in range(64):
-            for y in range(64):
-                f.write(" ".join("%.9g" % v for v in vals[z, y]) + "\n")
+    oc = _shell_dat(p, 64, 11)
     pos, aim = (12.0, 4.0, 6.0), 1.0
     rc, out, err = run(exe, "render", p, 1, 0.25, 0, 36.0, 36.0, *pos, aim, 44, 200, semantics)
     assert rc == 0, err
     img = _read_image(out)
-    # the coordinates the loader reads back define the cube's metadata
-    oc = O.OracleCube(vals, (float(xs[0]), float(ys[0]), float(ys[0])),
-                      (float(abs(xs[0] - xs[1])), float(abs(ys[0] - ys[1])), float(abs(ys[0] - ys[1]))))
     v = O.make_view(144, 144, 36.0, 36.0, list(pos), [aim, 0, 0], 44, 200)
     ref, _ = O.render_f32(oc, v, semantics)
     assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
@@ -209,3 +225,82 @@ def test_cmem_api_known_answers_and_render(exe, device):
 def test_render_of_other_space_types_is_refused(exe, device):
     rc, out, _ = run(exe, "unsupported_space")
     assert rc == 0 and out.startswith("invalid_argument")
+
+
+# ---------------------------------------------------------------------------------- the reference's CLI
+# oracle/_ref/birp_main is the reference's own src/main.cpp, compiled unchanged against include/ and
+# linked to libbirp_api.so (oracle/Makefile; built only where the reference sources are present).
+CLI = os.path.join(ROOT, "oracle", "_ref", "birp_main")
+needs_cli = pytest.mark.skipif(not os.path.exists(CLI), reason="oracle/_ref/birp_main not built (no reference sources)")
+
+
+def run_cli(*args, cwd=None):
+    r = subprocess.run([CLI, *map(str, args)], capture_output=True, text=True, cwd=cwd, timeout=120)
+    return r.returncode, r.stdout, r.stderr
+
+
+@needs_cli
+def test_reference_cli_links_this_library():
+    """The reference's main.cpp builds unchanged against the C++ API and loads libbirp_api.so;
+    without a GPU it fails loudly instead of falling back to a CPU render."""
+    out = subprocess.run(["ldd", CLI], capture_output=True, text=True).stdout
+    lib = [l for l in out.split("\n") if "libbirp_api.so" in l]
+    assert lib and os.path.realpath(LIB) in os.path.realpath(lib[0].split("=>")[1].split("(")[0].strip())
+    if not os.path.exists("/dev/kfd"):
+        rc, _, err = run_cli("-c", "-o", "/nonexistent/")
+        assert rc != 0 and "no GPU" in err
+
+
+@pytest.mark.gpu
+@needs_cli
+def test_reference_cli_datacube_render(device, tmp_path):
+    """main.cpp:266-305 single-cube mode: -i cube.dat -t, pose flags, default 0.25 deg pixels and
+    36 x 36 deg FOV; its FITS output equals the oracle's FP32 render bit for bit."""
+    oc = _shell_dat(tmp_path / "c.dat", 64, 12)
+    out = str(tmp_path) + "/"
+    rc, so, err = run_cli("-i", tmp_path / "c.dat", "-t", "-x", 12, "-y", 4, "-z", 6, "-a", 1, "-o", out)
+    assert rc == 0, so + err
+    cards, img = _read_fits(out + "c.dat.fits")
+    v = O.make_view(144, 144, 36.0, 36.0, [12.0, 4.0, 6.0], [1.0, 0, 0], 44, 200)
+    ref, _ = O.render_f32(oc, v, O.SEM_REFERENCE, nthreads=16)
+    assert float(cards["POS_X"]) == 12.0 and float(cards["AIM_X"]) == 1.0
+    assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.gpu
+@needs_cli
+def test_reference_cli_batch_directory(device, tmp_path):
+    """main.cpp:242-264 batch mode: every .dat in the input directory is rendered (nearest
+    sampling without -t) and written as <name>.fits; each equals the oracle bit for bit."""
+    d = tmp_path / "cubes"
+    d.mkdir()
+    ocs = {name: _shell_dat(d / name, n, seed) for name, n, seed in (("a.dat", 48, 21), ("b.dat", 40, 22))}
+    for oc in ocs.values():
+        oc.c.trilinear = 0
+    out = tmp_path / "out"
+    out.mkdir()
+    rc, so, err = run_cli("-i", str(d) + "/", "-x", 9, "-y", 2, "-z", 3, "-a", 0.5, "-s", 0.5, "-o", str(out) + "/")
+    assert rc == 0, so + err
+    v = O.make_view(72, 72, 36.0, 36.0, [9.0, 2.0, 3.0], [0.5, 0, 0], 44, 200)
+    for name, oc in ocs.items():
+        _, img = _read_fits(str(out / name) + ".fits")
+        ref, _ = O.render_f32(oc, v, O.SEM_REFERENCE, nthreads=16)
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), name
+
+
+@pytest.mark.gpu
+@needs_cli
+def test_reference_cli_cmem_default_render(device, tmp_path):
+    """main.cpp:276-305 with -c and no other flag: CMEM through the CLI's parameter Init
+    (p = {1,1,3,4}, the NULL floats 0), the default pose (5.91813, 6.74116, 17.7067) aiming at
+    7.81768, 0.25 deg pixels over 36 x 36 deg. Checked by the CMEM zone-edge rule."""
+    out = str(tmp_path) + "/"
+    rc, so, err = run_cli("-c", "-o", out)
+    assert rc == 0, so + err
+    _, img = _read_fits(out + "cmembirp.fits")
+    oc = O.OracleCmem(dict(dipole=0.0, p0=1.0, p1=1, p2=3, p3=4, B=0.0, alpha=0.0, beta=0.0, bs=0.0, A1=0.0, A2=0.0))
+    v = O.make_view(144, 144, 36.0, 36.0, [5.91813, 6.74116, 17.7067], [7.81768, 0, 0], 44, 200)
+    ref, _ = O.render_f32(oc, v, O.SEM_REFERENCE, nthreads=16)
+    ref64, _ = O.render_f64(oc, v, O.SEM_REFERENCE, nthreads=16)
+    cmem_render_parity(img, ref, ref64, oc.edge_pixels(v, CMEM_EDGE_BAND))
+
