@@ -50,18 +50,23 @@ def max_rel(a, b, floor_frac=1e-3):
     return float(np.max(np.abs(a[m] - b[m]) / np.abs(b[m]))) if m.any() else 0.0
 
 
-def read_traffic():
-    """HBM bytes per launch of the plain kernel from the committed rocprofv3 --pmc summary."""
-    p = os.path.join(ROOT, "profiles", "plain_exact_config2_pmc.txt")
+def read_profile_value(name, key):
+    """A `key: value` figure from a committed rocprofv3 summary under profiles/ (None if absent)."""
+    p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
         return None
     for line in open(p):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             try:
                 return float(line.split(":", 1)[1])
             except ValueError:
                 return None
     return None
+
+
+def read_traffic():
+    """HBM bytes per launch of the plain kernel from the committed rocprofv3 --pmc summary."""
+    return read_profile_value("plain_exact_config2_pmc.txt", "traffic_bytes_per_launch")
 
 
 def bench_plain(capi, W, dev, steps, warmup):
@@ -289,7 +294,10 @@ def main():
                    "in_cube_samples": n_in, "nominal_samples": v.width * v.height * v.samples},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                     "kernel_ms": round(kern_ms, 5), "algorithmic_bytes": int(alg_bytes)},
+                     "kernel_ms": round(kern_ms, 5), "algorithmic_bytes": int(alg_bytes),
+                     "valu": {"lane_ops_per_sample": read_profile_value("plain_exact_config2_valu_pmc.txt", "valu_lane_ops_per_sample"),
+                              "issue_util": read_profile_value("plain_exact_config2_valu_pmc.txt", "valu_issue_util"),
+                              "source": "profiles/plain_exact_config2_valu_pmc.txt"}},
         "forms": {"plain_renders_per_s": round(p["sync_rps"], 1),
                   "plain_renders_per_s_async": round(args.steps / p["wall"], 1)},
     }
