@@ -109,7 +109,7 @@ struct F64View {
 
 // Launch wrappers implemented in render.hip (host side of the HIP translation unit).
 namespace birp {
-int launch_exact(const DevVolume& v, const ExactView& ev, int lanes_per_ray, void* stream);
+int launch_exact(const DevVolume& v, const ExactView& ev, int lanes_per_ray, int pipe, void* stream);
 int launch_f64(const DevVolume& v, const F64View& fv, void* stream);
 int launch_sample_points(const DevVolume& v, const float* xyz, long long n, float* out, void* stream);
 int launch_fast_batch(const DevVolume* vols_dev, int dtype, const FastPose* poses_dev, const int* tile_prefix_dev,

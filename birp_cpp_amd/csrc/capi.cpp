@@ -172,7 +172,8 @@ bool take_injected_failure(birp_device* dev) {
     return false;
 }
 
-// Tuning knobs of the exact path, read once: BIRP_EXACT_LANES (lanes per ray: 1, 2, 4) and
+// Tuning knobs of the exact path, read once: BIRP_EXACT_LANES (lanes per ray: 1, 2, 4),
+// BIRP_EXACT_PIPE (samples in flight per lane in the positive-cube loop: 1..4) and
 // BIRP_EXACT_LAYOUT ("linear" reads the [z][y][x] copy instead of the quad bricks).
 int default_exact_lanes() {
     static int lanes = [] {
@@ -181,6 +182,14 @@ int default_exact_lanes() {
         return (v == 1 || v == 2 || v == 4) ? v : 2;
     }();
     return lanes;
+}
+int default_exact_pipe() {
+    static int pipe = [] {
+        const char* e = getenv("BIRP_EXACT_PIPE");
+        int v = e ? atoi(e) : 2;
+        return (v >= 1 && v <= 4) ? v : 2;
+    }();
+    return pipe;
 }
 bool exact_force_linear() {
     static bool lin = [] {
@@ -558,7 +567,7 @@ static int render_exact_into(birp_volume* vol, const birp_view* view, float* dim
     if (take_injected_failure(dev)) return fail(BIRP_ERR_INJECTED, "injected failure");
     DevVolume dv = vol->dv;
     if (exact_force_linear() && dv.vclass != VC_NONFINITE && dv.kind == SK_CUBE) dv.vclass = VC_NONFINITE;
-    rc = launch_exact(dv, ev, default_exact_lanes(), dev->stream);
+    rc = launch_exact(dv, ev, default_exact_lanes(), default_exact_pipe(), dev->stream);
     return rc ? fail(rc, "exact render launch failed") : BIRP_OK;
 }
 

@@ -179,20 +179,69 @@ struct QuadCtx {
 // conversion of the clamped index (the clamp only matters to lanes outside the cube, whose
 // result is discarded) and the weight x - floor(x) is v_fract_f32 (exact: the subtraction of
 // floor(x) from x >= 0 is exact, so fract's clamp below 1.0 never applies).
+struct QuadFetch {
+    float4 a, b;       // the z and z+1 quad records
+    float fx, fy, fz;  // lerp weights
+};
+
+__device__ __forceinline__ QuadFetch quad_fetch(const QuadCtx& c, float xd, float yd, float zd) {
+    QuadFetch q;
+    q.fx = __builtin_amdgcn_fractf(xd);
+    q.fy = __builtin_amdgcn_fractf(yd);
+    q.fz = __builtin_amdgcn_fractf(zd);
+    unsigned off = c.tx[to_index(xd, c.hx)] + c.ty[to_index(yd, c.hy)] + c.tz[to_index(zd, c.hz)];
+    q.a = load16(c.rs, off);
+    q.b = load16(c.rs, off + 1024u);
+    return q;
+}
+
+// x first, then y, then z (DataCube.cpp:260-270).
+template <bool POS>
+__device__ __forceinline__ float quad_lerp(const QuadFetch& q) {
+    const float4 a = q.a, b = q.b;
+    if constexpr (POS) {
+        float ly = lerp_positive(lerp_positive(a.x, a.y, q.fx), lerp_positive(a.z, a.w, q.fx), q.fy);
+        float hy = lerp_positive(lerp_positive(b.x, b.y, q.fx), lerp_positive(b.z, b.w, q.fx), q.fy);
+        return lerp_positive(ly, hy, q.fz);
+    } else {
+        float ly = lerp_general(lerp_general(a.x, a.y, q.fx), lerp_general(a.z, a.w, q.fx), q.fy);
+        float hy = lerp_general(lerp_general(b.x, b.y, q.fx), lerp_general(b.z, b.w, q.fx), q.fy);
+        return lerp_general(ly, hy, q.fz);
+    }
+}
+
 template <bool POS>
 __device__ __forceinline__ float sample_quad(const QuadCtx& c, float xd, float yd, float zd) {
-    float fx = __builtin_amdgcn_fractf(xd), fy = __builtin_amdgcn_fractf(yd), fz = __builtin_amdgcn_fractf(zd);
-    unsigned off = c.tx[to_index(xd, c.hx)] + c.ty[to_index(yd, c.hy)] + c.tz[to_index(zd, c.hz)];
-    float4 a = load16(c.rs, off);
-    float4 b = load16(c.rs, off + 1024u);
-    if constexpr (POS) {
-        float ly = lerp_positive(lerp_positive(a.x, a.y, fx), lerp_positive(a.z, a.w, fx), fy);
-        float hy = lerp_positive(lerp_positive(b.x, b.y, fx), lerp_positive(b.z, b.w, fx), fy);
-        return lerp_positive(ly, hy, fz);
+    return quad_lerp<POS>(quad_fetch(c, xd, yd, zd));
+}
+
+// The scale chain of Camera.cpp:201 for a positive cube whose scaled samples stay normal floats:
+// the folded product, unless it lies within 16 double ulps of a float rounding midpoint.
+// The chain is taken by about one sample in 10^7: it must stay a branch (if-converted, its two
+// FP64 divisions would run for every sample).
+// The test |m29 - 2^28| <= 16 on the low 29 mantissa bits m29 is done on m29 * 8 (the low word
+// shifted left by 3): m29 * 8 + 2^31 + 128 wraps into [0, 256] exactly when it holds.
+__device__ __forceinline__ float scale_pos(float vw) {
+    double y = (double)vw * BIRP_SCALE_FOLDED;
+    const uint32_t lo = (uint32_t)birp_f64_bits(y);
+    float s = (float)y;
+    if (__builtin_expect((lo << 3) + 0x80000080u < 257u, 0)) s = birp_scale_chain(vw);
+    return s;
+}
+
+// acc += the S sub-lane samples of this ray, in sub-lane (= k) order.
+template <int S>
+__device__ __forceinline__ float add_in_order(float acc, float s) {
+    if constexpr (S == 1) {
+        return acc + s;
     } else {
-        float ly = lerp_general(lerp_general(a.x, a.y, fx), lerp_general(a.z, a.w, fx), fy);
-        float hy = lerp_general(lerp_general(b.x, b.y, fx), lerp_general(b.z, b.w, fx), fy);
-        return lerp_general(ly, hy, fz);
+        acc = acc + quad_bcast_f<S, 0>(s);
+        acc = acc + quad_bcast_f<S, 1>(s);
+        if constexpr (S == 4) {
+            acc = acc + quad_bcast_f<S, 2>(s);
+            acc = acc + quad_bcast_f<S, 3>(s);
+        }
+        return acc;
     }
 }
 
@@ -314,7 +363,7 @@ enum SampleMode { SM_QUAD_POS = 0, SM_QUAD_GEN = 1, SM_LIN_TRI = 2, SM_LIN_NEARE
 // ------------------------------------------------------------------------------------------
 // the exact render kernel
 // ------------------------------------------------------------------------------------------
-template <int S, int MODE>
+template <int S, int MODE, int P = 1>
 __global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact(DevVolume v, ExactView ev, int wg_x, int nblocks) {
     constexpr bool QUAD = (MODE == SM_QUAD_POS || MODE == SM_QUAD_GEN);
     constexpr bool POS = (MODE == SM_QUAD_POS);
@@ -377,15 +426,36 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact(DevVolume v, ExactV
     const float w = ev.w;
     const float px = ev.px, py = ev.py, pz = ev.pz;
 
-    for (int kb = 0; kb < trips; kb += S) {
+    if constexpr (POS) {
+        // P samples per lane per iteration (k, k + S, ...), staged so that the later samples' LDS
+        // and buffer reads are in flight while the first ones' lerps run. The sum order is
+        // unchanged: the S samples at k, then the S samples at k + S, and so on.
+        for (int kb = 0; kb < trips; kb += P * S) {
+            // one clamp per iteration: past the clamp, a lane's later t reads land at most
+            // (P-1)*S <= 12 floats beyond t[n-1], in the offset tables that follow or the 16
+            // floats of padding after them (the lane is inactive then; its sample is discarded)
+            const float* tp = (const float*)((const char*)s_t + min(t_first + (unsigned)kb * 4u, t_last));
+            QuadFetch q[P];
+#pragma unroll
+            for (int u = 0; u < P; ++u) {
+                const float t = tp[u * S];
+                const ExactIdx r = exact_index<false>(v, dx, dy, dz, px, py, pz, t);
+                q[u] = quad_fetch(qc, r.xd, r.yd, r.zd);
+            }
+#pragma unroll
+            for (int u = 0; u < P; ++u) {
+                float sc = scale_pos(quad_lerp<true>(q[u]) * w);  // Camera.cpp:201
+                sc = (kb + u * S < lim) ? sc : -0.0f;
+                acc = add_in_order<S>(acc, sc);
+            }
+        }
+    }
+    for (int kb = 0; !POS && kb < trips; kb += S) {
         const float t = *(const float*)((const char*)s_t + min(t_first + (unsigned)kb * 4u, t_last));
         const bool act = kb < lim;
         float val;
         if constexpr (MODE == SM_CMEM) {
             val = cmem_sample(v.cmem, dx * t + px, dy * t + py, dz * t + pz);
-        } else if constexpr (POS) {
-            ExactIdx r = exact_index<false>(v, dx, dy, dz, px, py, pz, t);
-            val = sample_quad<true>(qc, r.xd, r.yd, r.zd);
         } else {
             ExactIdx r = exact_index<true>(v, dx, dy, dz, px, py, pz, t);
             if constexpr (MODE == SM_QUAD_GEN) val = sample_quad<false>(qc, r.xd, r.yd, r.zd);
@@ -393,22 +463,7 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact(DevVolume v, ExactV
             else val = sample_linear_nearest(v, r);
         }
         const float vw = val * w;  // Camera.cpp:201
-        if constexpr (POS) {
-            double y = (double)vw * BIRP_SCALE_FOLDED;
-            const uint32_t m29 = (uint32_t)birp_f64_bits(y) & 0x1FFFFFFFu;
-            float s = ((m29 - 0x0FFFFFF0u) < 0x21u) ? birp_scale_chain(vw) : (float)y;
-            s = act ? s : -0.0f;
-            if constexpr (S == 1) {
-                acc = acc + s;
-            } else {
-                acc = acc + quad_bcast_f<S, 0>(s);
-                acc = acc + quad_bcast_f<S, 1>(s);
-                if constexpr (S == 4) {
-                    acc = acc + quad_bcast_f<S, 2>(s);
-                    acc = acc + quad_bcast_f<S, 3>(s);
-                }
-            }
-        } else {
+        {
             const float s = birp_scale_sample(vw);
             if constexpr (S == 1) {
                 bool add = act && alive;
@@ -862,7 +917,7 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
 static int hip_status(hipError_t e) { return e == hipSuccess ? BIRP_OK : BIRP_ERR_DEVICE; }
 
 template <int S>
-static int launch_exact_s(const DevVolume& v, const ExactView& ev, hipStream_t st) {
+static int launch_exact_s(const DevVolume& v, const ExactView& ev, int pipe, hipStream_t st) {
     constexpr int TW = ExactTile<S>::TW, TH = ExactTile<S>::TH;
     int wg_x = (ev.W + 2 * TW - 1) / (2 * TW);
     int wg_y = (ev.H + 2 * TH - 1) / (2 * TH);
@@ -876,9 +931,15 @@ static int launch_exact_s(const DevVolume& v, const ExactView& ev, hipStream_t s
     else mode = SM_QUAD_GEN;
     size_t lds = (size_t)((ev.n + 3) & ~3) * sizeof(float);
     if (mode == SM_QUAD_POS || mode == SM_QUAD_GEN) lds += (size_t)(v.nx + v.ny + v.nz) * sizeof(unsigned);
+    if (mode == SM_QUAD_POS) lds += 16 * sizeof(float);  // room for the positive loop's unclamped t reads
     if (lds > 64 * 1024) return BIRP_ERR_UNSUPPORTED;
     switch (mode) {
-        case SM_QUAD_POS: hipLaunchKernelGGL((k_render_exact<S, SM_QUAD_POS>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks); break;
+        case SM_QUAD_POS:
+            if (pipe >= 4) hipLaunchKernelGGL((k_render_exact<S, SM_QUAD_POS, 4>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks);
+            else if (pipe == 3) hipLaunchKernelGGL((k_render_exact<S, SM_QUAD_POS, 3>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks);
+            else if (pipe == 1) hipLaunchKernelGGL((k_render_exact<S, SM_QUAD_POS, 1>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks);
+            else hipLaunchKernelGGL((k_render_exact<S, SM_QUAD_POS, 2>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks);
+            break;
         case SM_QUAD_GEN: hipLaunchKernelGGL((k_render_exact<S, SM_QUAD_GEN>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks); break;
         case SM_LIN_TRI: hipLaunchKernelGGL((k_render_exact<S, SM_LIN_TRI>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks); break;
         case SM_LIN_NEAREST: hipLaunchKernelGGL((k_render_exact<S, SM_LIN_NEAREST>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks); break;
@@ -887,12 +948,12 @@ static int launch_exact_s(const DevVolume& v, const ExactView& ev, hipStream_t s
     return hip_status(hipGetLastError());
 }
 
-int launch_exact(const DevVolume& v, const ExactView& ev, int lanes_per_ray, void* stream) {
+int launch_exact(const DevVolume& v, const ExactView& ev, int lanes_per_ray, int pipe, void* stream) {
     if (ev.n < 2 || ev.n > BIRP_MAX_EXACT_SAMPLES || ev.W <= 0 || ev.H <= 0) return BIRP_ERR_INVALID_ARGUMENT;
     hipStream_t st = (hipStream_t)stream;
-    if (lanes_per_ray == 4) return launch_exact_s<4>(v, ev, st);
-    if (lanes_per_ray == 2) return launch_exact_s<2>(v, ev, st);
-    return launch_exact_s<1>(v, ev, st);
+    if (lanes_per_ray == 4) return launch_exact_s<4>(v, ev, pipe, st);
+    if (lanes_per_ray == 2) return launch_exact_s<2>(v, ev, pipe, st);
+    return launch_exact_s<1>(v, ev, pipe, st);
 }
 
 int launch_f64(const DevVolume& v, const F64View& fv, void* stream) {
