@@ -61,7 +61,9 @@ struct DevVolume {
     unsigned quad_bytes;  // buffer range of the quad records (hardware bounds check)
     unsigned oct_bytes;
     int origin_far;       // every |origin| >= 2^-30: a nonzero ingress is never below 2^-56
-    float vmin, vmax;     // value range (positive volumes)
+    float vmin, vmax;     // smallest and largest |v| over the nonzero finite values
+    int f16_uniform;      // F16 records all share one power-of-two scale (f16_scale)
+    float f16_scale;      // that scale: records hold v / f16_scale
     CmemParams cmem;
 };
 
@@ -120,5 +122,13 @@ int launch_zero_f32(float* p, size_t n, void* stream);
 int launch_scan_class(const float* values, size_t n, unsigned* flags_dev, void* stream);
 int launch_build_quad(const DevVolume& v, void* quad, void* stream);
 int launch_build_oct(const DevVolume& v, void* oct, float* bscale, void* stream);
+
+// Fast-path kernel variants: the storage dtype (BIRP_DTYPE_*), or F16 whose records share one
+// scale for the whole volume, which the kernel then applies once per ray instead of per sample.
+#define BIRP_KDT_F16_UNIFORM 3
+#define BIRP_KDT_COUNT 4
+inline int fast_kernel_dtype(const DevVolume& d) {
+    return (d.dtype == BIRP_DTYPE_F16 && d.f16_uniform) ? BIRP_KDT_F16_UNIFORM : d.dtype;
+}
 int fast_grid_size(int dtype, int lanes_per_ray, int unroll);
 }  // namespace birp

@@ -128,9 +128,9 @@ int fast_pose(const birp_volume* vol, const birp_view* v, int vol_index, float* 
     fp.vol = vol_index;
     fp.tiles_x = (v->width + tile_w - 1) / tile_w;
     fp.w = (float)((double)v->ray_length / v->samples);
+    if (fast_kernel_dtype(vol->dv) == BIRP_KDT_F16_UNIFORM) fp.w *= vol->dv.f16_scale;  // power of two: exact
     fp.pad = 0.f;
     fp.out = out_dev;
-    (void)vol;
     (void)tile_h;
     return BIRP_OK;
 }
@@ -466,6 +466,18 @@ extern "C" int birp_volume_upload(birp_device* dev, const birp_volume_desc* desc
         d.otab = v->otab;
         d.oct_bytes = (unsigned)oct_bytes;
         d.bscale = v->bscale;
+        if (desc->dtype == BIRP_DTYPE_F16) {
+            // One power-of-two scale for the whole volume when every nonzero |v| then lands in
+            // F16's normal range [2^-14, 2^15): no per-sample scale lookup in the kernel.
+            double s = 1.0;
+            if (d.vmax > 0.f) {
+                int ex = 0;
+                std::frexp((double)d.vmax, &ex);  // vmax = f * 2^ex, f in [0.5, 1)
+                s = std::ldexp(1.0, ex - 15);     // vmax / s < 2^15
+            }
+            d.f16_uniform = (d.vmax == 0.f || (double)d.vmin / s >= 0x1p-14) && s >= 0x1p-126 && s <= 0x1p120;
+            d.f16_scale = d.f16_uniform ? (float)s : 1.f;
+        }
         if ((rc = launch_build_oct(d, v->oct, v->bscale, dev->stream)) != BIRP_OK) goto fail_out;
     }
     if ((e = hipStreamSynchronize(dev->stream)) != hipSuccess) { rc = hip_fail(e, "layout build"); goto fail_out; }
@@ -788,13 +800,13 @@ int batch_setup(birp_batch* b) {
     if (e == hipSuccess) e = hipMemcpyAsync(b->d_vols, &b->vol->dv, sizeof(DevVolume), hipMemcpyHostToDevice, dev->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
     if (e != hipSuccess) return hip_fail(e, "batch upload");
-    b->grid = fast_grid_size(b->vol->desc.dtype, t.lanes_per_ray, t.unroll);
+    b->grid = fast_grid_size(fast_kernel_dtype(b->vol->dv), t.lanes_per_ray, t.unroll);
     // capture: reset the work counter (a kernel, not a memset node), then the persistent kernel
     hipStream_t cs = b->capture_stream;
     if ((e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal)) != hipSuccess) return hip_fail(e, "begin capture");
     int rc = launch_reset_counter(b->d_work, cs);
     if (rc == BIRP_OK)
-        rc = launch_fast_batch(b->d_vols, b->vol->desc.dtype, b->d_poses, b->d_prefix, n, b->total_tiles, b->d_work, t.tile_w,
+        rc = launch_fast_batch(b->d_vols, fast_kernel_dtype(b->vol->dv), b->d_poses, b->d_prefix, n, b->total_tiles, b->d_work, t.tile_w,
                                t.tile_h, t.lanes_per_ray, t.unroll, b->grid, cs);
     e = hipStreamEndCapture(cs, &b->graph);
     if (rc != BIRP_OK) return fail(rc, "batch capture launch failed");
@@ -947,7 +959,7 @@ static int render_fast_into(birp_volume* vol, const birp_view* view, float* dimg
     if (e != hipSuccess) return hip_fail(e, "fast render upload");
     if (take_injected_failure(dev)) return fail(BIRP_ERR_INJECTED, "injected failure");
     if ((rc = launch_reset_counter(dwork, dev->stream))) return rc;
-    int grid = std::min(fast_grid_size(vol->desc.dtype, t.lanes_per_ray, t.unroll), (prefix[1] + 3) / 4 + 1);
-    return launch_fast_batch(dv, vol->desc.dtype, dp, dpre, 1, prefix[1], dwork, t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll,
+    int grid = std::min(fast_grid_size(fast_kernel_dtype(vol->dv), t.lanes_per_ray, t.unroll), (prefix[1] + 3) / 4 + 1);
+    return launch_fast_batch(dv, fast_kernel_dtype(vol->dv), dp, dpre, 1, prefix[1], dwork, t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll,
                              grid, dev->stream);
 }

@@ -619,15 +619,16 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_sample_points(DevVolume v, const
 // layout builders and scans
 // ------------------------------------------------------------------------------------------
 // flags[0]: bit 0 = some value <= 0, bit 1 = some value non-finite; flags[1], flags[2]: min and
-// max of the positive finite values as float bits (ordered as unsigned for positive floats).
+// max of |v| over the nonzero finite values as float bits (ordered as unsigned for positive floats).
 __global__ __launch_bounds__(BIRP_BLOCK) void k_scan_class(const float* vals, size_t n, unsigned* flags) {
     unsigned f = 0, mn = 0x7F7FFFFFu, mx = 0u;
     for (size_t idx = (size_t)blockIdx.x * BIRP_BLOCK + threadIdx.x; idx < n; idx += (size_t)gridDim.x * BIRP_BLOCK) {
         float x = vals[idx];
         if (!(x > 0.f)) f |= 1u;
         if (!isfinite(x)) f |= 2u;
-        if (x > 0.f && isfinite(x)) {
-            unsigned u = __float_as_uint(x);
+        float a = fabsf(x);
+        if (a > 0.f && isfinite(a)) {
+            unsigned u = __float_as_uint(a);
             mn = min(mn, u);
             mx = max(mx, u);
         }
@@ -675,9 +676,18 @@ __device__ __forceinline__ unsigned short f32_to_f16_bits(float x) {
     return *reinterpret_cast<unsigned short*>(&h);
 }
 
-// Per-brick power-of-two scale for F16: max|v| over the brick's corners maps to < 2^15.
+// Per-brick power-of-two scale for F16: max|v| over the brick's corners maps to < 2^15. With
+// v.f16_uniform every brick gets the volume's scale v.f16_scale instead.
 __global__ __launch_bounds__(BIRP_BLOCK) void k_brick_scale(DevVolume v, float* bscale) {
     int brick = blockIdx.x;
+    if (v.f16_uniform) {
+        if (threadIdx.x == 0) {
+            int bx = brick % v.nbx, by = (brick / v.nbx) % v.nby, bz = brick / (v.nbx * v.nby);
+            const unsigned* t = v.otab;
+            bscale[(t[bx * 8] + t[v.nx + by * 8] + t[v.nx + v.ny + bz * 8]) >> 13] = v.f16_scale;
+        }
+        return;
+    }
     int bx = brick % v.nbx, by = (brick / v.nbx) % v.nby, bz = brick / (v.nbx * v.nby);
     float mx = 0.f;
     for (int t = threadIdx.x; t < 729; t += BIRP_BLOCK) {
@@ -752,16 +762,16 @@ __device__ __forceinline__ float flerp(float a, float b, float t) { return __bui
 template <int DT>
 __device__ __forceinline__ float fast_sample(const DevVolume& v, __amdgpu_buffer_rsrc_t rs, const unsigned* tab, float xd,
                                              float yd, float zd) {
-    float x0f = floorf(xd), y0f = floorf(yd), z0f = floorf(zd);
-    float fx = xd - x0f, fy = yd - y0f, fz = zd - z0f;
-    int ix = to_index(x0f, v.fnx - 2.0f), iy = to_index(y0f, v.fny - 2.0f), iz = to_index(z0f, v.fnz - 2.0f);
+    // xd in [0, n-1): the cell is the truncation and the weight the fraction
+    float fx = __builtin_amdgcn_fractf(xd), fy = __builtin_amdgcn_fractf(yd), fz = __builtin_amdgcn_fractf(zd);
+    int ix = to_index(xd, v.fnx - 2.0f), iy = to_index(yd, v.fny - 2.0f), iz = to_index(zd, v.fnz - 2.0f);
     unsigned off = tab[ix] + tab[v.nx + iy] + tab[v.nx + v.ny + iz];
     if constexpr (DT == BIRP_DTYPE_F32) {
         float4 a = load16(rs, off), b = load16(rs, off + 1024u);
         float ly = flerp(flerp(a.x, a.y, fx), flerp(a.z, a.w, fx), fy);
         float hy = flerp(flerp(b.x, b.y, fx), flerp(b.z, b.w, fx), fy);
         return flerp(ly, hy, fz);
-    } else {
+    } else {  // BF16, F16 with per-brick scale, F16 with one scale (BIRP_KDT_F16_UNIFORM)
         uint4 o = load16u(rs, off);
         float c0, c1, c2, c3, c4, c5, c6, c7;
         if constexpr (DT == BIRP_DTYPE_BF16) {
@@ -881,7 +891,11 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
         const int cnt = (kl >= kf) ? (kl - kf + 1) : 0;
         const int my = (cnt > sub) ? (cnt - sub + S - 1) / S : 0;  // samples of this sub-lane
         const int trips = wave_max_i(my);
-        const float hx = V.fnx - 1.0f, hy = V.fny - 1.0f, hz = V.fnz - 1.0f;
+        // the largest floats below n - 1: a sample rounded onto the far face keeps weight ~1
+        // on the last grid point instead of falling into the cell before it
+        const float hx = __uint_as_float(__float_as_uint(V.fnx - 1.0f) - 1u);
+        const float hy = __uint_as_float(__float_as_uint(V.fny - 1.0f) - 1u);
+        const float hz = __uint_as_float(__float_as_uint(V.fnz - 1.0f) - 1u);
         float sum = 0.f, comp = 0.f;
         for (int q = 0; q < trips; q += U) {
 #pragma unroll
@@ -1046,6 +1060,8 @@ int launch_fast_batch(const DevVolume* vols_dev, int dtype, const FastPose* pose
     if ((1 << twl) != tile_w || tile_w * tile_h * lanes_per_ray != 64) return BIRP_ERR_INVALID_ARGUMENT;
     grid = ((grid + 7) / 8) * 8;
     hipStream_t st = (hipStream_t)stream;
+    if (dtype == BIRP_KDT_F16_UNIFORM)
+        return launch_fast_d<BIRP_KDT_F16_UNIFORM>(lanes_per_ray, unroll, grid, st, vols_dev, poses_dev, tile_prefix_dev, npose, total_tiles, work_counter, twl);
     if (dtype == BIRP_DTYPE_F16)
         return launch_fast_d<BIRP_DTYPE_F16>(lanes_per_ray, unroll, grid, st, vols_dev, poses_dev, tile_prefix_dev, npose, total_tiles, work_counter, twl);
     if (dtype == BIRP_DTYPE_BF16)
@@ -1065,7 +1081,8 @@ int fast_grid_size(int dtype, int lanes_per_ray, int unroll) {
     if (lanes_per_ray == 4) fn = unroll == 4 ? fast_kernel_ptr<DT, 4, 4>() : unroll == 2 ? fast_kernel_ptr<DT, 4, 2>() : fast_kernel_ptr<DT, 4, 1>(); \
     else if (lanes_per_ray == 2) fn = unroll == 4 ? fast_kernel_ptr<DT, 2, 4>() : unroll == 2 ? fast_kernel_ptr<DT, 2, 2>() : fast_kernel_ptr<DT, 2, 1>(); \
     else fn = unroll == 4 ? fast_kernel_ptr<DT, 1, 4>() : unroll == 2 ? fast_kernel_ptr<DT, 1, 2>() : fast_kernel_ptr<DT, 1, 1>();
-    if (dtype == BIRP_DTYPE_F16) { BIRP_FK(BIRP_DTYPE_F16) }
+    if (dtype == BIRP_KDT_F16_UNIFORM) { BIRP_FK(BIRP_KDT_F16_UNIFORM) }
+    else if (dtype == BIRP_DTYPE_F16) { BIRP_FK(BIRP_DTYPE_F16) }
     else if (dtype == BIRP_DTYPE_BF16) { BIRP_FK(BIRP_DTYPE_BF16) }
     else { BIRP_FK(BIRP_DTYPE_F32) }
 #undef BIRP_FK

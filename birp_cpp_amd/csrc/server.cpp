@@ -124,18 +124,18 @@ int launch_slot(birp_server* S, Slot& s) {
     int rc = slot_reserve_images(s, total);
     if (rc) return rc;
     const birp_tuning& t = S->tuning;
-    // group by dtype: poses of group g occupy [g*nb, g*nb + count_g)
-    int counts[3] = {0, 0, 0};
+    // group by kernel variant: poses of group g occupy [g*nb, g*nb + count_g)
+    int counts[BIRP_KDT_COUNT] = {0, 0, 0, 0};
     for (size_t k = 0; k < s.reqs.size(); k++) {
         Request* r = s.reqs[k];
         birp_volume* vol = r->vol;
-        int g = vol->desc.dtype;
+        int g = fast_kernel_dtype(vol->dv);
         FastPose& fp = s.h_poses[g * nb + counts[g]];
         rc = fast_pose(vol, &r->req.view, r->req.volume, s.d_images + s.offsets[k], t.tile_w, t.tile_h, fp);
         if (rc) return rc;
         counts[g]++;
     }
-    for (int g = 0; g < 3; g++) {
+    for (int g = 0; g < BIRP_KDT_COUNT; g++) {
         int* pre = s.h_prefix + g * (nb + 1);
         pre[0] = 0;
         for (int k = 0; k < counts[g]; k++) {
@@ -146,10 +146,10 @@ int launch_slot(birp_server* S, Slot& s) {
             pre[k + 1] = pre[k] + tiles_of(tmp, t.tile_w, t.tile_h);
         }
     }
-    hipError_t e = hipMemcpyAsync(s.d_poses, s.h_poses, sizeof(FastPose) * 3 * nb, hipMemcpyHostToDevice, s.stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(s.d_prefix, s.h_prefix, sizeof(int) * 3 * (nb + 1), hipMemcpyHostToDevice, s.stream);
+    hipError_t e = hipMemcpyAsync(s.d_poses, s.h_poses, sizeof(FastPose) * BIRP_KDT_COUNT * nb, hipMemcpyHostToDevice, s.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s.d_prefix, s.h_prefix, sizeof(int) * BIRP_KDT_COUNT * (nb + 1), hipMemcpyHostToDevice, s.stream);
     if (e != hipSuccess) return hip_fail(e, "server descriptor upload");
-    for (int g = 0; g < 3; g++) {
+    for (int g = 0; g < BIRP_KDT_COUNT; g++) {
         if (!counts[g]) continue;
         int total_tiles = s.h_prefix[g * (nb + 1) + counts[g]];
         if ((rc = launch_reset_counter(s.d_work + g * 256, s.stream))) return rc;
@@ -333,11 +333,11 @@ extern "C" int birp_server_create(birp_device* dev, const birp_server_config* cf
         if (rc) break;
         if ((e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking)) != hipSuccess) { rc = hip_fail(e, "stream"); break; }
         if ((e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming)) != hipSuccess) { rc = hip_fail(e, "event"); break; }
-        if ((e = hipHostMalloc(&s.h_poses, sizeof(FastPose) * 3 * nb, hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
-        if ((e = hipMalloc(&s.d_poses, sizeof(FastPose) * 3 * nb)) != hipSuccess) { rc = hip_fail(e, "poses"); break; }
-        if ((e = hipHostMalloc(&s.h_prefix, sizeof(int) * 3 * (nb + 1), hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
-        if ((e = hipMalloc(&s.d_prefix, sizeof(int) * 3 * (nb + 1))) != hipSuccess) { rc = hip_fail(e, "prefix"); break; }
-        if ((e = hipMalloc(&s.d_work, sizeof(unsigned) * 3 * 256)) != hipSuccess) { rc = hip_fail(e, "counter"); break; }
+        if ((e = hipHostMalloc(&s.h_poses, sizeof(FastPose) * BIRP_KDT_COUNT * nb, hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
+        if ((e = hipMalloc(&s.d_poses, sizeof(FastPose) * BIRP_KDT_COUNT * nb)) != hipSuccess) { rc = hip_fail(e, "poses"); break; }
+        if ((e = hipHostMalloc(&s.h_prefix, sizeof(int) * BIRP_KDT_COUNT * (nb + 1), hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
+        if ((e = hipMalloc(&s.d_prefix, sizeof(int) * BIRP_KDT_COUNT * (nb + 1))) != hipSuccess) { rc = hip_fail(e, "prefix"); break; }
+        if ((e = hipMalloc(&s.d_work, sizeof(unsigned) * BIRP_KDT_COUNT * 256)) != hipSuccess) { rc = hip_fail(e, "counter"); break; }
     }
     if (rc) {
         birp_server_destroy(S);

@@ -100,6 +100,32 @@ def test_fast_path_vs_fp64(cube1, oracle, dtype, tol):
         assert err <= tol
 
 
+def test_f16_scale_modes(device):
+    """F16 storage keeps one power-of-two scale for the whole volume when every nonzero |v| then
+    lies in F16's normal range, and a scale per 8^3 brick otherwise. A cube spanning 30 decades
+    (v = 10^(-30 x/X), far beyond F16's 2^29 range) takes the per-brick path; its value range
+    scaled into one brick still fits, so both paths meet the F16 tolerance against FP64."""
+    n = 64
+    x = np.arange(n, dtype=np.float64)
+    wide = np.broadcast_to(10.0 ** (-30.0 * x / (n - 1)), (n, n, n)).astype(np.float32).copy()
+    narrow = np.broadcast_to(1e-6 * (1.0 + 0.5 * np.sin(x / 5.0)), (n, n, n)).astype(np.float32).copy()
+    org, spc = (-10.0, -15.0, -15.0), (30 / (n - 1),) * 3
+    for vals in (wide, narrow):
+        v32 = capi.Volume.upload(device, vals, org, spc, dtype=capi.DTYPE_F32)
+        v16 = capi.Volume.upload(device, vals, org, spc, dtype=capi.DTYPE_F16)
+        try:
+            for pose in [(15.0, 5.0, 8.0), (3.0, -7.0, 2.0)]:
+                view = W.view(96, 64, 0.25, pos=pose, math=capi.MATH_FAST)
+                ref = v32.render_f64(view)
+                img, _ = v16.render(view)
+                err = rel_l2(img, ref)
+                print(f"range {vals.max() / vals.min():.1e} pose {pose}: F16 rel L2 {err:.3e}")
+                assert err <= 1e-4
+        finally:
+            v32.free()
+            v16.free()
+
+
 def test_fast_path_reference_semantics(cube1, oracle):
     vals, org, spc, vols, oc = cube1
     v = W.view(96, 64, 0.25, pos=(15.0, 5.0, 8.0), aim=(0.0, 0.0, 0.0), semantics=capi.SEM_REFERENCE, math=capi.MATH_FAST, length=44.0)
