@@ -9,12 +9,17 @@ Also measured in the same run (keys under "forms"):
   batched               256 orbit poses, mixed 144x84/288x168/576x336 images, steps 0.1/0.25 RE,
                         F16 storage, one persistent-kernel launch replayed from a HIP graph; plain
                         renders/s of the same 256 views for the ratio
-  precision             512^3 cube stored FP32/BF16/F16 (per-brick scale), 576x336, step 0.05 RE:
-                        relative L2 and max relative error of every image against the FP64 render
+  cmem                  the analytic CMEM model (default Init) on the same 576x336 x 800 grid:
+                        samples/s of the exact path (VALU-bound) and its error against FP64
+  precision             512^3 cube stored FP32/BF16/F16 (one or per-brick power-of-two scale),
+                        576x336, step 0.05 RE: relative L2 and max relative error of every image
+                        against the FP64 render
   serving               Poisson requests at 200/s over 8 resident cubes: p50/p99 latency, renders/s;
                         the same requests as one burst (saturated server) and through the plain path
 The CPU baseline is the oracle's FP32 restatement of the reference render (a port: the reference's
-own Camera.cpp cannot be compiled here) on the same 576x336 image, on the host's cores.
+own Camera.cpp cannot be compiled here) on the same 576x336 image, on the host's cores; its image
+also checks the headline render bit for bit ("error": "vs_oracle_fp32"), beside the error against
+the FP64 render of the same sample grid.
 """
 from __future__ import annotations
 
@@ -101,12 +106,12 @@ def cpu_baseline(O, W, capi, vals, org, spc, v, n_in):
     oc = O.OracleCube(vals, org, spc)
     ov = O.make_view(v.width, v.height, v.fov_w, v.fov_h, list(v.position), list(v.aim), v.ray_length, v.samples)
     t = time.perf_counter()
-    _, cnt = O.render_f32(oc, ov, O.SEM_EXTENDED, nthreads=nthreads)
+    ref, cnt = O.render_f32(oc, ov, O.SEM_EXTENDED, nthreads=nthreads)
     el = time.perf_counter() - t
     assert cnt == n_in
     return {"value": round(cnt / el / 1e9, 6), "unit": "Gsamples/s", "cores": nthreads, "kind": "port",
             "sample": f"one full 576x336 render (n=800, {cnt} in-cube samples) of the 256^3 cube by the oracle's FP32 "
-                      f"restatement of Camera::Render, {nthreads} threads, {el:.2f} s"}
+                      f"restatement of Camera::Render, {nthreads} threads, {el:.2f} s"}, ref
 
 
 def bench_batched(capi, W, dev, vals, org, spc, fp32_vol):
@@ -144,6 +149,32 @@ def bench_batched(capi, W, dev, vals, org, spc, fp32_vol):
             "device_ms_per_batch": round(last_ms, 3), "nominal_gsamples_per_s": round(nominal / el / 1e9, 2),
             "plain_renders_per_s_same_views": round(256 / plain_el, 1), "speedup_vs_plain": round(plain_el / el, 2),
             "max_rel_l2_vs_fp64": float(f"{max(errs):.3e}")}
+
+
+def bench_cmem(capi, W, dev, steps=20):
+    """The analytic CMEM model (default Init) rendered by the exact path over the config-2 image
+    and ray grid: every sample is evaluated (the model has no bounds), about 25 transcendentals
+    each, so this is VALU-bound."""
+    vol = capi.Volume.cmem(dev, capi.cmem_init())
+    v = W.view_config2()
+    img = capi.Image(dev, v.width, v.height)
+    host, st = vol.render(v)
+    ref64 = vol.render_f64(v)
+    for _ in range(3):
+        capi.render_async(vol, v, img)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    dev.timer_begin()
+    for _ in range(steps):
+        capi.render_async(vol, v, img)
+    dev_ms = dev.timer_end() / steps
+    wall = (time.perf_counter() - t0) / steps
+    img.free()
+    vol.free()
+    nsamp = v.width * v.height * v.samples
+    return {"image": f"{v.width}x{v.height}", "samples_per_ray": v.samples, "samples": nsamp,
+            "kernel_ms": round(dev_ms, 4), "gsamples_per_s": round(nsamp / wall / 1e9, 2),
+            "rel_l2_vs_fp64": float(f"{rel_l2(host, ref64):.4e}"), "bound": "valu"}
 
 
 def bench_precision(capi, W, dev, n=512, count=64):
@@ -253,7 +284,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--forms", default="batched,precision,serving", help="extra forms to measure ('' for none)")
+    ap.add_argument("--forms", default="batched,cmem,precision,serving", help="extra forms to measure ('' for none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (torch) to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -301,10 +332,16 @@ def main():
         "forms": {"plain_renders_per_s": round(p["sync_rps"], 1),
                   "plain_renders_per_s_async": round(args.steps / p["wall"], 1)},
     }
+    # every result carries its error against the FP64 render of the same sample grid
+    ref64 = p["vol"].render_f64(v)
+    line["error"] = {"vs_fp64_rel_l2": float(f"{rel_l2(p['image'], ref64):.4e}"),
+                     "vs_fp64_max_rel": float(f"{max_rel(p['image'], ref64):.4e}")}
     if not args.no_cpu:
         log("cpu baseline: oracle render")
         from oracle import oracle as O
-        line["cpu_baseline"] = cpu_baseline(O, W, capi, p["vals"], p["org"], p["spc"], v, n_in)
+        line["cpu_baseline"], ref32 = cpu_baseline(O, W, capi, p["vals"], p["org"], p["spc"], v, n_in)
+        line["error"]["vs_oracle_fp32"] = "bitwise" if np.array_equal(ref32.view(np.uint32), p["image"].view(np.uint32)) \
+            else float(f"{rel_l2(p['image'], ref32):.4e}")
     else:
         line["cpu_baseline"] = None
     forms = [f for f in args.forms.split(",") if f]
@@ -313,6 +350,9 @@ def main():
         line["forms"]["batched"] = bench_batched(capi, W, dev, p["vals"], p["org"], p["spc"], p["vol"])
     p["vol"].free()
     del p["vals"]
+    if "cmem" in forms:
+        log("cmem: analytic model, config-2 grid")
+        line["forms"]["cmem"] = bench_cmem(capi, W, dev)
     if "precision" in forms:
         log("precision: 512^3 FP32/BF16/F16")
         line["forms"]["precision"] = bench_precision(capi, W, dev)

@@ -43,6 +43,11 @@ class CmemState(C.Structure):
                [("beta_c", _f * 4)] + [(n, _f) for n in ("charlie", "dn", "ds", "theta_n", "theta_s", "r0_lin")]
 
 
+class CmemParams(C.Structure):
+    _fields_ = [("has_v", _i), ("v", _i * 3), ("has_b", _i), ("b", _i * 3), ("dipole", _f), ("p0", _f), ("p1", _i), ("p2", _i),
+                ("p3", _i)] + [(n, _f) for n in ("B", "alpha", "beta", "bs", "A1", "A2", "ay_bs", "az_bs", "density")]
+
+
 class View(C.Structure):
     _fields_ = [("width", _i), ("height", _i), ("fov_w", _f), ("fov_h", _f), ("position", _f * 3), ("aim", _f * 3),
                 ("ray_length", _f), ("samples", _i), ("semantics", _i), ("math", _i), ("flags", _i)]
@@ -88,6 +93,8 @@ SIGNATURES = [
     ("birp_timer_begin", _i, [_P]),
     ("birp_timer_end", _i, [_P, _DP]),
     ("birp_volume_upload", _i, [_P, C.POINTER(VolumeDesc), _FP, _PP]),
+    ("birp_cmem_default_params", None, [C.POINTER(CmemParams)]),
+    ("birp_cmem_init", _i, [C.POINTER(CmemParams), C.POINTER(CmemState)]),
     ("birp_cmem_create", _i, [_P, C.POINTER(CmemState), _PP]),
     ("birp_volume_free", _i, [_P]),
     ("birp_volume_bytes", _i, [_P, C.POINTER(C.c_size_t)]),
@@ -405,6 +412,27 @@ def synth_orbit(count, seed, perigee=1.8, apogee=19.0, incl_deg=70.0) -> np.ndar
     out = np.zeros((count, 3), np.float32)
     check(lib().birp_synth_orbit(count, perigee, apogee, incl_deg, seed, fptr(out)), "birp_synth_orbit")
     return out
+
+
+def cmem_params(**kw) -> CmemParams:
+    """CMEM::Init's arguments: the defaults of CMEM::Init() with keyword overrides (v=[...] and
+    b=[...] set the vectors; the other names are the birp_cmem_params fields)."""
+    p = CmemParams()
+    lib().birp_cmem_default_params(C.byref(p))
+    for name, val in kw.items():
+        if name in ("v", "b"):
+            setattr(p, "has_" + name, 1)
+            getattr(p, name)[:] = [int(x) for x in val]
+        else:
+            setattr(p, name, val)
+    return p
+
+
+def cmem_init(params: CmemParams | None = None) -> CmemState:
+    """The post-Init CMEM state (CMEM::Init on the host, bit-identical to the reference)."""
+    s = CmemState()
+    check(lib().birp_cmem_init(C.byref(params if params is not None else cmem_params()), C.byref(s)), "birp_cmem_init")
+    return s
 
 
 def cmem_state_from(state: dict) -> CmemState:

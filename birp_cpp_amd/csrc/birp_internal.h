@@ -38,6 +38,9 @@ struct CmemParams {
     float a5, a21;
     float beta_c[4];
     float charlie, dn, ds, theta_n, theta_s, r0_lin;
+    // cosf/sinf of theta_n and theta_s, constant per model: evaluated once on the host with the
+    // same libm as the reference (Cmem.cpp:205, 209)
+    float cos_theta_n, sin_theta_n, cos_theta_s, sin_theta_s;
 };
 
 struct DevVolume {

@@ -512,6 +512,10 @@ extern "C" int birp_cmem_create(birp_device* dev, const birp_cmem_state* s, birp
     c.ay_bs = s->ay_bs; c.az_bs = s->az_bs; c.a5 = s->a5; c.a21 = s->a21;
     for (int k = 0; k < 4; k++) c.beta_c[k] = s->beta_c[k];
     c.charlie = s->charlie; c.dn = s->dn; c.ds = s->ds; c.theta_n = s->theta_n; c.theta_s = s->theta_s; c.r0_lin = s->r0_lin;
+    c.cos_theta_n = std::cos(c.theta_n);  // float overloads: cosf/sinf, as the reference calls them
+    c.sin_theta_n = std::sin(c.theta_n);
+    c.cos_theta_s = std::cos(c.theta_s);
+    c.sin_theta_s = std::sin(c.theta_s);
     v->dv.dtype = BIRP_DTYPE_F32;
     v->dv.interp = BIRP_INTERP_TRILINEAR;
     v->bytes = 0;

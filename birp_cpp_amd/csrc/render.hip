@@ -280,8 +280,8 @@ __device__ __forceinline__ float cmem_sample(const CmemParams& m, float x, float
     // LinScaled (Cmem.cpp:188-222)
     const double PI = 3.14159265358979323846;
     float ct = cosf(theta), st = sinf(theta);
-    float phi_n = (float)acos((double)(ct * cosf(m.theta_n)) + (double)(st * sinf(m.theta_n)) * cos((double)phi - PI / 2));
-    float phi_s = (float)acos((double)(ct * cosf(m.theta_s)) + (double)(st * sinf(m.theta_s)) * cos((double)phi - (3 * PI / 2)));
+    float phi_n = (float)acos((double)(ct * m.cos_theta_n) + (double)(st * m.sin_theta_n) * cos((double)phi - PI / 2));
+    float phi_s = (float)acos((double)(ct * m.cos_theta_s) + (double)(st * m.sin_theta_s) * cos((double)phi - (3 * PI / 2)));
     float sin_phi = sinf(phi);
     float f = powf(cosf(theta / 2) + m.a5 * sinf(2 * theta) * (1 - expf(-theta)),
                    (m.p1 * (m.beta_c[0] + m.beta_c[1] * cosf(phi) + m.beta_c[3] * (sin_phi * sin_phi))));

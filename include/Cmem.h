@@ -2,14 +2,13 @@
 //
 // Init keeps the reference's signature and effects, quirks included: p1..p3 are ints, the NULL
 // floats mean 0, p2 takes p3's value (Cmem.cpp:90) and CalcMagneticPressure overwrites B with
-// |b| * 1e-9 (Cmem.cpp:310). Init runs on the host; GetSample and renders run on the GPU.
+// |b| * 1e-9 (Cmem.cpp:310). Init runs on the host (birp_cmem_init); GetSample and renders run
+// on the GPU.
 #pragma once
 #include <vector>
 
 #include "Space.h"
-
-struct birp_volume;
-struct birp_cmem_state;
+#include "birp.h"
 
 class CMEM : public Space {
 public:
@@ -31,38 +30,9 @@ public:
     birp_volume* DeviceVolume();             // created on first use after each Init
 
 private:
-    int temp = 200000;
-    float v[3];
-    float b[3];
-    float pressure_dynamic;
-    float pressure_magnetic;
-    float dipole;
-    std::vector<float> a;
-    std::vector<float> beta_c;
-    float charlie;
-    float dn;
-    float ds;
-    float theta_n;
-    float theta_s;
-    float r0_lin;
-    float p0;
-    float bs;
-    float A1;
-    float A2;
-    float B;
-    float alpha;
-    float beta;
-    float p1;
-    float p2;
-    float p3;
-    float ay_bs;
-    float az_bs;
-    float density;
+    // The reference's private members (Cmem.h:43-96) are replaced by the post-Init state that
+    // birp_cmem_init computes (the same values, bit for bit).
+    birp_cmem_state state_{};
     bool initialised_ = false;
     birp_volume* device_ = nullptr;
-
-    void DefineLinearCoeffs();
-    void CalcDynamicPressure();
-    void CalcMagneticPressure();
-    float CalcInitialAlpha();
 };

@@ -91,6 +91,16 @@ typedef struct birp_cmem_state {
     float charlie, dn, ds, theta_n, theta_s, r0_lin;
 } birp_cmem_state;
 
+/* CMEM::Init's positional arguments (reference include/Cmem.h:21-39). has_v / has_b = 0 stands for
+ * the reference's empty vector (defaults (400,0,0) and (0,0,5)); a 0 float means "default". */
+typedef struct birp_cmem_params {
+    int has_v, v[3];
+    int has_b, b[3];
+    float dipole, p0;
+    int p1, p2, p3;
+    float B, alpha, beta, bs, A1, A2, ay_bs, az_bs, density;
+} birp_cmem_params;
+
 /* One camera pose and ray grid (reference Camera members, Camera.h:23-53). */
 typedef struct birp_view {
     int width, height;   /* W x H pixels; the reference derives them from the FOV (Camera.cpp:38-39) */
@@ -130,6 +140,11 @@ int birp_timer_end(birp_device* dev, double* elapsed_ms);
  * [z][y][x]. The values are checked once for finiteness and sign to pick the kernel variant. */
 int birp_volume_upload(birp_device* dev, const birp_volume_desc* desc, const float* values,
                        birp_volume** out);
+/* Replaces CMEM::Init (Cmem.cpp:35-115, 250-325): the post-Init model state, bit-identical to the
+ * reference's members. Host-only. birp_cmem_default_params fills the arguments CMEM::Init()
+ * passes (Cmem.cpp:18-34, the defaults of Cmem.h:21-39). */
+void birp_cmem_default_params(birp_cmem_params* params);
+int birp_cmem_init(const birp_cmem_params* params, birp_cmem_state* state);
 /* Replaces a CMEM object as a Space (Cmem.h:19-97). */
 int birp_cmem_create(birp_device* dev, const birp_cmem_state* state, birp_volume** out);
 int birp_volume_free(birp_volume* vol);

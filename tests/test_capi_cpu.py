@@ -64,6 +64,30 @@ def test_view_basis_matches_oracle_bitwise(oracle):
         assert rinv.tobytes() == orinv.tobytes()
 
 
+@pytest.mark.parametrize("kw,oracle_params", [
+    ({}, None),
+    (dict(v=[400, 0, 0], b=[0, 0, 4], dipole=0.0, p0=1.0, p1=4, p2=6, p3=3, B=3.6, alpha=-1.0),
+     dict(v=[400, 0, 0], b=[0, 0, 4], dipole=0.0, p0=1, p1=4, p2=6, p3=3, B=3.6, alpha=-1.0)),
+    (dict(dipole=0.0, p0=1.0, p1=1, p2=3, p3=4, B=0.0, alpha=0.0, beta=0.0, bs=0.0, A1=0.0, A2=0.0),
+     dict(dipole=0.0, p0=1.0, p1=1, p2=3, p3=4, B=0.0, alpha=0.0, beta=0.0, bs=0.0, A1=0.0, A2=0.0)),
+    (dict(v=[350, -20, 10], b=[2, -3, -4], dipole=0.3, density=7.5, ay_bs=0.8, az_bs=0.7),
+     dict(v=[350, -20, 10], b=[2, -3, -4], dipole=0.3, density=7.5, ay_bs=0.8, az_bs=0.7)),
+])
+def test_cmem_init_matches_oracle_bitwise(oracle, kw, oracle_params):
+    """birp_cmem_init (CMEM::Init, Cmem.cpp:35-115) gives the oracle's post-Init state bit for bit:
+    the defaults, the reference test's parameter binding (tests/CMEM_test.cpp:51-63), the CLI's
+    NULL-float call (main.cpp:285-294) and a general set."""
+    got = capi.cmem_init(capi.cmem_params(**kw))
+    want = capi.cmem_state_from(oracle.OracleCmem(oracle_params).state())
+    names = [n for n, _ in capi.CmemState._fields_]
+    for n in names:
+        a, b = getattr(got, n), getattr(want, n)
+        if n == "beta_c":
+            assert np.array(a[:], np.float32).tobytes() == np.array(b[:], np.float32).tobytes(), n
+        else:
+            assert np.float32(a).tobytes() == np.float32(b).tobytes(), (n, a, b)
+
+
 def test_view_basis_rejects_nonfinite_pose():
     v = capi.make_view(16, 16, 26.5, 15.5, (float("nan"), 0, 0), (10, 0, 0), 80, 160)
     st, *_ = capi.view_basis(v)
