@@ -69,6 +69,22 @@ def read_profile_value(name, key):
     return None
 
 
+PEAK_VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9  # non-packed FP32 issue: CUs x SIMDs x lanes x clock
+
+
+def valu_roofline(kern_ms):
+    """VALU issue rate of the plain kernel: the committed rocprofv3 instruction count per launch
+    (SQ_INSTS_VALU, 64 lanes each) over this run's measured kernel time."""
+    name = "plain_exact_config2_valu_pmc.txt"
+    insts = read_profile_value(name, "valu_wave_insts_per_launch")
+    if insts is None:
+        return None
+    achieved = insts * 64 / (kern_ms * 1e-3)
+    return {"bound": "valu", "achieved": round(achieved / 1e12, 2), "peak": round(PEAK_VALU_LANE_OPS / 1e12, 2),
+            "unit": "T lane-instructions/s", "frac": round(achieved / PEAK_VALU_LANE_OPS, 3),
+            "lane_ops_per_sample": read_profile_value(name, "valu_lane_ops_per_sample"), "source": "profiles/" + name}
+
+
 def read_traffic():
     """HBM bytes per launch of the plain kernel from the committed rocprofv3 --pmc summary."""
     return read_profile_value("plain_exact_config2_pmc.txt", "traffic_bytes_per_launch")
@@ -143,12 +159,19 @@ def bench_batched(capi, W, dev, vals, org, spc, fp32_vol):
     plain_el = time.perf_counter() - t
     # accuracy of the batch against the FP64 render (a sample of images)
     errs = [rel_l2(b.image(k), fp32_vol.render_f64(views[k])) for k in range(0, 256, 16)]
+    # in-cube samples of the 256 views (the exact path's count; the fast path's FP64 entry can
+    # differ by a sample at a face) and the batch's algorithmic bytes: 16 B per F16 sample + 4 B/px
+    in_cube = sum(fp32_vol.count_samples(x) for x in ex)
+    pixels = sum(x.width * x.height for x in views)
+    alg = 16.0 * in_cube + 4.0 * pixels
     b.free()
     vol16.free()
     return {"poses": 256, "storage": "f16", "renders_per_s": round(256 / el, 1), "ms_per_batch": round(el * 1e3, 3),
             "device_ms_per_batch": round(last_ms, 3), "nominal_gsamples_per_s": round(nominal / el / 1e9, 2),
             "plain_renders_per_s_same_views": round(256 / plain_el, 1), "speedup_vs_plain": round(plain_el / el, 2),
-            "max_rel_l2_vs_fp64": float(f"{max(errs):.3e}")}
+            "max_rel_l2_vs_fp64": float(f"{max(errs):.3e}"), "in_cube_samples": int(in_cube),
+            "in_cube_gsamples_per_s": round(in_cube / el / 1e9, 2),
+            "algorithmic_gb_per_s_device": round(alg / (last_ms * 1e-3) / 1e9, 1)}
 
 
 def bench_cmem(capi, W, dev, steps=20):
@@ -326,9 +349,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                      "kernel_ms": round(kern_ms, 5), "algorithmic_bytes": int(alg_bytes),
-                     "valu": {"lane_ops_per_sample": read_profile_value("plain_exact_config2_valu_pmc.txt", "valu_lane_ops_per_sample"),
-                              "issue_util": read_profile_value("plain_exact_config2_valu_pmc.txt", "valu_issue_util"),
-                              "source": "profiles/plain_exact_config2_valu_pmc.txt"}},
+                     "valu": valu_roofline(kern_ms)},
         "forms": {"plain_renders_per_s": round(p["sync_rps"], 1),
                   "plain_renders_per_s_async": round(args.steps / p["wall"], 1)},
     }
