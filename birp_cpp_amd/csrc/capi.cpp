@@ -762,10 +762,11 @@ int choose_tuning(const birp_volume* vol, const std::vector<birp_view>& views, b
     }
     if (!views.empty() && birp_autotune_lookup(vol->desc.dtype, views[bi].width, views[bi].height, views[bi].samples, &t) == BIRP_OK)
         return BIRP_OK;
-    t.tile_w = 4;
-    t.tile_h = 4;
-    t.lanes_per_ray = 4;
-    t.unroll = 2;
+    // untuned default: the tuner's most frequent winner on the standard workloads
+    t.tile_w = 8;
+    t.tile_h = 8;
+    t.lanes_per_ray = 1;
+    t.unroll = 4;
     return BIRP_OK;
 }
 

@@ -78,7 +78,7 @@ struct birp_server {
     int vol_cap = 64;
     size_t resident = 0;
     size_t inflight_bytes = 0;
-    birp_tuning tuning{4, 4, 4, 2};
+    birp_tuning tuning{8, 8, 1, 4};
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> pending;
