@@ -19,7 +19,9 @@ Also measured in the same run (keys under "forms"):
 The CPU baseline is the oracle's FP32 restatement of the reference render (a port: the reference's
 own Camera.cpp cannot be compiled here) on the same 576x336 image, on the host's cores; its image
 also checks the headline render bit for bit ("error": "vs_oracle_fp32"), beside the error against
-the FP64 render of the same sample grid.
+the FP64 render of the same sample grid. "cpu_reference" times the reference's own DataCube
+sampler (compiled from its sources into oracle/_ref) in the render loop, single-threaded, on a
+third of the image rows, and checks those rows against both images.
 """
 from __future__ import annotations
 
@@ -128,6 +130,31 @@ def cpu_baseline(O, W, capi, vals, org, spc, v, n_in):
     return {"value": round(cnt / el / 1e9, 6), "unit": "Gsamples/s", "cores": nthreads, "kind": "port",
             "sample": f"one full 576x336 render (n=800, {cnt} in-cube samples) of the 256^3 cube by the oracle's FP32 "
                       f"restatement of Camera::Render, {nthreads} threads, {el:.2f} s"}, ref
+
+
+def cpu_reference(O, vals, org, spc, v, ref32, gpu_img, rows=(112, 224)):
+    """The reference's own DataCube::GetSample (oracle/_ref, compiled from its sources) inside the
+    Camera::Render loop, single-threaded as the reference runs, on the middle third of the
+    config-2 image rows; its rows are checked against the oracle's and the GPU's images."""
+    if not os.path.exists(O.REF_PATH):
+        return None
+    ov = O.make_view(v.width, v.height, v.fov_w, v.fov_h, list(v.position), list(v.aim), v.ray_length, v.samples)
+    dirs = O.ray_dirs(ov)
+    rc = O.RefCube(vals, org, spc)
+    try:
+        t = time.perf_counter()
+        img, cnt = rc.render_rows(ov, dirs, rows[0], rows[1], O.SEM_EXTENDED)
+        el = time.perf_counter() - t
+    finally:
+        rc.free()
+    part = slice(rows[0], rows[1])
+    same = bool(np.array_equal(img[part].view(np.uint32), ref32[part].view(np.uint32)) and
+                np.array_equal(img[part].view(np.uint32), gpu_img[part].view(np.uint32)))
+    return {"value": round(cnt / el / 1e9, 6), "unit": "Gsamples/s", "cores": 1, "kind": "reference",
+            "ns_per_sample": round(el / cnt * 1e9, 1), "rows_bitwise_equal_to_gpu_and_oracle": same,
+            "sample": f"rows {rows[0]}..{rows[1] - 1} of the 576x336 config-2 render ({cnt} in-cube samples) with the "
+                      f"reference's DataCube::GetSample compiled from its sources, in the Camera::Render loop, "
+                      f"1 thread, {el:.2f} s"}
 
 
 def bench_batched(capi, W, dev, vals, org, spc, fp32_vol):
@@ -363,6 +390,8 @@ def main():
         line["cpu_baseline"], ref32 = cpu_baseline(O, W, capi, p["vals"], p["org"], p["spc"], v, n_in)
         line["error"]["vs_oracle_fp32"] = "bitwise" if np.array_equal(ref32.view(np.uint32), p["image"].view(np.uint32)) \
             else float(f"{rel_l2(p['image'], ref32):.4e}")
+        log("cpu reference: the reference's DataCube sampler on a third of the rows")
+        line["cpu_reference"] = cpu_reference(O, p["vals"], p["org"], p["spc"], v, ref32, p["image"])
     else:
         line["cpu_baseline"] = None
     forms = [f for f in args.forms.split(",") if f]

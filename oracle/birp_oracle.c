@@ -839,3 +839,10 @@ int or_cmem_edge_pixels(const or_cmem* m, const or_view* v, double rel_band, uns
     free(t);
     return count;
 }
+
+void or_ray_dirs(const or_view* v, float* dirs) {
+    float rot[9], rinv[9];
+    if (or_view_basis(v, rot, rinv) != 0) return;
+    for (int j = 0; j < v->H; ++j)
+        for (int i = 0; i < v->W; ++i) or_ray_dir(v, rinv, i, j, dirs + ((size_t)j * v->W + i) * 3);
+}

@@ -131,6 +131,9 @@ int or_view_basis_f64(const or_view* v, double rot[9], double rinv[9]);
 void or_ray_dir(const or_view* v, const float rinv[9], int i, int j, float d[3]);
 void or_ray_dir_f64(const or_view* v, const double rinv[9], int i, int j, double d[3]);
 
+/* All W*H unit rays of a view (x, y, z per pixel, row-major), as or_render_f32 traces them. */
+void or_ray_dirs(const or_view* v, float* dirs);
+
 /* The reference's per-sample scale (Camera.cpp:201): f32(f64(v*w) * 637810000 / (4/pi) / 1e4). */
 float or_scale_sample(float v, float w);
 

@@ -97,6 +97,7 @@ def lib():
         L.or_view_basis_f64.argtypes = [C.POINTER(View), _dp, _dp]
         L.or_ray_dir.argtypes = [C.POINTER(View), _fp, _i, _i, _fp]
         L.or_ray_dir_f64.argtypes = [C.POINTER(View), _dp, _i, _i, _dp]
+        L.or_ray_dirs.argtypes = [C.POINTER(View), _fp]
         L.or_scale_sample.restype = _f
         L.or_scale_sample.argtypes = [_f, _f]
         L.or_render_f32.restype = _i
@@ -252,6 +253,13 @@ def ray_dir(view: View, rinv, i, j):
     return d
 
 
+def ray_dirs(view: View) -> np.ndarray:
+    """H x W x 3 unit rays of the view, as the FP32 render traces them."""
+    d = np.zeros((view.H, view.W, 3), np.float32)
+    lib().or_ray_dirs(C.byref(view), _fptr(d))
+    return d
+
+
 def ray_dir_f64(view: View, rinv, i, j):
     r = np.ascontiguousarray(rinv, np.float64).reshape(9)
     d = np.zeros(3, np.float64)
@@ -309,6 +317,10 @@ def ref():
         R.ref_cube_sample.restype = _f
         R.ref_cube_sample.argtypes = [C.c_void_p, _f, _f, _f, _ip]
         R.ref_cube_free.argtypes = [C.c_void_p]
+        R.ref_cube_from_grid.restype = C.c_void_p
+        R.ref_cube_from_grid.argtypes = [_i, _i, _i, _fp, _fp, _fp, _i]
+        R.ref_render_rows.restype = C.c_longlong
+        R.ref_render_rows.argtypes = [C.c_void_p, _i, _i, _fp, _fp, _f, _i, _i, _i, _i, _fp]
         R.ref_cmem_default.restype = C.c_void_p
         R.ref_cmem_init.restype = C.c_void_p
         R.ref_cmem_init.argtypes = [_ip, _i, _ip, _i, _f, _f, _i, _i, _i, _f, _f, _f, _f, _f, _f, _f, _f, _f]
@@ -332,3 +344,28 @@ def ref_cmem(params=None):
     va = (_i * max(len(v), 1))(*v)
     ba = (_i * max(len(b), 1))(*b)
     return R.ref_cmem_init(va, len(v), ba, len(b), *cmem_positional(p))
+
+
+class RefCube:
+    """The reference's own DataCube (oracle/_ref) holding an in-memory grid."""
+
+    def __init__(self, data: np.ndarray, origin, spacing, trilinear: bool = True):
+        self.data = np.ascontiguousarray(data, np.float32)
+        nz, ny, nx = self.data.shape
+        o = np.asarray(origin, np.float32)
+        sp = np.asarray(spacing, np.float32)
+        self.h = ref().ref_cube_from_grid(nx, ny, nz, _fptr(o), _fptr(sp), _fptr(self.data), int(trilinear))
+
+    def render_rows(self, view: View, dirs: np.ndarray, row0: int, row1: int, semantics=SEM_EXTENDED):
+        """Rows [row0, row1) with the reference's sampler; returns (H x W image, samples added)."""
+        img = np.zeros((view.H, view.W), np.float32)
+        pos = np.asarray(list(view.pos), np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        cnt = ref().ref_render_rows(self.h, view.W, view.H, _fptr(d), _fptr(pos), view.L, view.n, row0, row1,
+                                    semantics, _fptr(img))
+        return img, cnt
+
+    def free(self):
+        if self.h:
+            ref().ref_cube_free(self.h)
+            self.h = None

@@ -1,8 +1,8 @@
 // ref_shim.cpp -- a C-ABI driver over the reference's own DataCube, CMEM and Helper classes,
 // compiled together with their sources (src/DataCube.cpp, src/Cmem.cpp, src/Helper.cpp) into
 // oracle/_ref/libbirp_ref.so by oracle/Makefile. TEST INFRASTRUCTURE ONLY: it is used to pin
-// the oracle (birp_oracle.c) against the reference and to write golden fixtures; the product
-// never loads it. src/Camera.cpp is not part of this build: it includes EleFits headers
+// the oracle (birp_oracle.c) against the reference, to write golden fixtures, and as the timed
+// reference CPU baseline of bench.py (ref_render_rows); the product never loads it. src/Camera.cpp is not part of this build: it includes EleFits headers
 // (Camera.cpp:23-27), which this image does not have.
 #include <cmath>
 #include <limits>
@@ -64,6 +64,65 @@ float ref_cube_sample(void* h, float x, float y, float z, int* err) {
 }
 
 void ref_cube_free(void* h) { delete static_cast<DataCube*>(h); }
+
+// A DataCube holding an in-memory uniform grid, set up through its public members
+// (DataCube.h:28-42) to the state DataCube::Load leaves: coordinates origin + spacing * i, size,
+// spacing, origin and the [z][y][x] slices.
+void* ref_cube_from_grid(int nx, int ny, int nz, const float* origin3, const float* spacing3, const float* values,
+                         int trilinear) {
+    DataCube* c = new DataCube();
+    const int n[3] = {nx, ny, nz};
+    std::vector<float>* coords[3] = {&c->coords_x, &c->coords_y, &c->coords_z};
+    for (int a = 0; a < 3; a++) {
+        coords[a]->resize(n[a]);
+        for (int i = 0; i < n[a]; i++) (*coords[a])[i] = origin3[a] + spacing3[a] * (float)i;
+        c->spacing[a] = spacing3[a];
+        c->origin_coords[a] = origin3[a];
+    }
+    c->size.x = nx;
+    c->size.y = ny;
+    c->size.z = nz;
+    c->slices.assign(nz, std::vector<std::vector<float>>(ny, std::vector<float>(nx)));
+    size_t k = 0;
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) c->slices[z][y][x] = values[k++];
+    c->SetTrilinear(trilinear != 0);
+    return c;
+}
+
+// Image rows [row0, row1) rendered with the reference's own sampler: the march and integration
+// of Camera::Render (Camera.cpp:170-229) restated around Space::GetSample, called virtually as
+// Camera.cpp:191 calls it, on the reference's DataCube (its GetSample, DataCube.cpp:203-277, with
+// its per-call allocations). Rays come in as unit directions (W*H*3 floats, from the oracle's
+// ray generation); t_k and w follow GenerateRayDistWidth (Camera.cpp:69-81). semantics 0 stops a
+// ray at its first negative scaled sample (the out-of-cube -1 included); 1 skips out-of-cube
+// samples. Returns the number of samples added.
+long long ref_render_rows(void* h, int W, int H, const float* dirs, const float* pos3, float L, int n, int row0,
+                          int row1, int semantics, float* img) {
+    Space* space = static_cast<DataCube*>(h);
+    std::vector<float> t(n);
+    for (int k = 0; k < n; k++) t[k] = L * ((k + 0.5f) / n);
+    const float w = n > 1 ? t[1] - t[0] : 0.f;
+    long long count = 0;
+    for (int j = row0; j < row1 && j < H; ++j)
+        for (int i = 0; i < W; ++i) {
+            const float* d = dirs + ((size_t)j * W + i) * 3;
+            float acc = 0.f;
+            for (int k = 0; k < n; ++k) {
+                float x = d[0] * t[k] + pos3[0], y = d[1] * t[k] + pos3[1], z = d[2] * t[k] + pos3[2];
+                float v = space->GetSample(x, y, z);
+                if (semantics == 1 && v == -1.f) continue;  // out of the cube: skipped
+                float sample = v;
+                sample = (float)(((double)(sample * w) * (1000 * 100 * 6378.1) / (4 / M_PI)) / 10000);
+                if (sample < 0) break;
+                acc = acc + sample;
+                count++;
+            }
+            img[(size_t)j * W + i] = acc;
+        }
+    return count;
+}
 
 // CMEM::Init() (Cmem.cpp:18-34).
 void* ref_cmem_default() {

@@ -335,6 +335,35 @@ def test_threads_do_not_change_results(lib):
     assert ca == cb and np.array_equal(a, b)
 
 
+@pytest.mark.skipif(not os.path.exists(O.REF_PATH), reason="oracle/_ref not built (no reference sources)")
+@pytest.mark.parametrize("semantics", [O.SEM_REFERENCE, O.SEM_EXTENDED])
+def test_reference_sampler_render_equals_oracle(lib, semantics):
+    """Whole images rendered around the reference's own DataCube::GetSample (compiled from its
+    sources, called virtually as Camera.cpp:191 calls it) equal the oracle's bit for bit: a
+    positive shell cube from an outside camera (extended semantics) and from inside, and a signed
+    cube with zeros, where std::lerp's straddle branch and the early stop on a negative sample
+    matter."""
+    from birp_cpp_amd import capi
+    vals, org, spc, _ = capi.synth_shell(48, (8.0, 11.0), 0.05, seed=4)
+    rng = np.random.default_rng(9)
+    signed = (rng.standard_normal((36, 36, 36)) * 1e-5 + 2e-6).astype(np.float32)
+    signed[rng.random(signed.shape) < 0.15] = 0.0
+    cases = [(vals, org, spc, [0.0, -20.0, 10.0], [10.0, 0.0, 0.0]), (vals, org, spc, [12.0, 4.0, 6.0], [1.0, 0.0, 0.0]),
+             (signed, (-10.0, -15.0, -15.0), (30.0 / 35,) * 3, [3.0, 2.0, -4.0], [9.0, 0.0, 0.0])]
+    for data, o, s_, pos, aim in cases:
+        if semantics == O.SEM_REFERENCE and pos[1] < -15:
+            continue  # an outside camera renders zeros under the reference semantics
+        v = O.make_view(72, 42, 26.5, 15.5, pos, aim, 44, 220)
+        rc = O.RefCube(data, o, s_)
+        try:
+            img, cnt = rc.render_rows(v, O.ray_dirs(v), 0, v.H, semantics)
+        finally:
+            rc.free()
+        ref, cnt2 = O.render_f32(O.OracleCube(data, o, s_), v, semantics)
+        assert cnt == cnt2 and cnt > 0
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+
+
 def test_f32_and_f64_renders_agree(lib):
     vals = np.random.default_rng(4).lognormal(-14, 0.3, (48, 48, 48)).astype(np.float32)
     cube = O.OracleCube(vals, (-10.0, -15.0, -15.0), (30 / 47, 30 / 47, 30 / 47))
