@@ -195,6 +195,19 @@ __device__ __forceinline__ QuadFetch quad_fetch(const QuadCtx& c, float xd, floa
     return q;
 }
 
+// For a sample that passed the reference's bounds test (0 <= ingress, index + 1 < n): the index
+// is in [0, n-1), so its truncation is the lower corner and needs no clamp.
+__device__ __forceinline__ QuadFetch quad_fetch_inside(const QuadCtx& c, float xd, float yd, float zd) {
+    QuadFetch q;
+    q.fx = __builtin_amdgcn_fractf(xd);
+    q.fy = __builtin_amdgcn_fractf(yd);
+    q.fz = __builtin_amdgcn_fractf(zd);
+    unsigned off = c.tx[(int)xd] + c.ty[(int)yd] + c.tz[(int)zd];
+    q.a = load16(c.rs, off);
+    q.b = load16(c.rs, off + 1024u);
+    return q;
+}
+
 // x first, then y, then z (DataCube.cpp:260-270).
 template <bool POS>
 __device__ __forceinline__ float quad_lerp(const QuadFetch& q) {
@@ -430,24 +443,35 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact(DevVolume v, ExactV
         // P samples per lane per iteration (k, k + S, ...), staged so that the later samples' LDS
         // and buffer reads are in flight while the first ones' lerps run. The sum order is
         // unchanged: the S samples at k, then the S samples at k + S, and so on.
+        // Every lane samples inside the cube, so the corner indices need no clamp: a lane past
+        // its ray's last in-cube sample kl re-reads kl's t (its result is masked), and a lane
+        // whose ray has no in-cube sample is sent to the cube's origin (zero direction), which is
+        // inside. The in-cube samples kf..kl are contiguous (the FP32 index is monotone in t).
+        const bool has = len > 0;
+        const float rx = has ? dx : 0.f, ry = has ? dy : 0.f, rz = has ? dz : 0.f;
+        const float qx = has ? px : v.ox, qy = has ? py : v.oy, qz = has ? pz : v.oz;
+        // t is read at LDS byte address min(t_lo + 4k, t_hi), both bounds including the LDS base
+        typedef __attribute__((address_space(3))) const float* lds_float_ptr;
+        const unsigned t_base = (unsigned)(uintptr_t)(lds_float_ptr)s_t;
+        unsigned t_lo = t_base + (has ? t_first : 0u), t_hi = t_base + (has ? (unsigned)kl * 4u : 0u);
+        // keep the selects above out of the loop (the compiler would otherwise redo them per sample)
+        asm volatile("" : "+v"(t_lo), "+v"(t_hi));
         for (int kb = 0; kb < trips; kb += P * S) {
-            // one clamp per iteration: past the clamp, a lane's later t reads land at most
-            // (P-1)*S <= 12 floats beyond t[n-1], in the offset tables that follow or the 16
-            // floats of padding after them (the lane is inactive then; its sample is discarded)
-            const float* tp = (const float*)((const char*)s_t + min(t_first + (unsigned)kb * 4u, t_last));
             QuadFetch q[P];
 #pragma unroll
             for (int u = 0; u < P; ++u) {
-                const float t = tp[u * S];
-                const ExactIdx r = exact_index<false>(v, dx, dy, dz, px, py, pz, t);
-                q[u] = quad_fetch(qc, r.xd, r.yd, r.zd);
+                const float t = *(lds_float_ptr)(uintptr_t)min(t_lo + (unsigned)(kb + u * S) * 4u, t_hi);
+                const ExactIdx r = exact_index<false>(v, rx, ry, rz, qx, qy, qz, t);
+                q[u] = quad_fetch_inside(qc, r.xd, r.yd, r.zd);
             }
+            float sc[P];
 #pragma unroll
             for (int u = 0; u < P; ++u) {
-                float sc = scale_pos(quad_lerp<true>(q[u]) * w);  // Camera.cpp:201
-                sc = (kb + u * S < lim) ? sc : -0.0f;
-                acc = add_in_order<S>(acc, sc);
+                sc[u] = scale_pos(quad_lerp<true>(q[u]) * w);  // Camera.cpp:201
+                sc[u] = (kb + u * S < lim) ? sc[u] : -0.0f;
             }
+#pragma unroll
+            for (int u = 0; u < P; ++u) acc = add_in_order<S>(acc, sc[u]);
         }
     }
     for (int kb = 0; !POS && kb < trips; kb += S) {
@@ -945,7 +969,7 @@ static int launch_exact_s(const DevVolume& v, const ExactView& ev, int pipe, hip
     else mode = SM_QUAD_GEN;
     size_t lds = (size_t)((ev.n + 3) & ~3) * sizeof(float);
     if (mode == SM_QUAD_POS || mode == SM_QUAD_GEN) lds += (size_t)(v.nx + v.ny + v.nz) * sizeof(unsigned);
-    if (mode == SM_QUAD_POS) lds += 16 * sizeof(float);  // room for the positive loop's unclamped t reads
+
     if (lds > 64 * 1024) return BIRP_ERR_UNSUPPORTED;
     switch (mode) {
         case SM_QUAD_POS:
