@@ -215,6 +215,47 @@ def test_server_alone_equals_in_mix(device, cube1):
         srv.destroy()
 
 
+def test_server_concurrent_clients(device, cube1):
+    """Four client threads submit and wait concurrently (ctypes releases the GIL in the C calls);
+    every result equals the same request rendered alone, and requests share launches."""
+    import threading
+    vals, org, spc, vols, oc = cube1
+    srv = capi.Server(device, memory_budget=4 << 30, max_batch=32, max_inflight=3, batch_window_ms=0.5)
+    try:
+        idx = [srv.add_volume(vols[capi.DTYPE_F16]), srv.add_volume(vols[capi.DTYPE_F32]), srv.add_volume(vols[capi.DTYPE_BF16])]
+        views = W.views_config3(count=60, seed=11)
+        which = [k % 3 for k in range(len(views))]
+        alone = []
+        for v, w in zip(views, which):
+            st, img, info = srv.wait(srv.submit(idx[w], v), 30000.0)
+            assert st == capi.OK
+            alone.append(img)
+        results, errors, sizes = {}, [], []
+
+        def client(part):
+            try:
+                ids = [(k, srv.submit(idx[which[k]], views[k])) for k in part]
+                for k, rid in ids:
+                    st, img, info = srv.wait(rid, 60000.0)
+                    results[k] = (st, img)
+                    sizes.append(info.batch_size)
+            except Exception as e:  # reported below
+                errors.append(e)
+
+        threads = [threading.Thread(target=client, args=(range(c, len(views), 4),)) for c in range(4)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join(120)
+        assert not errors, errors
+        assert len(results) == len(views)
+        for k, (st, img) in results.items():
+            assert st == capi.OK and np.array_equal(img, alone[k]), k
+        assert max(sizes) > 1
+    finally:
+        srv.destroy()
+
+
 def test_server_admission_and_injected_failure(device, cube1):
     vals, org, spc, vols, oc = cube1
     vb = vols[capi.DTYPE_F16].nbytes()
