@@ -112,6 +112,7 @@ SIGNATURES = [
     ("birp_batch_free", _i, [_P]),
     ("birp_batch_set_tuning", _i, [_P, C.POINTER(Tuning)]),
     ("birp_batch_launch", _i, [_P]),
+    ("birp_batch_launch_eager", _i, [_P]),
     ("birp_batch_wait", _i, [_P]),
     ("birp_batch_download", _i, [_P, _i, _FP]),
     ("birp_batch_stats", _i, [_P, C.POINTER(C.c_longlong), _DP]),
@@ -312,6 +313,9 @@ class Batch:
 
     def launch(self):
         check(lib().birp_batch_launch(self.h), "birp_batch_launch")
+
+    def launch_eager(self):
+        check(lib().birp_batch_launch_eager(self.h), "birp_batch_launch_eager")
 
     def wait(self):
         check(lib().birp_batch_wait(self.h), "birp_batch_wait")

@@ -902,6 +902,22 @@ extern "C" int birp_batch_launch(birp_batch* b) {
     return BIRP_OK;
 }
 
+extern "C" int birp_batch_launch_eager(birp_batch* b) {
+    if (!b || !b->d_poses) return fail(BIRP_ERR_INVALID_ARGUMENT, "batch not ready");
+    birp_device* dev = b->vol->dev;
+    if (take_injected_failure(dev)) return fail(BIRP_ERR_INJECTED, "injected failure");
+    const birp_tuning& t = b->tuning;
+    (void)hipEventRecord(b->ev0, dev->stream);
+    int rc = launch_reset_counter(b->d_work, dev->stream);
+    if (rc == BIRP_OK)
+        rc = launch_fast_batch(b->d_vols, fast_kernel_dtype(b->vol->dv), b->d_poses, b->d_prefix, (int)b->views.size(),
+                               b->total_tiles, b->d_work, t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll, b->grid, dev->stream);
+    if (rc) return fail(rc, "batch launch failed");
+    (void)hipEventRecord(b->ev1, dev->stream);
+    b->launched = true;
+    return BIRP_OK;
+}
+
 extern "C" int birp_batch_wait(birp_batch* b) {
     if (!b) return fail(BIRP_ERR_INVALID_ARGUMENT, "null batch");
     hipError_t e = hipStreamSynchronize(b->vol->dev->stream);

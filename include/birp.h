@@ -195,6 +195,9 @@ int birp_batch_free(birp_batch* b);
 int birp_batch_set_tuning(birp_batch* b, const birp_tuning* tuning);
 /* Launch (graph replay) without waiting; birp_batch_wait waits for completion. */
 int birp_batch_launch(birp_batch* b);
+/* The same launch without the graph: the two kernels go straight onto the device stream (for
+ * callers that cannot replay graphs, and to check graph replays against it). */
+int birp_batch_launch_eager(birp_batch* b);
 int birp_batch_wait(birp_batch* b);
 /* Copy image i (views[i].height * views[i].width floats) to host memory. */
 int birp_batch_download(birp_batch* b, int index, float* host);

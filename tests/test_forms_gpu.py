@@ -143,19 +143,23 @@ def test_batch_matches_fp64_and_replays(cube1):
     views = W.views_config3(count=24)
     b = capi.Batch(vols[capi.DTYPE_F16], views)
     try:
-        b.launch()
+        b.launch_eager()  # the same kernels without the graph
         b.wait()
-        first = [b.image(k) for k in range(len(views))]
+        eager = [b.image(k) for k in range(len(views))]
         for k, v in enumerate(views):
             ref = vols[capi.DTYPE_F32].render_f64(v)
-            assert rel_l2(first[k], ref) <= 1e-4, k
-        # graph replays with eager work in between give identical images
-        for _ in range(3):
+            assert rel_l2(eager[k], ref) <= 1e-4, k
+        # every graph replay, with the program's own eager work and syncs in between, equals
+        # the eager launch bit for bit
+        for rep in range(4):
             _ = vols[capi.DTYPE_F32].render(W.view_config1())
             b.launch()
             b.wait()
             for k in range(len(views)):
-                assert np.array_equal(b.image(k), first[k])
+                assert np.array_equal(b.image(k), eager[k]), (rep, k)
+            if rep == 1:
+                b.launch_eager()
+                b.wait()
         n, ms = b.stats()
         assert n == sum(v.width * v.height * v.samples for v in views) and ms > 0
     finally:
