@@ -1,7 +1,5 @@
-// Helper.h -- math and string helpers (reference include/Helper.h:19-70): the subset the render
-// path and the cube loader use. The pointing utilities (Gei2GseTransforms, RaDecToGSI,
-// XYToRaDec, Interp1, XYZToSpherical, ...) have no caller on the render path and are not
-// provided.
+// Helper.h -- math, string and pointing helpers (reference include/Helper.h:19-70), every
+// function of the reference's class with its signature.
 #pragma once
 #include <algorithm>
 #include <cctype>
@@ -37,6 +35,16 @@ public:
     static float GetDeterminant(const std::vector<std::vector<float>> vect);
     static std::vector<std::vector<float>> GetTranspose(const std::vector<std::vector<float>> matrix1);
     static std::vector<std::vector<float>> GetCofactor(const std::vector<std::vector<float>> vect);
+    static std::vector<std::vector<float>> Gei2GseTransforms();
+    static std::vector<float> RaDecToGSI(float ra, float dec);
+    static std::vector<float> RaDecGSEConversion(std::vector<float>, std::vector<std::vector<float>>);
+    static std::vector<float> TwoMin(std::vector<float>, std::vector<int>);
+    static std::vector<float> XYToRaDec(float, float, std::vector<float>, std::vector<float>, std::vector<float>,
+                                        std::vector<float>, float);
+    static std::vector<float> Interp1(std::vector<float>& x, std::vector<float>& y, std::vector<float> x_new);
+    static int findNearestNeighbourIndex(float value, std::vector<float>& x);
+    static std::vector<float> XYZToSpherical(float*);
+    static std::vector<float> XYZToSphericalAlt(float*);
     static float VectorDistance(float* xyz);
     static float VectorDistance2D(float* xy);
     static bool EqualDistance(std::vector<float> steps);

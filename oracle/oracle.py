@@ -330,6 +330,15 @@ def ref():
         R.ref_matmul3.argtypes = [_fp, _fp, _fp]
         R.ref_get_inverse3.restype = _i
         R.ref_get_inverse3.argtypes = [_fp, _fp]
+        R.ref_gei2gse.argtypes = [_fp]
+        R.ref_radec_to_gsi.argtypes = [_f, _f, _fp]
+        R.ref_radec_gse.argtypes = [_f, _f, _fp, _fp]
+        R.ref_two_min.argtypes = [_fp, _i, _ip, _i, _fp]
+        R.ref_xy_to_radec.argtypes = [_f, _f, _fp, _fp, _fp, _fp, _i, _f, _fp]
+        R.ref_interp1.argtypes = [_fp, _fp, _i, _fp, _i, _fp]
+        R.ref_nearest_below.restype = _i
+        R.ref_nearest_below.argtypes = [_f, _fp, _i]
+        R.ref_xyz_to_spherical.argtypes = [_fp, _i, _fp]
         _ref = R
     return _ref
 

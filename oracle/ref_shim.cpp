@@ -176,4 +176,63 @@ int ref_get_inverse3(const float* m, float* out) {
     }
 }
 
+// Helper::Gei2GseTransforms (Helper.cpp:306-341): the 4x4 rotation, row-major.
+void ref_gei2gse(float* out16) {
+    auto M = Helper::Gei2GseTransforms();
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) out16[i * 4 + j] = M[i][j];
+}
+
+// Helper::RaDecToGSI (Helper.cpp:343-380).
+void ref_radec_to_gsi(float ra, float dec, float* out3) {
+    auto v = Helper::RaDecToGSI(ra, dec);
+    for (int i = 0; i < 3; i++) out3[i] = v[i];
+}
+
+// Helper::RaDecGSEConversion (Helper.cpp:382-390) with a 4x4 row-major matrix.
+void ref_radec_gse(float ra, float dec, const float* m16, float* out3) {
+    std::vector<std::vector<float>> M(4, std::vector<float>(4));
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) M[i][j] = m16[i * 4 + j];
+    auto v = Helper::RaDecGSEConversion({ra, dec}, M);
+    for (int i = 0; i < 3; i++) out3[i] = v[i];
+}
+
+// Helper::TwoMin (Helper.cpp:392-413).
+void ref_two_min(const float* arr, int n, const int* mask, int nm, float* out2) {
+    auto v = Helper::TwoMin(std::vector<float>(arr, arr + n), std::vector<int>(mask, mask + nm));
+    out2[0] = v[0];
+    out2[1] = v[1];
+}
+
+// Helper::XYToRaDec (Helper.cpp:415-476) on a sky grid of n points.
+void ref_xy_to_radec(float x, float y, const float* xsky, const float* ysky, const float* sky1, const float* sky2,
+                     int n, float skysep, float* out2) {
+    auto v = Helper::XYToRaDec(x, y, std::vector<float>(xsky, xsky + n), std::vector<float>(ysky, ysky + n),
+                               std::vector<float>(sky1, sky1 + n), std::vector<float>(sky2, sky2 + n), skysep);
+    out2[0] = v[0];
+    out2[1] = v[1];
+}
+
+// Helper::Interp1 (Helper.cpp:478-513). Every x_new must lie strictly above some x (the
+// reference indexes slope[-1] otherwise).
+void ref_interp1(const float* x, const float* y, int n, const float* xn, int m, float* out) {
+    std::vector<float> X(x, x + n), Y(y, y + n);
+    auto v = Helper::Interp1(X, Y, std::vector<float>(xn, xn + m));
+    for (int i = 0; i < m; i++) out[i] = v[i];
+}
+
+// Helper::findNearestNeighbourIndex (Helper.cpp:515-527).
+int ref_nearest_below(float value, const float* x, int n) {
+    std::vector<float> X(x, x + n);
+    return Helper::findNearestNeighbourIndex(value, X);
+}
+
+// Helper::XYZToSpherical / XYZToSphericalAlt (Helper.cpp:529-585).
+void ref_xyz_to_spherical(const float* xyz, int alt, float* out3) {
+    float p[3] = {xyz[0], xyz[1], xyz[2]};
+    auto v = alt ? Helper::XYZToSphericalAlt(p) : Helper::XYZToSpherical(p);
+    for (int i = 0; i < 3; i++) out3[i] = v[i];
+}
+
 }  // extern "C"

@@ -1,7 +1,7 @@
 // api_tests.cpp -- drives the C++ API (include/DataCube.h, Cmem.h, Camera.h, Helper.h; built into
 // libbirp_api.so) for tests/test_api_cpp.py. Each subcommand reads its inputs from argv/stdin and
 // prints results (floats as IEEE bit patterns) for the Python side to compare with the golden
-// vectors and the oracle. Host-only subcommands: inverse, cmem_state, load, write_outputs,
+// vectors and the oracle. Host-only subcommands: inverse, pointing, cmem_state, load, write_outputs,
 // save_load. GPU subcommands: sample, render, cmem_sample, cmem_render, render_twice,
 // unsupported_space.
 #include <cstdint>
@@ -77,6 +77,92 @@ int main(int argc, char** argv) {
                     for (int k = 0; k < 9; k++) std::printf("%u%c", bits(inv[k / 3][k % 3]), k == 8 ? '\n' : ' ');
                 } catch (const std::runtime_error& e) {
                     std::printf("throw %s\n", e.what());
+                }
+            }
+            return 0;
+        }
+        if (cmd == "pointing") {
+            // One case per stdin line, floats as bits; one output line per case:
+            //   gei                                   -> 16 matrix entries
+            //   gsi <ra> <dec>                        -> RaDecToGSI (3)
+            //   gse <ra> <dec> <m0..m15>              -> RaDecGSEConversion (3)
+            //   twomin <n> <a0..> <nm> <k0..>         -> TwoMin (2)
+            //   sky <n> <skysep> <xsky..> <ysky..> <sky1..> <sky2..>   (sets the grid, no output)
+            //   xyradec <x> <y>                       -> XYToRaDec on the grid (2)
+            //   interp <n> <m> <x..> <y..> <xn..>     -> Interp1 (m), or "throw"
+            //   nearest <n> <value> <x..>             -> findNearestNeighbourIndex (int)
+            //   sph <x> <y> <z>                       -> XYZToSpherical (3) then XYZToSphericalAlt (3)
+            auto rf = [] {
+                uint32_t u;
+                if (!(std::cin >> u)) throw std::runtime_error("pointing: truncated input");
+                return from_bits(u);
+            };
+            auto rfv = [&](int n) {
+                std::vector<float> v(n);
+                for (auto& x : v) x = rf();
+                return v;
+            };
+            auto put = [](const std::vector<float>& v) {
+                for (size_t k = 0; k < v.size(); k++) std::printf("%u%c", bits(v[k]), k + 1 == v.size() ? '\n' : ' ');
+            };
+            std::vector<float> xsky, ysky, sky1, sky2;
+            float skysep = 0.f;
+            std::string op;
+            while (std::cin >> op) {
+                if (op == "gei") {
+                    std::vector<float> flat;
+                    for (const auto& r : Helper::Gei2GseTransforms()) flat.insert(flat.end(), r.begin(), r.end());
+                    put(flat);
+                } else if (op == "gsi") {
+                    float ra = rf(), dec = rf();
+                    put(Helper::RaDecToGSI(ra, dec));
+                } else if (op == "gse") {
+                    float ra = rf(), dec = rf();
+                    std::vector<std::vector<float>> m(4);
+                    for (auto& r : m) r = rfv(4);
+                    put(Helper::RaDecGSEConversion({ra, dec}, m));
+                } else if (op == "twomin") {
+                    int n;
+                    std::cin >> n;
+                    auto a = rfv(n);
+                    int nm;
+                    std::cin >> nm;
+                    std::vector<int> mask(nm);
+                    for (auto& k : mask) std::cin >> k;
+                    put(Helper::TwoMin(a, mask));
+                } else if (op == "sky") {
+                    int n;
+                    std::cin >> n;
+                    skysep = rf();
+                    xsky = rfv(n);
+                    ysky = rfv(n);
+                    sky1 = rfv(n);
+                    sky2 = rfv(n);
+                } else if (op == "xyradec") {
+                    float x = rf(), y = rf();
+                    put(Helper::XYToRaDec(x, y, xsky, ysky, sky1, sky2, skysep));
+                } else if (op == "interp") {
+                    int n, m;
+                    std::cin >> n >> m;
+                    auto x = rfv(n), y = rfv(n), xn = rfv(m);
+                    try {
+                        put(Helper::Interp1(x, y, xn));
+                    } catch (const std::out_of_range&) {
+                        std::printf("throw\n");
+                    }
+                } else if (op == "nearest") {
+                    int n;
+                    std::cin >> n;
+                    float v = rf();
+                    auto x = rfv(n);
+                    std::printf("%d\n", Helper::findNearestNeighbourIndex(v, x));
+                } else if (op == "sph") {
+                    float p[3] = {rf(), rf(), rf()};
+                    auto a = Helper::XYZToSpherical(p), b = Helper::XYZToSphericalAlt(p);
+                    a.insert(a.end(), b.begin(), b.end());
+                    put(a);
+                } else {
+                    throw std::invalid_argument("pointing: unknown op " + op);
                 }
             }
             return 0;

@@ -11,6 +11,11 @@ oracle/_ref/libbirp_ref.so from the reference's src/DataCube.cpp, src/Cmem.cpp a
                          several Init(...) parameter sets, at random points
   helper_matrix.json     Helper::MatrixMultiply and Helper::GetInverse (Helper.cpp:70-264) on
                          rotation products built like Camera::Integrate's, plus non-rotations
+  helper_pointing.json   the pointing utilities (Helper.cpp:266-585): Gei2GseTransforms,
+                         RaDecToGSI and RaDecGSEConversion at random and quadrant-boundary
+                         angles, TwoMin with random masks, XYToRaDec on a polar sky grid (10 deg,
+                         across the 0/360 seam), Interp1 and findNearestNeighbourIndex on ragged
+                         grids, XYZToSpherical(Alt) at random, axis and zero points
 
 Floats are stored as their IEEE bit patterns (uint32) so the fixtures are exact.
 Usage: python tests/golden/make_golden.py
@@ -150,6 +155,94 @@ def helper(R, rng):
     return out
 
 
+def pointing(R, rng):
+    fp = C.POINTER(C.c_float)
+    ip = C.POINTER(C.c_int)
+    f = np.float32
+    out = {}
+    m = np.zeros(16, f)
+    R.ref_gei2gse(m.ctypes.data_as(fp))
+    out["gei2gse"] = bits(m)
+
+    ra = rng.uniform(0, 360, 300).astype(f).tolist() + [0, 45, 89.5, 90, 90.5, 135, 180, 225, 270, 315, 359.5, 360]
+    dec = rng.uniform(-90, 90, 300).astype(f).tolist() + [0, 30, -30, 90, -90, 60, 0, -60, 45, 10, -89.9, 89.9]
+    out["radec"] = []
+    for a, d in zip(ra, dec):
+        g = np.zeros(3, f)
+        e = np.zeros(3, f)
+        R.ref_radec_to_gsi(a, d, g.ctypes.data_as(fp))
+        R.ref_radec_gse(a, d, m.ctypes.data_as(fp), e.ctypes.data_as(fp))
+        out["radec"].append({"ra": bits([a])[0], "dec": bits([d])[0], "gsi": bits(g), "gse": bits(e)})
+
+    out["two_min"] = []
+    for _ in range(150):
+        n = int(rng.integers(3, 24))
+        arr = rng.standard_normal(n).astype(f)
+        if rng.random() < 0.3:
+            arr = np.round(arr).astype(f)  # ties
+        mask = np.sort(rng.choice(n + 2, size=int(rng.integers(0, n + 2)), replace=False)).astype(np.int32)
+        r2 = np.zeros(2, f)
+        R.ref_two_min(arr.ctypes.data_as(fp), n, mask.ctypes.data_as(ip), len(mask), r2.ctypes.data_as(fp))
+        out["two_min"].append({"arr": bits(arr), "mask": mask.tolist(), "out": bits(r2)})
+
+    # a polar sky grid: RA 0..350 and Dec -60..80 in 10 degree steps, image position (90 - dec) at angle ra
+    skysep = f(10)
+    ras, decs = np.meshgrid(np.arange(0, 360, 10, dtype=f), np.arange(-60, 90, 10, dtype=f))
+    sky1, sky2 = ras.ravel().astype(f), decs.ravel().astype(f)
+    rad = np.deg2rad(sky1.astype(np.float64))
+    xsky = ((90 - sky2) * np.cos(rad)).astype(f)
+    ysky = ((90 - sky2) * np.sin(rad)).astype(f)
+    out["sky"] = {"skysep": bits([skysep])[0], "xsky": bits(xsky), "ysky": bits(ysky), "sky1": bits(sky1), "sky2": bits(sky2),
+                  "points": []}
+    r = rng.uniform(15, 140, 300)
+    th = rng.uniform(0, 2 * np.pi, 300)
+    th[:20] = rng.uniform(-0.05, 0.05, 20)  # near the 0/360 seam
+    for x, y in zip((r * np.cos(th)).astype(f), (r * np.sin(th)).astype(f)):
+        r2 = np.zeros(2, f)
+        R.ref_xy_to_radec(x, y, xsky.ctypes.data_as(fp), ysky.ctypes.data_as(fp), sky1.ctypes.data_as(fp),
+                          sky2.ctypes.data_as(fp), len(sky1), skysep, r2.ctypes.data_as(fp))
+        out["sky"]["points"].append({"x": bits([x])[0], "y": bits([y])[0], "out": bits(r2)})
+
+    out["interp1"] = []
+    for _ in range(60):
+        n = int(rng.integers(2, 30))
+        x = np.sort(rng.uniform(-10, 10, n)).astype(f)
+        x = np.unique(x)
+        y = rng.standard_normal(len(x)).astype(f)
+        k = int(rng.integers(1, 40))
+        xn = rng.uniform(float(x[0]), float(x[-1]) + 2, k).astype(f)
+        xn = xn[xn > x[0]]
+        if len(x) < 2 or len(xn) == 0:
+            continue
+        yn = np.zeros(len(xn), f)
+        R.ref_interp1(x.ctypes.data_as(fp), y.ctypes.data_as(fp), len(x), xn.ctypes.data_as(fp), len(xn),
+                      yn.ctypes.data_as(fp))
+        out["interp1"].append({"x": bits(x), "y": bits(y), "xn": bits(xn), "yn": bits(yn)})
+
+    out["nearest"] = []
+    for _ in range(100):
+        n = int(rng.integers(1, 20))
+        x = rng.uniform(-5, 5, n).astype(f)
+        if rng.random() < 0.3:
+            x = np.round(x).astype(f)
+        v = f(rng.uniform(-6, 6)) if rng.random() < 0.8 else x[int(rng.integers(0, n))]
+        out["nearest"].append({"x": bits(x), "value": bits([v])[0],
+                               "index": R.ref_nearest_below(v, x.ctypes.data_as(fp), n)})
+
+    pts = rng.standard_normal((200, 3)).astype(f) * f(20)
+    pts = np.concatenate([pts, np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [-1, 0, 0], [0, -1, 0], [0, 0, -1],
+                                         [-0.0, 0, 0], [0, -0.0, -2], [3, -4, 0]], f)])
+    out["spherical"] = []
+    for p in pts:
+        p = np.ascontiguousarray(p)
+        a = np.zeros(3, f)
+        b = np.zeros(3, f)
+        R.ref_xyz_to_spherical(p.ctypes.data_as(fp), 0, a.ctypes.data_as(fp))
+        R.ref_xyz_to_spherical(p.ctypes.data_as(fp), 1, b.ctypes.data_as(fp))
+        out["spherical"].append({"xyz": bits(p), "sph": bits(a), "alt": bits(b)})
+    return out
+
+
 def main():
     O.build()
     if not O.ref_available():
@@ -161,6 +254,10 @@ def main():
         with open(os.path.join(HERE, name), "w") as f:
             json.dump(data, f, separators=(",", ":"))
         print("wrote", name, os.path.getsize(os.path.join(HERE, name)), "bytes")
+    data = pointing(R, np.random.default_rng(20240602))
+    with open(os.path.join(HERE, "helper_pointing.json"), "w") as f:
+        json.dump(data, f, separators=(",", ":"))
+    print("wrote helper_pointing.json", os.path.getsize(os.path.join(HERE, "helper_pointing.json")), "bytes")
 
 
 if __name__ == "__main__":

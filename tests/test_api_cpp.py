@@ -61,6 +61,57 @@ def test_helper_inverse_matches_reference(exe):
             assert [int(x) for x in line.split()] == w
 
 
+def test_helper_pointing_matches_reference(exe):
+    """Gei2GseTransforms, RaDecToGSI, RaDecGSEConversion, TwoMin, XYToRaDec, Interp1,
+    findNearestNeighbourIndex and XYZToSpherical(Alt) against the reference's own outputs
+    (tests/golden/helper_pointing.json), bit for bit."""
+    g = json.load(open(os.path.join(GOLDEN, "helper_pointing.json")))
+    j = lambda v: " ".join(map(str, v))  # noqa: E731
+    lines, want = ["gei"], [g["gei2gse"]]
+    for c in g["radec"]:
+        lines += [f"gsi {c['ra']} {c['dec']}", f"gse {c['ra']} {c['dec']} {j(g['gei2gse'])}"]
+        want += [c["gsi"], c["gse"]]
+    for c in g["two_min"]:
+        lines.append(f"twomin {len(c['arr'])} {j(c['arr'])} {len(c['mask'])} {j(c['mask'])}")
+        want.append(c["out"])
+    s = g["sky"]
+    lines.append(f"sky {len(s['xsky'])} {s['skysep']} {j(s['xsky'])} {j(s['ysky'])} {j(s['sky1'])} {j(s['sky2'])}")
+    for c in s["points"]:
+        lines.append(f"xyradec {c['x']} {c['y']}")
+        want.append(c["out"])
+    for c in g["interp1"]:
+        lines.append(f"interp {len(c['x'])} {len(c['xn'])} {j(c['x'])} {j(c['y'])} {j(c['xn'])}")
+        want.append(c["yn"])
+    for c in g["nearest"]:
+        lines.append(f"nearest {len(c['x'])} {c['value']} {j(c['x'])}")
+        want.append([c["index"]])
+    for c in g["spherical"]:
+        lines.append(f"sph {j(c['xyz'])}")
+        want.append(c["sph"] + c["alt"])
+    # below every grid point: the reference reads slope[-1]; this API throws instead
+    one = int(np.float32(1).view(np.uint32))
+    lines.append(f"interp 2 1 {one} {int(np.float32(2).view(np.uint32))} 0 0 {one}")
+    rc, out, err = run(exe, "pointing", stdin="\n".join(lines) + "\n")
+    assert rc == 0, err
+    got = out.strip().split("\n")
+    assert len(got) == len(want) + 1
+    for k, (line, w) in enumerate(zip(got, want)):
+        assert [int(x) for x in line.split()] == w, k
+    assert got[-1] == "throw"
+    # sanity of the fixture itself: the GEI->GSE matrix is a rotation about x then z (orthonormal)
+    m = f32(g["gei2gse"]).reshape(4, 4).astype(np.float64)
+    assert np.allclose(m @ m.T, np.eye(4), atol=1e-6)
+    # RA/Dec -> unit vectors of the right quadrant
+    for c in g["radec"][:300]:
+        ra, dec = float(f32([c["ra"]])[0]), float(f32([c["dec"]])[0])
+        v = f32(c["gsi"]).astype(np.float64)
+        assert abs(np.linalg.norm(v) - 1) < 1e-5
+        e = np.array([np.cos(np.radians(dec)) * np.cos(np.radians(ra)), np.cos(np.radians(dec)) * np.sin(np.radians(ra)),
+                      np.sin(np.radians(dec))])
+        if 0 < ra < 270 and abs(ra - 90) > 1 and abs(ra - 180) > 1:
+            assert np.allclose(v, e, atol=2e-4), (ra, dec, v, e)
+
+
 @pytest.mark.parametrize("which,params", [("default", None),
                                           ("test_params", dict(v=[400, 0, 0], b=[0, 0, 4], dipole=0.0, p0=1, p1=4, p2=6, p3=3,
                                                                B=3.6, alpha=-1.0)),
