@@ -777,18 +777,26 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_zero_f32(float* p, size_t n) {
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ float bf16lo(unsigned u) { return __uint_as_float(u << 16); }
 __device__ __forceinline__ float bf16hi(unsigned u) { return __uint_as_float(u & 0xFFFF0000u); }
-__device__ __forceinline__ float f16lo(unsigned u) { return __half2float(__ushort_as_half((unsigned short)(u & 0xFFFFu))); }
-__device__ __forceinline__ float f16hi(unsigned u) { return __half2float(__ushort_as_half((unsigned short)(u >> 16))); }
 
 __device__ __forceinline__ float flerp(float a, float b, float t) { return __builtin_fmaf(t, b - a, a); }
 
-// Fast trilinear sample at index-space point (xd, yd, zd), already clamped into [0, n-1].
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+// Lerp of the FP16 pair (a, b) packed in one dword, in FP32: (a - t a) + t b as two fused
+// multiply-adds, each reading its FP16 operand directly (v_fma_mix_f32, no conversions).
+__device__ __forceinline__ float xlerp_f16(unsigned u, float t) {
+    const half2_t h = __builtin_bit_cast(half2_t, u);
+    const float a = (float)h.x, b = (float)h.y;
+    return __builtin_fmaf(t, b, __builtin_fmaf(-t, a, a));
+}
+
+// Fast trilinear sample at index-space point (xd, yd, zd), already clamped into
+// [0, nextbelow(n-1)] per axis, so the truncation is a valid cell index (0 .. n-2).
 template <int DT>
 __device__ __forceinline__ float fast_sample(const DevVolume& v, __amdgpu_buffer_rsrc_t rs, const unsigned* tab, float xd,
                                              float yd, float zd) {
-    // xd in [0, n-1): the cell is the truncation and the weight the fraction
     float fx = __builtin_amdgcn_fractf(xd), fy = __builtin_amdgcn_fractf(yd), fz = __builtin_amdgcn_fractf(zd);
-    int ix = to_index(xd, v.fnx - 2.0f), iy = to_index(yd, v.fny - 2.0f), iz = to_index(zd, v.fnz - 2.0f);
+    int ix = (int)xd, iy = (int)yd, iz = (int)zd;
     unsigned off = tab[ix] + tab[v.nx + iy] + tab[v.nx + v.ny + iz];
     if constexpr (DT == BIRP_DTYPE_F32) {
         float4 a = load16(rs, off), b = load16(rs, off + 1024u);
@@ -796,28 +804,35 @@ __device__ __forceinline__ float fast_sample(const DevVolume& v, __amdgpu_buffer
         float hy = flerp(flerp(b.x, b.y, fx), flerp(b.z, b.w, fx), fy);
         return flerp(ly, hy, fz);
     } else {  // BF16, F16 with per-brick scale, F16 with one scale (BIRP_KDT_F16_UNIFORM)
+        // each dword of the record holds the x-pair (x0, x1) of one (y, z) corner row
         uint4 o = load16u(rs, off);
-        float c0, c1, c2, c3, c4, c5, c6, c7;
+        float c00, c10, c01, c11;  // the four x-lerps, at (y0,z0), (y1,z0), (y0,z1), (y1,z1)
         if constexpr (DT == BIRP_DTYPE_BF16) {
-            c0 = bf16lo(o.x); c1 = bf16hi(o.x); c2 = bf16lo(o.y); c3 = bf16hi(o.y);
-            c4 = bf16lo(o.z); c5 = bf16hi(o.z); c6 = bf16lo(o.w); c7 = bf16hi(o.w);
+            c00 = flerp(bf16lo(o.x), bf16hi(o.x), fx);
+            c10 = flerp(bf16lo(o.y), bf16hi(o.y), fx);
+            c01 = flerp(bf16lo(o.z), bf16hi(o.z), fx);
+            c11 = flerp(bf16lo(o.w), bf16hi(o.w), fx);
         } else {
-            c0 = f16lo(o.x); c1 = f16hi(o.x); c2 = f16lo(o.y); c3 = f16hi(o.y);
-            c4 = f16lo(o.z); c5 = f16hi(o.z); c6 = f16lo(o.w); c7 = f16hi(o.w);
+            c00 = xlerp_f16(o.x, fx);
+            c10 = xlerp_f16(o.y, fx);
+            c01 = xlerp_f16(o.z, fx);
+            c11 = xlerp_f16(o.w, fx);
         }
-        float ly = flerp(flerp(c0, c1, fx), flerp(c2, c3, fx), fy);
-        float hy = flerp(flerp(c4, c5, fx), flerp(c6, c7, fx), fy);
-        float val = flerp(ly, hy, fz);
+        float val = flerp(flerp(c00, c10, fy), flerp(c01, c11, fy), fz);
         if constexpr (DT == BIRP_DTYPE_F16) val *= v.bscale[off >> 13];
         return val;
     }
 }
 
-// Kahan step: sum += x with compensation c.
-__device__ __forceinline__ void kahan_add(float& sum, float& c, float x) {
-    float y = x - c;
-    float t = sum + y;
-    c = (t - sum) - y;
+// One sample folded into a Kahan sum (sum, nc), nc holding the negated compensation -c:
+// y = x - c; t = sum + y; -c' = y - (t - sum); sum = t. An inactive lane adds x = 0.
+template <int DT>
+__device__ __forceinline__ void fast_accumulate(const DevVolume& v, __amdgpu_buffer_rsrc_t rs, const unsigned* tab,
+                                                float xd, float yd, float zd, bool act, float& sum, float& nc) {
+    const float x = fast_sample<DT>(v, rs, tab, xd, yd, zd);
+    const float y = (act ? x : 0.f) + nc;
+    const float t = sum + y;
+    nc = y - (t - sum);
     sum = t;
 }
 
@@ -920,7 +935,7 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
         const float hx = __uint_as_float(__float_as_uint(V.fnx - 1.0f) - 1u);
         const float hy = __uint_as_float(__float_as_uint(V.fny - 1.0f) - 1u);
         const float hz = __uint_as_float(__float_as_uint(V.fnz - 1.0f) - 1u);
-        float sum = 0.f, comp = 0.f;
+        float sum = 0.f, nc = 0.f;
         for (int q = 0; q < trips; q += U) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -931,12 +946,11 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
                 float xd = __builtin_amdgcn_fmed3f(__builtin_fmaf(fj, dix, ix0), 0.f, hx);
                 float yd = __builtin_amdgcn_fmed3f(__builtin_fmaf(fj, diy, iy0), 0.f, hy);
                 float zd = __builtin_amdgcn_fmed3f(__builtin_fmaf(fj, diz, iz0), 0.f, hz);
-                float val = fast_sample<DT>(V, rs, tab, xd, yd, zd);
-                kahan_add(sum, comp, act ? val : 0.f);
+                fast_accumulate<DT>(V, rs, tab, xd, yd, zd, act, sum, nc);
             }
         }
         // combine the S sub-lane partials in sub order, in double
-        double tot = (double)sum - (double)comp;
+        double tot = (double)sum + (double)nc;
         if constexpr (S > 1) {
             double t1 = __shfl_xor(tot, 1, 64);
             tot = (sub & 1) ? t1 + tot : tot + t1;
