@@ -291,10 +291,13 @@ __device__ __forceinline__ float cmem_sample(const CmemParams& m, float x, float
     float theta = (x != 0) ? acosf(x / r) : 0.f;
     float phi = (y != 0) ? acosf(y / sqrtf(y * y + z * z)) : 0.f;
     // LinScaled (Cmem.cpp:188-222)
-    const double PI = 3.14159265358979323846;
+    // cos(phi - pi/2) and cos(phi - 3 pi/2) in double (Cmem.cpp:205, 209) are sin(phi) and
+    // -sin(phi): one double sine serves both (they differ from the reference's by ~1e-16
+    // absolute, far below the float rounding of phi_n and phi_s)
     float ct = cosf(theta), st = sinf(theta);
-    float phi_n = (float)acos((double)(ct * m.cos_theta_n) + (double)(st * m.sin_theta_n) * cos((double)phi - PI / 2));
-    float phi_s = (float)acos((double)(ct * m.cos_theta_s) + (double)(st * m.sin_theta_s) * cos((double)phi - (3 * PI / 2)));
+    const double sphi = sin((double)phi);
+    float phi_n = (float)acos((double)(ct * m.cos_theta_n) + (double)(st * m.sin_theta_n) * sphi);
+    float phi_s = (float)acos((double)(ct * m.cos_theta_s) + (double)(st * m.sin_theta_s) * -sphi);
     float sin_phi = sinf(phi);
     float f = powf(cosf(theta / 2) + m.a5 * sinf(2 * theta) * (1 - expf(-theta)),
                    (m.p1 * (m.beta_c[0] + m.beta_c[1] * cosf(phi) + m.beta_c[3] * (sin_phi * sin_phi))));
@@ -304,7 +307,7 @@ __device__ __forceinline__ float cmem_sample(const CmemParams& m, float x, float
     // ShueModel (Cmem.cpp:280-288)
     float base = 2 / (1 + ct);
     float ry = m.bs * powf(base, m.ay_bs);
-    float rz = m.bs * powf(base, m.az_bs);
+    float rz = (m.az_bs == m.ay_bs) ? ry : m.bs * powf(base, m.az_bs);  // equal unless set apart
     float rzc = rz * cosf(phi), rys = ry * sin_phi;
     float radius_bs = (ry * rz) / sqrtf((rzc * rzc) + (rys * rys));
     float h = 0.05f / 2;
