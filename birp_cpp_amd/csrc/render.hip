@@ -304,29 +304,31 @@ __device__ __forceinline__ float cmem_sample(const CmemParams& m, float x, float
     float q = m.p2 * m.charlie * expf(m.p3 * m.dn * (powf(phi_n, m.a21))) +
               m.p2 * m.charlie * expf(m.p3 * m.ds * (powf(phi_s, m.a21)));
     float radius_mp = m.p0 * m.r0_lin * f + q;
+    float h = 0.05f / 2;
+    if (r < radius_mp - h) return 0;  // inside the magnetopause: the bow shock is not needed
     // ShueModel (Cmem.cpp:280-288)
     float base = 2 / (1 + ct);
     float ry = m.bs * powf(base, m.ay_bs);
     float rz = (m.az_bs == m.ay_bs) ? ry : m.bs * powf(base, m.az_bs);  // equal unless set apart
     float rzc = rz * cosf(phi), rys = ry * sin_phi;
     float radius_bs = (ry * rz) / sqrtf((rzc * rzc) + (rys * rys));
-    float h = 0.05f / 2;
-    if (r < radius_mp - h) return 0;
+    // the reference's zone tests in its order (Cmem.cpp:136-176); the sheath emissivity is
+    // evaluated only for the three zones that use it
+    const bool z_mp = r >= radius_mp - h && r <= radius_mp + h;
+    const bool z_sheath = !z_mp && r > radius_mp + h && r < radius_bs - h;
+    const bool z_bs = !z_mp && !z_sheath && r >= radius_bs - h && r <= radius_bs + h;
+    double sd = r / 10;
+    if (!(z_mp || z_sheath || z_bs)) return (float)(m.A2 * (1 / (sd * sd * sd)));
     float half_t = theta / 2;
     float inner = m.A1 * (expf(-m.B * (half_t * half_t * half_t * half_t))) * powf(r / 10, (-m.alpha - (m.beta * (st * st))));
-    if (r >= radius_mp - h && r <= radius_mp + h) {
+    if (z_mp) {
         float norm = (r - (radius_mp - h)) / (radius_mp + h);
         return lerp_general(0.f, inner, norm);
-    } else if (r > radius_mp + h && r < radius_bs - h) {
-        return inner;
-    } else if (r >= radius_bs - h && r <= radius_bs + h) {
-        double sd = r / 10;
-        float norm = (r - (radius_bs - h)) / (radius_bs + h);
-        float b2 = (float)(m.A2 * (1 / (sd * sd * sd)));
-        return lerp_general(inner, b2, norm);
     }
-    double sd = r / 10;
-    return (float)(m.A2 * (1 / (sd * sd * sd)));
+    if (z_sheath) return inner;
+    float norm = (r - (radius_bs - h)) / (radius_bs + h);
+    float b2 = (float)(m.A2 * (1 / (sd * sd * sd)));
+    return lerp_general(inner, b2, norm);
 }
 
 // ------------------------------------------------------------------------------------------
