@@ -236,17 +236,16 @@ extern "C" int birp_device_open(int ordinal, birp_device** out) {
     if ((e = hipSetDevice(ordinal)) != hipSuccess) return hip_fail(e, "hipSetDevice");
     birp_device* d = new birp_device();
     d->ordinal = ordinal;
-    if ((e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking)) != hipSuccess) {
-        delete d;
-        return hip_fail(e, "hipStreamCreate");
-    }
-    (void)hipEventCreate(&d->ev0);
-    (void)hipEventCreate(&d->ev1);
-    (void)hipEventCreate(&d->ev_a);
-    (void)hipEventCreate(&d->ev_b);
-    if ((e = hipMalloc(&d->d_counter, 64)) != hipSuccess) {
-        delete d;
-        return hip_fail(e, "hipMalloc(counter)");
+    const char* what = nullptr;
+    if ((e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking)) != hipSuccess) what = "hipStreamCreate";
+    else if ((e = hipEventCreate(&d->ev0)) != hipSuccess || (e = hipEventCreate(&d->ev1)) != hipSuccess ||
+             (e = hipEventCreate(&d->ev_a)) != hipSuccess || (e = hipEventCreate(&d->ev_b)) != hipSuccess)
+        what = "hipEventCreate";
+    else if ((e = hipMalloc(&d->d_counter, 64)) != hipSuccess) what = "hipMalloc(counter)";
+    if (what) {
+        int rc = hip_fail(e, what);
+        birp_device_close(d);
+        return rc;
     }
     *out = d;
     return BIRP_OK;
@@ -255,16 +254,14 @@ extern "C" int birp_device_open(int ordinal, birp_device** out) {
 extern "C" int birp_device_close(birp_device* dev) {
     if (!dev) return BIRP_OK;
     (void)hipSetDevice(dev->ordinal);
-    (void)hipStreamSynchronize(dev->stream);
+    if (dev->stream) (void)hipStreamSynchronize(dev->stream);
     if (dev->scratch) (void)hipFree(dev->scratch);
     if (dev->scratch2) (void)hipFree(dev->scratch2);
     if (dev->pinned) (void)hipHostFree(dev->pinned);
     if (dev->d_counter) (void)hipFree(dev->d_counter);
-    (void)hipEventDestroy(dev->ev0);
-    (void)hipEventDestroy(dev->ev1);
-    (void)hipEventDestroy(dev->ev_a);
-    (void)hipEventDestroy(dev->ev_b);
-    (void)hipStreamDestroy(dev->stream);
+    for (hipEvent_t ev : {dev->ev0, dev->ev1, dev->ev_a, dev->ev_b})
+        if (ev) (void)hipEventDestroy(ev);
+    if (dev->stream) (void)hipStreamDestroy(dev->stream);
     delete dev;
     return BIRP_OK;
 }
@@ -848,8 +845,7 @@ extern "C" int birp_batch_create(birp_volume* vol, const birp_view* views, int c
     if ((e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking)) != hipSuccess) { rc = hip_fail(e, "stream"); goto bad; }
     if ((e = hipMalloc(&b->d_images, off * sizeof(float))) != hipSuccess) { rc = hip_fail(e, "hipMalloc(images)"); goto bad; }
     if ((e = hipMalloc(&b->d_work, 1024)) != hipSuccess) { rc = hip_fail(e, "hipMalloc(counter)"); goto bad; }
-    (void)hipEventCreate(&b->ev0);
-    (void)hipEventCreate(&b->ev1);
+    if ((e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess) { rc = hip_fail(e, "event"); goto bad; }
     choose_tuning(vol, b->views, b->tuning);
     if ((rc = batch_setup(b)) != BIRP_OK) goto bad;
     *out = b;
@@ -967,6 +963,7 @@ static int render_fast_into(birp_volume* vol, const birp_view* view, float* dimg
         if (e != hipSuccess) return hip_fail(e, "hipMalloc(scratch2)");
         dev->scratch2_bytes = 4096;
     }
+    static_assert(sizeof(DevVolume) <= 1024 && sizeof(FastPose) <= 1024, "fast render staging layout");
     char* base = (char*)dev->scratch2;
     DevVolume* dv = (DevVolume*)base;
     FastPose* dp = (FastPose*)(base + 1024);

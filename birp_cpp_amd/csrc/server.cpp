@@ -48,6 +48,7 @@ struct Request {
     int batch_size = 0;
     int status = BIRP_OK;
     bool finished = false;
+    bool waited = false;  // a thread is in birp_server_wait for it
 };
 
 struct Slot {
@@ -436,10 +437,15 @@ extern "C" int birp_server_wait(birp_server* S, uint64_t id, double timeout_ms, 
     auto it = S->reqs.find(id);
     if (it == S->reqs.end()) return fail(BIRP_ERR_NOT_FOUND, "unknown request id");
     Request* r = it->second;
+    // one waiter per request: the waiter that sees it finish frees it
+    if (r->waited) return fail(BIRP_ERR_INVALID_ARGUMENT, "another thread is waiting for this request");
+    r->waited = true;
     auto ready = [&] { return r->finished; };
     if (timeout_ms < 0) S->cv_done.wait(lk, ready);
-    else if (!S->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready))
+    else if (!S->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready)) {
+        r->waited = false;
         return fail(BIRP_ERR_TIMEOUT, "request not finished");
+    }
     if (info) {
         info->status = r->status;
         info->submit_s = r->submit_s;

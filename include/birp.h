@@ -238,7 +238,9 @@ int birp_server_destroy(birp_server* s);
 int birp_server_add_volume(birp_server* s, birp_volume* vol, int* index);
 /* Queues a request; the result buffer (H*W floats, host) must stay valid until it completes. */
 int birp_server_submit(birp_server* s, const birp_request* req, float* result, uint64_t* id);
-/* Waits up to timeout_ms (< 0: forever) for request id; fills info and forgets the request. */
+/* Waits up to timeout_ms (< 0: forever) for request id; fills info and forgets the request.
+ * One thread at a time may wait for a given id (a second concurrent waiter gets
+ * BIRP_ERR_INVALID_ARGUMENT); after BIRP_ERR_TIMEOUT the id can be waited for again. */
 int birp_server_wait(birp_server* s, uint64_t id, double timeout_ms, birp_result_info* info);
 /* Steady-clock seconds as used in birp_result_info. */
 double birp_clock_s(void);
