@@ -62,7 +62,8 @@ class Tuning(C.Structure):
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("memory_budget", C.c_size_t), ("max_batch", _i), ("max_inflight", _i), ("batch_window_ms", _d)]
+    _fields_ = [("memory_budget", C.c_size_t), ("max_batch", _i), ("max_inflight", _i), ("batch_window_ms", _d),
+                ("reserve_pixels", C.c_size_t)]
 
 
 class Request(C.Structure):
@@ -339,9 +340,9 @@ class Batch:
 
 
 class Server:
-    def __init__(self, dev: Device, memory_budget: int, max_batch=64, max_inflight=3, batch_window_ms=0.0):
+    def __init__(self, dev: Device, memory_budget: int, max_batch=64, max_inflight=3, batch_window_ms=0.0, reserve_pixels=0):
         self.dev = dev
-        cfg = ServerConfig(int(memory_budget), int(max_batch), int(max_inflight), float(batch_window_ms))
+        cfg = ServerConfig(int(memory_budget), int(max_batch), int(max_inflight), float(batch_window_ms), int(reserve_pixels))
         self.h = C.c_void_p()
         check(lib().birp_server_create(dev.h, C.byref(cfg), C.byref(self.h)), "birp_server_create")
         self._bufs = {}

@@ -159,6 +159,7 @@ int pinned_staging(birp_device* dev, size_t bytes) {
         dev->pinned_bytes = 0;
         hipError_t e = hipHostMalloc(&dev->pinned, bytes, hipHostMallocDefault);
         if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(staging)");
+        memset(dev->pinned, 0, bytes);  // fault the pages in now, not on the first render's copy
         dev->pinned_bytes = bytes;
     }
     return BIRP_OK;

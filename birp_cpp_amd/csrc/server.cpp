@@ -97,18 +97,23 @@ struct birp_server {
 
 namespace {
 
-int slot_reserve_images(Slot& s, size_t pixels) {
+// Grows slot s to hold `pixels` image pixels; on demand with headroom (hipFree synchronises the
+// device, so regrowth should be rare), or exactly when reserving at creation.
+int slot_reserve_images(Slot& s, size_t pixels, bool exact = false) {
     if (s.image_cap >= pixels) return BIRP_OK;
     if (s.d_images) (void)hipFree(s.d_images);
     if (s.h_images) (void)hipHostFree(s.h_images);
     s.d_images = nullptr;
     s.h_images = nullptr;
     s.image_cap = 0;
-    size_t cap = std::max(pixels + pixels / 2, (size_t)1 << 22);  // headroom: hipFree synchronises the device
+    size_t cap = exact ? pixels : std::max(pixels + pixels / 2, (size_t)1 << 22);
     hipError_t e = hipMalloc(&s.d_images, cap * 4);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc(server images)");
     e = hipHostMalloc(&s.h_images, cap * 4, hipHostMallocDefault);
     if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(server images)");
+    // touch the staging now: the first host access of each page of a fresh pinned allocation
+    // faults it into the process, which would otherwise land on the first batches' copy-out
+    memset(s.h_images, 0, cap * 4);
     s.image_cap = cap;
     return BIRP_OK;
 }
@@ -319,6 +324,7 @@ extern "C" int birp_server_create(birp_device* dev, const birp_server_config* cf
     if (!dev || !cfg || !out) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
     if (cfg->max_batch < 1 || cfg->max_batch > 4096 || cfg->max_inflight < 1 || cfg->max_inflight > 8)
         return fail(BIRP_ERR_INVALID_ARGUMENT, "max_batch in [1,4096], max_inflight in [1,8]");
+    if (cfg->reserve_pixels > ((size_t)1 << 32)) return fail(BIRP_ERR_INVALID_ARGUMENT, "reserve_pixels above 2^32");
     (void)hipSetDevice(dev->ordinal);
     birp_server* S = new birp_server();
     S->dev = dev;
@@ -339,6 +345,7 @@ extern "C" int birp_server_create(birp_device* dev, const birp_server_config* cf
         if ((e = hipHostMalloc(&s.h_prefix, sizeof(int) * BIRP_KDT_COUNT * (nb + 1), hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
         if ((e = hipMalloc(&s.d_prefix, sizeof(int) * BIRP_KDT_COUNT * (nb + 1))) != hipSuccess) { rc = hip_fail(e, "prefix"); break; }
         if ((e = hipMalloc(&s.d_work, sizeof(unsigned) * BIRP_KDT_COUNT * 256)) != hipSuccess) { rc = hip_fail(e, "counter"); break; }
+        if (cfg->reserve_pixels && (rc = slot_reserve_images(s, cfg->reserve_pixels, true))) break;
     }
     if (rc) {
         birp_server_destroy(S);

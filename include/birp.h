@@ -216,6 +216,9 @@ typedef struct birp_server_config {
     int max_batch;         /* most requests one launch may carry */
     int max_inflight;      /* launches in flight at once (each on its own stream) */
     double batch_window_ms;/* wait this long for more arrivals before launching a non-full batch */
+    size_t reserve_pixels; /* image pixels each slot allocates at creation (device + pinned
+                              staging), so that batches up to that size allocate nothing while
+                              serving; 0: slots grow on demand */
 } birp_server_config;
 
 typedef struct birp_request {

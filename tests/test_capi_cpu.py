@@ -147,3 +147,35 @@ def test_workload_definitions():
     t, which, views = W.requests_config5(1000)
     rate = (len(t) - 1) / (t[-1] - t[0])
     assert rate == pytest.approx(200, rel=0.15) and which.max() < 8
+
+
+STRUCTS = {"VolumeDesc": "birp_volume_desc", "CmemState": "birp_cmem_state", "CmemParams": "birp_cmem_params",
+           "View": "birp_view", "RenderStats": "birp_render_stats", "Tuning": "birp_tuning",
+           "ServerConfig": "birp_server_config", "Request": "birp_request", "ResultInfo": "birp_result_info",
+           "ShellParams": "birp_shell_params"}
+
+
+def test_ctypes_structs_match_header_layout(tmp_path):
+    """Every ctypes mirror in capi.py has the size, field names and field offsets that gcc gives
+    the C struct of include/birp.h (an appended or reordered field breaks the ABI)."""
+    import subprocess
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "birp.h"', "int main(void) {"]
+    for py, c in STRUCTS.items():
+        lines.append(f'    printf("{py} size %zu\\n", sizeof({c}));')
+        for name, _ in getattr(capi, py)._fields_:
+            lines.append(f'    printf("{py} {name} %zu\\n", offsetof({c}, {name}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n"):
+        if line:
+            py, name, val = line.split()
+            got[(py, name)] = int(val)
+    for py in STRUCTS:
+        cls = getattr(capi, py)
+        assert got[(py, "size")] == capi.C.sizeof(cls), py
+        for name, _ in cls._fields_:
+            assert got[(py, name)] == getattr(cls, name).offset, (py, name)

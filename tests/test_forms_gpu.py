@@ -224,7 +224,8 @@ def test_server_concurrent_clients(device, cube1):
     every result equals the same request rendered alone, and requests share launches."""
     import threading
     vals, org, spc, vols, oc = cube1
-    srv = capi.Server(device, memory_budget=4 << 30, max_batch=32, max_inflight=3, batch_window_ms=0.5)
+    srv = capi.Server(device, memory_budget=4 << 30, max_batch=32, max_inflight=3, batch_window_ms=0.5,
+                      reserve_pixels=32 * 576 * 336)
     try:
         idx = [srv.add_volume(vols[capi.DTYPE_F16]), srv.add_volume(vols[capi.DTYPE_F32]), srv.add_volume(vols[capi.DTYPE_BF16])]
         views = W.views_config3(count=60, seed=11)
