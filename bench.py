@@ -77,7 +77,7 @@ PEAK_VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9  # non-packed FP32 issue: CUs x SIMDs 
 def valu_roofline(kern_ms):
     """VALU issue rate of the plain kernel: the committed rocprofv3 instruction count per launch
     (SQ_INSTS_VALU, 64 lanes each) over this run's measured kernel time."""
-    name = "plain_exact_config2_valu_pmc.txt"
+    name = "plain_exact_256cube_576x336_valu_pmc.txt"
     insts = read_profile_value(name, "valu_wave_insts_per_launch")
     if insts is None:
         return None
@@ -89,13 +89,13 @@ def valu_roofline(kern_ms):
 
 def read_traffic():
     """HBM bytes per launch of the plain kernel from the committed rocprofv3 --pmc summary."""
-    return read_profile_value("plain_exact_config2_pmc.txt", "traffic_bytes_per_launch")
+    return read_profile_value("plain_exact_256cube_576x336_pmc.txt", "traffic_bytes_per_launch")
 
 
 def bench_plain(capi, W, dev, steps, warmup):
-    vals, org, spc, _ = W.cube_config2()
+    vals, org, spc, _ = W.cube_256()
     vol = capi.Volume.upload(dev, vals, org, spc)
-    v = W.view_config2()
+    v = W.view_576x336()
     img = capi.Image(dev, v.width, v.height)
     host, st = vol.render(v)
     n_in = st.samples_in_cube
@@ -135,7 +135,7 @@ def cpu_baseline(O, W, capi, vals, org, spc, v, n_in):
 def cpu_reference(O, vals, org, spc, v, ref32, gpu_img, rows=(112, 224)):
     """The reference's own DataCube::GetSample (oracle/_ref, compiled from its sources) inside the
     Camera::Render loop, single-threaded as the reference runs, on the middle third of the
-    config-2 image rows; its rows are checked against the oracle's and the GPU's images."""
+    576x336 image rows of the 256^3 cube; its rows are checked against the oracle's and the GPU's images."""
     if not os.path.exists(O.REF_PATH):
         return None
     ov = O.make_view(v.width, v.height, v.fov_w, v.fov_h, list(v.position), list(v.aim), v.ray_length, v.samples)
@@ -152,14 +152,14 @@ def cpu_reference(O, vals, org, spc, v, ref32, gpu_img, rows=(112, 224)):
                 np.array_equal(img[part].view(np.uint32), gpu_img[part].view(np.uint32)))
     return {"value": round(cnt / el / 1e9, 6), "unit": "Gsamples/s", "cores": 1, "kind": "reference",
             "ns_per_sample": round(el / cnt * 1e9, 1), "rows_bitwise_equal_to_gpu_and_oracle": same,
-            "sample": f"rows {rows[0]}..{rows[1] - 1} of the 576x336 config-2 render ({cnt} in-cube samples) with the "
+            "sample": f"rows {rows[0]}..{rows[1] - 1} of the 576x336 render of the 256^3 cube ({cnt} in-cube samples) with the "
                       f"reference's DataCube::GetSample compiled from its sources, in the Camera::Render loop, "
                       f"1 thread, {el:.2f} s"}
 
 
 def bench_batched(capi, W, dev, vals, org, spc, fp32_vol):
     vol16 = capi.Volume.upload(dev, vals, org, spc, dtype=capi.DTYPE_F16)
-    views = W.views_config3(256)
+    views = W.orbit_views_mixed(256)
     tune = os.path.join(ROOT, "birp_cpp_amd", "tuning.jsonl")
     if os.path.exists(tune):
         capi.lib().birp_autotune_load(tune.encode())
@@ -202,11 +202,11 @@ def bench_batched(capi, W, dev, vals, org, spc, fp32_vol):
 
 
 def bench_cmem(capi, W, dev, steps=20):
-    """The analytic CMEM model (default Init) rendered by the exact path over the config-2 image
+    """The analytic CMEM model (default Init) rendered by the exact path over the 576x336 image
     and ray grid: every sample is evaluated (the model has no bounds), about 25 transcendentals
     each, so this is VALU-bound."""
     vol = capi.Volume.cmem(dev, capi.cmem_init())
-    v = W.view_config2()
+    v = W.view_576x336()
     img = capi.Image(dev, v.width, v.height)
     host, st = vol.render(v)
     ref64 = vol.render_f64(v)
@@ -228,11 +228,11 @@ def bench_cmem(capi, W, dev, steps=20):
 
 
 def bench_precision(capi, W, dev, n=512, count=64):
-    vals, org, spc, _ = W.cube_config4(n)
+    vals, org, spc, _ = W.cube_512_wide_range(n)
     vols = {name: capi.Volume.upload(dev, vals, org, spc, dtype=dt)
             for name, dt in (("fp32", capi.DTYPE_F32), ("bf16", capi.DTYPE_BF16), ("f16", capi.DTYPE_F16))}
     del vals
-    views = W.views_config4(count)
+    views = W.orbit_views_576x336_n1600(count)
     out = {"cube": f"{n}^3", "poses": count, "image": "576x336", "samples_per_ray": views[0].samples}
     errs = {k: [] for k in ("exact_fp32", "fast_fp32", "bf16", "f16")}
     mx = {k: [] for k in errs}
@@ -307,12 +307,12 @@ def bench_serving(capi, W, dev, num=2000, rate=200.0):
     through the plain path (one birp_render and host sync per request) for the renders/s
     comparison."""
     cubes = []
-    for k, (n, dt) in enumerate(W.cubes_config5()):
+    for k, (n, dt) in enumerate(W.serving_cubes()):
         vals, org, spc, _ = capi.synth_shell(n, (7.0, 12.0), 0.1, seed=500 + k)
         cubes.append(capi.Volume.upload(dev, vals, org, spc, dtype=dt))
         del vals
     budget = sum(c.nbytes() for c in cubes) + (256 << 20)
-    t_arr, which, views = W.requests_config5(num, rate)
+    t_arr, which, views = W.serving_requests(num, rate)
     # six launches in flight; each slot reserves a full batch of the largest image at creation,
     # so no allocation happens while serving
     max_px = max(v.width * v.height for v in views)
@@ -359,14 +359,14 @@ def main():
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 3}))
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} requested)", file=sys.stderr)
+        print(f"bench.py measures one GPU: there is no multi-GPU path (--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
 
     from birp_cpp_amd import capi
     from birp_cpp_amd import workloads as W
 
     dev = capi.Device(0)
-    log("plain: config 2 exact renders")
+    log("plain: 256^3 cube, 576x336 exact renders")
     p = bench_plain(capi, W, dev, args.steps, args.warmup)
     v, n_in = p["view"], p["n_in"]
     ms_step = p["wall"] * 1e3 / args.steps
@@ -409,7 +409,7 @@ def main():
     p["vol"].free()
     del p["vals"]
     if "cmem" in forms:
-        log("cmem: analytic model, config-2 grid")
+        log("cmem: analytic model, 576x336 x 800 samples")
         line["forms"]["cmem"] = bench_cmem(capi, W, dev)
     if "precision" in forms:
         log("precision: 512^3 FP32/BF16/F16")

@@ -3,7 +3,7 @@
  *
  * This is the oracle the HIP renderer is checked against. It is never linked into, loaded
  * by, or called from the product library; only tests/, __graft_entry__.smoke() and
- * bench.py's cpu_baseline leg use it, and only as the checker or the timed CPU baseline.
+ * bench.py's cpu_baseline leg use it, and only to check results or as the timed CPU baseline.
  *
  * Two renderers live here:
  *   or_render_f32  -- follows the reference's operation order and float/double types

@@ -1,7 +1,7 @@
 """ctypes front end to the oracle (liboracle.so) and to the reference build (oracle/_ref).
 
 TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
-import this module, and only as the checker or as the timed CPU baseline; the product package
+import this module, and only to check results or as the timed CPU baseline; the product package
 never does. See birp_oracle.h for what each function restates and what pins it.
 """
 from __future__ import annotations

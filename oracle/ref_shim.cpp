@@ -2,8 +2,9 @@
 // compiled together with their sources (src/DataCube.cpp, src/Cmem.cpp, src/Helper.cpp) into
 // oracle/_ref/libbirp_ref.so by oracle/Makefile. TEST INFRASTRUCTURE ONLY: it is used to pin
 // the oracle (birp_oracle.c) against the reference, to write golden fixtures, and as the timed
-// reference CPU baseline of bench.py (ref_render_rows); the product never loads it. src/Camera.cpp is not part of this build: it includes EleFits headers
-// (Camera.cpp:23-27), which this image does not have.
+// reference CPU baseline of bench.py (ref_render_rows); the product never loads it.
+// src/Camera.cpp is not compiled here: it includes EleFits headers (Camera.cpp:23-27), which
+// this image does not have.
 #include <cmath>
 #include <limits>
 #include <string>

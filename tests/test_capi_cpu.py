@@ -133,18 +133,18 @@ def test_orbit_geometry():
 
 
 def test_workload_definitions():
-    v1, v2 = W.view_config1(), W.view_config2()
+    v1, v2 = W.view_144x84(), W.view_576x336()
     assert (v1.width, v1.height, v1.samples) == (144, 84, 160)
     assert (v2.width, v2.height, v2.samples) == (576, 336, 800)
     assert O.image_dims(0.1845, 15.5, 26.5) == (144, 84)  # the reference's FOV rule (Camera.cpp:38-39)
-    vs = W.views_config3(256)
+    vs = W.orbit_views_mixed(256)
     assert len(vs) == 256 and {(v.width, v.height) for v in vs} == set(W.SIZES)
     assert {v.samples for v in vs} == {800, 320}
     assert all(v.math == capi.MATH_FAST for v in vs)
-    assert W.views_config4(4)[0].samples == 1600
-    cubes = W.cubes_config5()
+    assert W.orbit_views_576x336_n1600(4)[0].samples == 1600
+    cubes = W.serving_cubes()
     assert len(cubes) == 8 and all(n in (128, 256, 384, 512) for n, _ in cubes)
-    t, which, views = W.requests_config5(1000)
+    t, which, views = W.serving_requests(1000)
     rate = (len(t) - 1) / (t[-1] - t[0])
     assert rate == pytest.approx(200, rel=0.15) and which.max() < 8
 

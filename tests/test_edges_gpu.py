@@ -76,7 +76,7 @@ def test_ragged_cubes_fast_paths(device, shape, org, spc, pos):
 def test_longest_exact_ray_and_beyond(device, oracle):
     """n = 8192 is the exact path's largest ray (its t table lives in LDS): bitwise there, and a
     clear BIRP_ERR_UNSUPPORTED one sample beyond; the fast path takes longer rays."""
-    vals, org, spc, _ = W.cube_config1()
+    vals, org, spc, _ = W.cube_128()
     vol = capi.Volume.upload(device, vals, org, spc)
     try:
         oc = oracle.OracleCube(vals, org, spc)

@@ -34,7 +34,7 @@ def oview(O, v):
 
 @pytest.fixture(scope="module")
 def cube1(device, oracle):
-    vals, org, spc, _ = W.cube_config1()
+    vals, org, spc, _ = W.cube_128()
     vols = {dt: capi.Volume.upload(device, vals, org, spc, dtype=dt) for dt in (capi.DTYPE_F32, capi.DTYPE_BF16, capi.DTYPE_F16)}
     yield vals, org, spc, vols, oracle.OracleCube(vals, org, spc)
     for v in vols.values():
@@ -76,7 +76,7 @@ def test_cmem_points_and_render(device, oracle, params):
 # ---------------------------------------------------------------------------------- FP64
 def test_f64_render_matches_oracle_f64(cube1, oracle):
     vals, org, spc, vols, oc = cube1
-    v = W.view_config1()
+    v = W.view_144x84()
     got = vols[capi.DTYPE_F32].render_f64(v)
     ref, _ = oracle.render_f64(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT)
     assert rel_l2(got, ref) <= 1e-12
@@ -140,7 +140,7 @@ def test_fast_path_reference_semantics(cube1, oracle):
 # ---------------------------------------------------------------------------------- batch
 def test_batch_matches_fp64_and_replays(cube1):
     vals, org, spc, vols, oc = cube1
-    views = W.views_config3(count=24)
+    views = W.orbit_views_mixed(count=24)
     b = capi.Batch(vols[capi.DTYPE_F16], views)
     try:
         b.launch_eager()  # the same kernels without the graph
@@ -152,7 +152,7 @@ def test_batch_matches_fp64_and_replays(cube1):
         # every graph replay, with the program's own eager work and syncs in between, equals
         # the eager launch bit for bit
         for rep in range(4):
-            _ = vols[capi.DTYPE_F32].render(W.view_config1())
+            _ = vols[capi.DTYPE_F32].render(W.view_144x84())
             b.launch()
             b.wait()
             for k in range(len(views)):
@@ -169,7 +169,7 @@ def test_batch_matches_fp64_and_replays(cube1):
 @pytest.mark.parametrize("tuning", [(8, 8, 1, 1), (16, 4, 1, 2), (8, 4, 2, 4), (4, 8, 2, 1), (4, 4, 4, 2), (8, 2, 4, 4)])
 def test_batch_tunings_agree(cube1, tuning):
     vals, org, spc, vols, oc = cube1
-    views = W.views_config3(count=6)
+    views = W.orbit_views_mixed(count=6)
     b = capi.Batch(vols[capi.DTYPE_F32], views)
     try:
         b.set_tuning(capi.Tuning(*tuning))
@@ -184,7 +184,7 @@ def test_batch_tunings_agree(cube1, tuning):
 
 def test_autotune_roundtrip(cube1, tmp_path):
     vals, org, spc, vols, oc = cube1
-    views = W.views_config3(count=8)
+    views = W.orbit_views_mixed(count=8)
     path = str(tmp_path / "tune.jsonl")
     capi.autotune(vols[capi.DTYPE_F16], views, path)
     assert capi.lib().birp_autotune_load(path.encode()) == capi.OK
@@ -200,7 +200,7 @@ def test_server_alone_equals_in_mix(device, cube1):
     try:
         i16 = srv.add_volume(vols[capi.DTYPE_F16])
         i32 = srv.add_volume(vols[capi.DTYPE_F32])
-        views = W.views_config3(count=20)
+        views = W.orbit_views_mixed(count=20)
         alone = []
         for k, v in enumerate(views):
             rid = srv.submit(i16 if k % 2 else i32, v)
@@ -228,7 +228,7 @@ def test_server_concurrent_clients(device, cube1):
                       reserve_pixels=32 * 576 * 336)
     try:
         idx = [srv.add_volume(vols[capi.DTYPE_F16]), srv.add_volume(vols[capi.DTYPE_F32]), srv.add_volume(vols[capi.DTYPE_BF16])]
-        views = W.views_config3(count=60, seed=11)
+        views = W.orbit_views_mixed(count=60, seed=11)
         which = [k % 3 for k in range(len(views))]
         alone = []
         for v, w in zip(views, which):

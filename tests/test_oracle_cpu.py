@@ -400,7 +400,7 @@ def test_division_shortcut_exhaustive(exact_checker, spacing):
 @pytest.mark.parametrize("spacing", ["0.236220479", "0.117647059", "0.0587084144", "0.99999994", "3.0", "1e-07"])
 def test_double_product_division_exhaustive(exact_checker, spacing):
     """birp_div_rn64(a, RN64(1/b)) == a / b for every float a in [2^-64, 2^64): the exact kernels'
-    division (the proof is in exact_math.h; this checks it for the configurations' spacings, an
+    division (the proof is in exact_math.h; this checks it for the workloads' spacings, an
     all-ones significand, an odd integer and a small spacing)."""
     r = subprocess.run([exact_checker, "div64range", spacing], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout

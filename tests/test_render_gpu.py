@@ -37,8 +37,8 @@ def assert_bitwise(img, ref):
 
 
 @pytest.fixture(scope="module")
-def cfg1(device, oracle):
-    vals, org, spc, _ = W.cube_config1()
+def shell128(device, oracle):
+    vals, org, spc, _ = W.cube_128()
     vol = capi.Volume.upload(device, vals, org, spc)
     oc = oracle.OracleCube(vals, org, spc)
     yield vol, oc
@@ -46,8 +46,8 @@ def cfg1(device, oracle):
 
 
 @pytest.fixture(scope="module")
-def cfg2(device, oracle):
-    vals, org, spc, _ = W.cube_config2()
+def shell256(device, oracle):
+    vals, org, spc, _ = W.cube_256()
     vol = capi.Volume.upload(device, vals, org, spc)
     oc = oracle.OracleCube(vals, org, spc)
     yield vol, oc
@@ -59,12 +59,12 @@ INCUBE_POSES = [((15.0, 5.0, 8.0), (0.0, 0.0, 0.0)), ((-5.0, 10.0, -12.0), (10.0
                 ((2.5, -7.5, 3.0), (12.0, 0.0, 0.0)), ((8.0, 0.5, -0.5), (-4.0, 0.0, 0.0))]
 
 
-def test_config1_extended_bitwise(cfg1, oracle):
-    vol, oc = cfg1
-    v = W.view_config1()
+def test_128cube_144x84_extended_bitwise(shell128, oracle):
+    vol, oc = shell128
+    v = W.view_144x84()
     img, st = vol.render(v)
     ref, cnt = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT)
-    print(f"config1: in-cube {st.samples_in_cube} oracle {cnt} nominal {st.samples_nominal} kernel {st.kernel_ms:.3f} ms")
+    print(f"128^3 144x84: in-cube {st.samples_in_cube} oracle {cnt} nominal {st.samples_nominal} kernel {st.kernel_ms:.3f} ms")
     assert st.samples_in_cube == cnt > 0
     assert rel_l2(img, ref) <= REL_TOL_FP32
     assert_bitwise(img, ref)
@@ -72,8 +72,8 @@ def test_config1_extended_bitwise(cfg1, oracle):
 
 @pytest.mark.parametrize("pose", INCUBE_POSES)
 @pytest.mark.parametrize("semantics", [0, 1])
-def test_incube_poses_bitwise(cfg1, oracle, pose, semantics):
-    vol, oc = cfg1
+def test_incube_poses_bitwise(shell128, oracle, pose, semantics):
+    vol, oc = shell128
     v = W.view(144, 84, 0.25, pos=pose[0], aim=pose[1], semantics=semantics, length=44.0)
     img, st = vol.render(v)
     ref, cnt = oracle.render_f32(oc, oview(oracle, v), semantics, nthreads=NT)
@@ -81,8 +81,8 @@ def test_incube_poses_bitwise(cfg1, oracle, pose, semantics):
     assert_bitwise(img, ref)
 
 
-def test_reference_semantics_outside_camera_is_zero(cfg1, oracle):
-    vol, oc = cfg1
+def test_reference_semantics_outside_camera_is_zero(shell128, oracle):
+    vol, oc = shell128
     v = W.view(144, 84, 0.5, semantics=capi.SEM_REFERENCE)  # camera at y=-20 is outside the cube
     img, st = vol.render(v)
     ref, cnt = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_REFERENCE, nthreads=NT)
@@ -90,9 +90,9 @@ def test_reference_semantics_outside_camera_is_zero(cfg1, oracle):
     assert not img.any() and not ref.any()
 
 
-def test_reference_fixed_ray_200_samples_L44(cfg1, oracle):
+def test_reference_fixed_ray_200_samples_L44(shell128, oracle):
     """Camera::Render's own ray grid: L = 44, n = 200 (Camera.cpp:31,65), pixel 0.25 deg, 36x36 deg."""
-    vol, oc = cfg1
+    vol, oc = shell128
     W_, H_ = oracle.image_dims(0.25, 36.0, 36.0)
     assert (W_, H_) == (144, 144)
     v = capi.make_view(W_, H_, 36.0, 36.0, (12.0, 4.0, 6.0), (1.0, 0.0, 0.0), 44.0, 200, capi.SEM_REFERENCE)
@@ -101,23 +101,23 @@ def test_reference_fixed_ray_200_samples_L44(cfg1, oracle):
     assert_bitwise(img, ref)
 
 
-def test_config2_full_size_bitwise(cfg2, oracle):
-    vol, oc = cfg2
-    v = W.view_config2()
+def test_256cube_576x336_full_size_bitwise(shell256, oracle):
+    vol, oc = shell256
+    v = W.view_576x336()
     img, st = vol.render(v)
     ref, cnt = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT)
-    print(f"config2: in-cube {st.samples_in_cube} oracle {cnt} nominal {st.samples_nominal} kernel {st.kernel_ms:.3f} ms")
+    print(f"256^3 576x336: in-cube {st.samples_in_cube} oracle {cnt} nominal {st.samples_nominal} kernel {st.kernel_ms:.3f} ms")
     assert st.samples_in_cube == cnt > 0
     assert rel_l2(img, ref) <= REL_TOL_FP32
     assert_bitwise(img, ref)
 
 
-def test_accumulate_matches_repeated_render(cfg1, oracle):
+def test_accumulate_matches_repeated_render(shell128, oracle):
     """Camera::Render never clears the image (Camera.cpp:223): a second render adds onto the first."""
-    vol, oc = cfg1
-    v = W.view_config1()
+    vol, oc = shell128
+    v = W.view_144x84()
     first, _ = vol.render(v)
-    v2 = W.view_config1()
+    v2 = W.view_144x84()
     v2.flags = capi.RENDER_ACCUMULATE
     second, _ = vol.render(v2, init=first)
     ref1, _ = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT)
@@ -126,11 +126,11 @@ def test_accumulate_matches_repeated_render(cfg1, oracle):
 
 
 def test_nearest_bitwise(device, oracle):
-    vals, org, spc, _ = W.cube_config1()
+    vals, org, spc, _ = W.cube_128()
     vol = capi.Volume.upload(device, vals, org, spc, interp=capi.INTERP_NEAREST)
     try:
         oc = oracle.OracleCube(vals, org, spc, trilinear=False)
-        v = W.view_config1()
+        v = W.view_144x84()
         img, st = vol.render(v)
         ref, cnt = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT)
         assert st.samples_in_cube == cnt
@@ -164,14 +164,14 @@ def test_signed_cube_with_zeros_bitwise(device, oracle, semantics):
 def test_nonfinite_cube(device, oracle):
     """inf/nan voxels take the [z][y][x] layout with floor/ceil corners: results match the oracle,
     NaN where the oracle has NaN."""
-    vals, org, spc, _ = W.cube_config1()
+    vals, org, spc, _ = W.cube_128()
     vals = vals.copy()
     vals[64, 60:70, 40:90] = np.nan
     vals[30, 10, 10:20] = np.inf
     vol = capi.Volume.upload(device, vals, org, spc)
     try:
         oc = oracle.OracleCube(vals, org, spc)
-        v = W.view_config1()
+        v = W.view_144x84()
         img, _ = vol.render(v)
         ref, _ = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT)
         assert np.array_equal(np.isnan(img), np.isnan(ref))
@@ -182,9 +182,9 @@ def test_nonfinite_cube(device, oracle):
         vol.free()
 
 
-def test_get_sample_points_bitwise(cfg1, oracle):
+def test_get_sample_points_bitwise(shell128, oracle):
     """Space::GetSample at arbitrary points (DataCube.cpp:203-277), incl. out-of-bounds (-1)."""
-    vol, oc = cfg1
+    vol, oc = shell128
     rng = np.random.default_rng(11)
     pts = np.column_stack([rng.uniform(-12, 22, 20000), rng.uniform(-17, 17, 20000), rng.uniform(-17, 17, 20000)]).astype(np.float32)
     # grid points, the upper boundary plane and corners
@@ -198,18 +198,18 @@ def test_get_sample_points_bitwise(cfg1, oracle):
     assert (ref == -1).any() and (ref > 0).any()
 
 
-def test_count_samples_matches_render(cfg2):
-    vol, _ = cfg2
-    v = W.view_config2()
+def test_count_samples_matches_render(shell256):
+    vol, _ = shell256
+    v = W.view_576x336()
     _, st = vol.render(v)
     assert vol.count_samples(v) == st.samples_in_cube
 
 
-def test_camera_at_aim_point_and_invalid_views(cfg1, oracle):
+def test_camera_at_aim_point_and_invalid_views(shell128, oracle):
     """A camera at its aim point: Orient's unit vector is 0/0, its Sun projection falls back to
     xo = yo = 0 (Camera.cpp:499-504), so the roll is 270 deg and the basis stays finite; the
     reference renders it, and so do we, identically."""
-    vol, oc = cfg1
+    vol, oc = shell128
     v = W.view(16, 16, 0.5, pos=(10.0, 0.0, 0.0), aim=(10.0, 0.0, 0.0))
     img, _ = vol.render(v)
     ref, _ = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED)
@@ -225,9 +225,9 @@ def test_camera_at_aim_point_and_invalid_views(cfg1, oracle):
     assert e.value.status == capi.ERR_INVALID_ARGUMENT
 
 
-def test_render_async_into_device_image(device, cfg1, oracle):
-    vol, oc = cfg1
-    v = W.view_config1()
+def test_render_async_into_device_image(device, shell128, oracle):
+    vol, oc = shell128
+    v = W.view_144x84()
     img = capi.Image(device, v.width, v.height)
     try:
         capi.render_async(vol, v, img)
