@@ -112,6 +112,28 @@ def test_256cube_576x336_full_size_bitwise(shell256, oracle):
     assert_bitwise(img, ref)
 
 
+def test_512cube_exact_bitwise_and_f16_storage(device, oracle):
+    """The largest workload cube: 512^3 with a 1e-8..1e-4 dynamic range (2.4 GB of quad bricks
+    in Morton order). A 576x336 orbit view with 1600 samples per ray is bit-identical to the
+    oracle on the exact path, and within 1e-4 of the FP64 render from F16 storage."""
+    vals, org, spc, _ = W.cube_512_wide_range()
+    vol = capi.Volume.upload(device, vals, org, spc)
+    v16 = capi.Volume.upload(device, vals, org, spc, dtype=capi.DTYPE_F16)
+    try:
+        v = W.orbit_views_576x336_n1600(1, math=capi.MATH_EXACT)[0]
+        img, st = vol.render(v)
+        ref, cnt = oracle.render_f32(oracle.OracleCube(vals, org, spc), oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT)
+        print(f"512^3 576x336 n=1600: in-cube {st.samples_in_cube} kernel {st.kernel_ms:.3f} ms")
+        assert st.samples_in_cube == cnt > 0
+        assert_bitwise(img, ref)
+        v.math = capi.MATH_FAST
+        f16, _ = v16.render(v)
+        assert rel_l2(f16, vol.render_f64(v)) <= 1e-4
+    finally:
+        vol.free()
+        v16.free()
+
+
 def test_accumulate_matches_repeated_render(shell128, oracle):
     """Camera::Render never clears the image (Camera.cpp:223): a second render adds onto the first."""
     vol, oc = shell128
