@@ -2,7 +2,9 @@
 #include "Camera.h"
 
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <stdexcept>
 #include <string>
@@ -72,17 +74,36 @@ void Camera::Render() {
     v.samples = ray_samples;
     v.semantics = semantics_;
     v.math = math_;
-    v.flags = (math_ == BIRP_MATH_EXACT) ? BIRP_RENDER_ACCUMULATE : 0;
-    std::vector<float> buf((size_t)W * H);
-    for (int j = 0; j < H; j++)
-        for (int i = 0; i < W; i++) buf[(size_t)j * W + i] = image[j][i];
-    birp_render_stats st{};
-    birp_api::check(birp_render(vol, &v, buf.data(), &st), "Camera::Render");
-    for (int j = 0; j < H; j++)
-        for (int i = 0; i < W; i++) {
-            // the fast form renders a fresh image; keep the accumulate contract on the host
-            image[j][i] = (math_ == BIRP_MATH_EXACT) ? buf[(size_t)j * W + i] : image[j][i] + buf[(size_t)j * W + i];
+    if ((int)image.size() != H) throw std::invalid_argument("Camera::Render: image has the wrong number of rows");
+    for (const auto& row : image)
+        if ((int)row.size() != W) throw std::invalid_argument("Camera::Render: image row of the wrong length");
+    staging_.resize((size_t)W * H);
+    float* buf = staging_.data();
+    v.flags = 0;
+    if (math_ == BIRP_MATH_EXACT) {
+        // the exact render adds to the image on the device (Camera.cpp:223); an image of +0.0
+        // bits (a fresh camera) is the same as starting from zero, so it is not uploaded
+        uint32_t any = 0;
+        for (int j = 0; j < H; j++) {
+            std::memcpy(buf + (size_t)j * W, image[j].data(), (size_t)W * 4);
+            for (int i = 0; i < W; i++) {
+                uint32_t u;
+                std::memcpy(&u, &buf[(size_t)j * W + i], 4);
+                any |= u;
+            }
         }
+        if (any) v.flags = BIRP_RENDER_ACCUMULATE;
+    }
+    birp_render_stats st{};
+    birp_api::check(birp_render(vol, &v, buf, &st), "Camera::Render");
+    for (int j = 0; j < H; j++) {
+        if (math_ == BIRP_MATH_EXACT) {
+            std::memcpy(image[j].data(), buf + (size_t)j * W, (size_t)W * 4);
+        } else {
+            // the fast form renders a fresh image; keep the accumulate contract on the host
+            for (int i = 0; i < W; i++) image[j][i] += buf[(size_t)j * W + i];
+        }
+    }
     last_in_cube_ = st.samples_in_cube;
     last_ms_ = st.kernel_ms;
 }

@@ -598,10 +598,17 @@ extern "C" int birp_render(birp_volume* vol, const birp_view* view, float* image
     if ((rc = device_scratch(dev, npix * sizeof(float), &buf))) return rc;
     float* dimg = (float*)buf;
     hipError_t e;
-    if (view->flags & BIRP_RENDER_ACCUMULATE) {
-        if ((e = hipMemcpyAsync(dimg, image, npix * 4, hipMemcpyHostToDevice, dev->stream)) != hipSuccess)
+    birp_view vv = *view;
+    if (vv.flags & BIRP_RENDER_ACCUMULATE) {
+        // an image of +0.0 bits adds nothing: render as a fresh image instead of uploading it
+        const uint32_t* bits = reinterpret_cast<const uint32_t*>(image);
+        size_t k = 0;
+        while (k < npix && bits[k] == 0u) k++;
+        if (k == npix) vv.flags &= ~BIRP_RENDER_ACCUMULATE;
+        else if ((e = hipMemcpyAsync(dimg, image, npix * 4, hipMemcpyHostToDevice, dev->stream)) != hipSuccess)
             return hip_fail(e, "hipMemcpy(image)");
     }
+    view = &vv;
     unsigned long long* cnt = (unsigned long long*)dev->d_counter + 1;
     if ((rc = launch_zero_f32((float*)cnt, 2, dev->stream))) return fail(rc, "counter reset");
     (void)hipEventRecord(dev->ev0, dev->stream);

@@ -58,4 +58,5 @@ private:
     int math_ = 0;
     long long last_in_cube_ = 0;
     double last_ms_ = 0.0;
+    std::vector<float> staging_;  // the flat H x W image handed to the C ABI, kept between renders
 };

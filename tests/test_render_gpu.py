@@ -145,6 +145,13 @@ def test_accumulate_matches_repeated_render(shell128, oracle):
     ref1, _ = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT)
     ref2, _ = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT, init=ref1)
     assert_bitwise(second, ref2)
+    # an image of +0.0 bits is not uploaded (a fresh render); one of -0.0 is, and sums from -0.0
+    fresh, _ = vol.render(v2, init=np.zeros_like(first))
+    assert_bitwise(fresh, first)
+    negz = np.full_like(first, -0.0)
+    got, _ = vol.render(v2, init=negz)
+    ref3, _ = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT, init=negz)
+    assert_bitwise(got, ref3)
 
 
 def test_nearest_bitwise(device, oracle):
