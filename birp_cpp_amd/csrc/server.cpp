@@ -7,8 +7,8 @@
 // volumes, and launches them as one persistent-kernel batch (one launch per storage dtype). A
 // request that arrives while other batches are on the GPU is launched in the next free slot, so
 // its work overlaps theirs on the device. When a slot's device work is done, a completion thread
-// copies its images from pinned staging into the callers' buffers (split over a few threads for
-// large batches) and releases the slot, so the dispatcher keeps launching meanwhile. Every
+// copies its images from pinned staging into the callers' buffers (split over a pool of helper
+// threads for large batches) and releases the slot, so the dispatcher keeps launching meanwhile. Every
 // request is rendered by the same kernel tuning and the same per-tile arithmetic whatever it is
 // batched with, so results do not depend on the mix.
 // No reference counterpart: the reference renders one pose per process run (main.cpp:266-316).
@@ -20,6 +20,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -49,6 +50,61 @@ struct Request {
     int status = BIRP_OK;
     bool finished = false;
     bool waited = false;  // a thread is in birp_server_wait for it
+};
+
+// Fork-join pool for the completion thread's copy-out: helper threads wait for a job, copy one
+// part each, and the caller copies part 0 and waits for the rest (no thread start per batch).
+class CopyPool {
+public:
+    void start(int helpers) {
+        for (int k = 0; k < helpers; k++) th_.emplace_back([this] { loop(); });
+    }
+    void stop() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            quit_ = true;
+        }
+        cv_job_.notify_all();
+        for (auto& t : th_) t.join();
+        th_.clear();
+    }
+    int helpers() const { return (int)th_.size(); }
+    // fn(part) for every part in [0, parts); part 0 runs on the calling thread
+    void run(int parts, const std::function<void(int)>& fn) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = &fn;
+            parts_ = parts;
+            next_ = 1;
+            pending_ = parts - 1;
+        }
+        cv_job_.notify_all();
+        fn(0);
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_done_.wait(lk, [&] { return pending_ == 0; });
+        job_ = nullptr;
+    }
+
+private:
+    void loop() {
+        std::unique_lock<std::mutex> lk(mu_);
+        while (true) {
+            cv_job_.wait(lk, [&] { return quit_ || (job_ && next_ < parts_); });
+            if (quit_) return;
+            const int part = next_++;
+            const std::function<void(int)>* fn = job_;
+            lk.unlock();
+            (*fn)(part);
+            lk.lock();
+            if (--pending_ == 0) cv_done_.notify_one();
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_job_, cv_done_;
+    std::vector<std::thread> th_;
+    const std::function<void(int)>* job_ = nullptr;
+    int parts_ = 0, next_ = 0, pending_ = 0;
+    bool quit_ = false;
 };
 
 struct Slot {
@@ -88,6 +144,7 @@ struct birp_server {
     std::deque<Slot> slots;  // deque: slots hold atomics and never move
     std::thread worker;
     std::thread completer;
+    CopyPool copy_pool;  // helpers of the completion thread
     std::deque<Slot*> to_copy;  // guarded by mu
     std::condition_variable cv_copy;
     bool stop = false;
@@ -170,20 +227,20 @@ int launch_slot(birp_server* S, Slot& s) {
     return BIRP_OK;
 }
 
-// Copy the slot's images out of pinned staging; batches above 8 MB are split over up to 4 threads.
-void copy_out(Slot& s) {
+// Copy the slot's images out of pinned staging, split by bytes into parts of at least 2 MB over
+// the copy pool's threads.
+void copy_out(CopyPool& pool, Slot& s) {
     size_t total = 0;
     for (Request* r : s.reqs) total += r->pixels * 4;
+    const size_t n = s.reqs.size();
     auto run = [&s](size_t a, size_t b) {
         for (size_t k = a; k < b; k++) memcpy(s.reqs[k]->result, s.h_images + s.offsets[k], s.reqs[k]->pixels * 4);
     };
-    const size_t n = s.reqs.size();
-    const size_t parts = std::min<size_t>(n, total > ((size_t)8 << 20) ? 4 : 1);
+    const size_t parts = std::min<size_t>({n, (size_t)pool.helpers() + 1, std::max<size_t>(1, total >> 21)});
     if (parts <= 1) {
         run(0, n);
         return;
     }
-    // split by bytes, not by count, so the threads get similar amounts of copying
     std::vector<size_t> cut(1, 0);
     size_t acc = 0, next = total / parts;
     for (size_t k = 0; k < n && cut.size() < parts; k++) {
@@ -194,10 +251,10 @@ void copy_out(Slot& s) {
         }
     }
     cut.push_back(n);
-    std::vector<std::thread> th;
-    for (size_t p = 1; p + 1 < cut.size(); p++) th.emplace_back(run, cut[p], cut[p + 1]);
-    run(cut[0], cut[1]);
-    for (auto& t : th) t.join();
+    const std::function<void(int)> part = [&](int p) {
+        if ((size_t)p + 1 < cut.size()) run(cut[p], cut[p + 1]);
+    };
+    pool.run((int)cut.size() - 1, part);
 }
 
 void finish_slot(birp_server* S, Slot& s, int status) {
@@ -226,7 +283,7 @@ void completer(birp_server* S) {
             s = S->to_copy.front();
             S->to_copy.pop_front();
         }
-        copy_out(*s);
+        copy_out(S->copy_pool, *s);
         finish_slot(S, *s, BIRP_OK);
     }
 }
@@ -351,6 +408,7 @@ extern "C" int birp_server_create(birp_device* dev, const birp_server_config* cf
         birp_server_destroy(S);
         return rc;
     }
+    S->copy_pool.start(7);
     S->completer = std::thread(completer, S);
     S->worker = std::thread(dispatcher, S);
     *out = S;
@@ -375,6 +433,7 @@ extern "C" int birp_server_destroy(birp_server* S) {
         S->cv_copy.notify_all();
         S->completer.join();
     }
+    S->copy_pool.stop();
     for (Slot& s : S->slots) {
         if (s.stream) (void)hipStreamSynchronize(s.stream);
         if (s.d_images) (void)hipFree(s.d_images);
