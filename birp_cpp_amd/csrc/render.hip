@@ -855,7 +855,7 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
     unsigned* tab = s_tab[wave];
     const int ray = lane / S, sub = lane % S;
     const int tw = 1 << tw_log2;
-    const double K = BIRP_SCALE_FOLDED;
+    const float K = (float)BIRP_SCALE_FOLDED;
     int cached_vol = -1;
     int queue = blockIdx.x & 7;
     for (int tried = 0; tried < 8;) {
@@ -954,17 +954,18 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
                 fast_accumulate<DT>(V, rs, tab, xd, yd, zd, act, sum, nc);
             }
         }
-        // combine the S sub-lane partials in sub order, in double
-        double tot = (double)sum + (double)nc;
+        // each sub-lane's Kahan total, then the S partials combined in sub order; the step weight
+        // and the scale constant are applied once per ray (all in FP32)
+        float tot = sum + nc;
         if constexpr (S > 1) {
-            double t1 = __shfl_xor(tot, 1, 64);
+            float t1 = __shfl_xor(tot, 1, 64);
             tot = (sub & 1) ? t1 + tot : tot + t1;
             if constexpr (S == 4) {
-                double t2 = __shfl_xor(tot, 2, 64);
+                float t2 = __shfl_xor(tot, 2, 64);
                 tot = (sub & 2) ? t2 + tot : tot + t2;
             }
         }
-        if (valid && sub == 0) P.out[(size_t)j * P.W + i] = (float)(tot * (double)P.w * K);
+        if (valid && sub == 0) P.out[(size_t)j * P.W + i] = (tot * P.w) * K;
     }
 }
 
