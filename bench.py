@@ -108,11 +108,14 @@ def bench_plain(capi, W, dev, steps, warmup):
         capi.render_async(vol, v, img)
     dev_ms = dev.timer_end()  # waits for the last render
     wall = time.perf_counter() - t0
-    # the drop-in contract: one launch + wait + host copy per image
-    k2 = max(10, steps // 2)
+    # the drop-in contract: one launch + wait + host copy per image, into an image that persists
+    # across renders as Camera::image does
+    k2 = max(10, steps)
+    for _ in range(5):
+        vol.render(v, out=host)
     t1 = time.perf_counter()
     for _ in range(k2):
-        vol.render(v)
+        vol.render(v, out=host)
     sync_rps = k2 / (time.perf_counter() - t1)
     img.free()
     return dict(vol=vol, vals=vals, org=org, spc=spc, view=v, n_in=n_in, wall=wall, dev_ms=dev_ms, sync_rps=sync_rps,
