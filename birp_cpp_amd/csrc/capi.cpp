@@ -259,6 +259,7 @@ extern "C" int birp_device_close(birp_device* dev) {
     if (dev->scratch) (void)hipFree(dev->scratch);
     if (dev->scratch2) (void)hipFree(dev->scratch2);
     if (dev->pinned) (void)hipHostFree(dev->pinned);
+    if (dev->rbuf) (void)hipFree(dev->rbuf);
     if (dev->d_counter) (void)hipFree(dev->d_counter);
     for (hipEvent_t ev : {dev->ev0, dev->ev1, dev->ev_a, dev->ev_b})
         if (ev) (void)hipEventDestroy(ev);
@@ -571,6 +572,32 @@ extern "C" int birp_view_basis(const birp_view* view, float* angle_sun, float ro
     return ok ? BIRP_OK : fail(BIRP_ERR_SINGULAR, "Determinant is 0");
 }
 
+// The buffer of birp_render and birp_count_samples: a u64 in-cube sample counter at offset 0 and
+// the image at offset 256, so that one copy brings both back. The counter is not cleared per
+// render: each render reads how far it advanced. It is cleared when the buffer is (re)allocated
+// and after a render that failed on the device side.
+static int render_buffer(birp_device* dev, size_t npix, float** img, unsigned long long** cnt) {
+    const size_t need = 256 + npix * 4;
+    if (dev->rbuf_bytes < need) {
+        if (dev->rbuf) (void)hipFree(dev->rbuf);
+        dev->rbuf = nullptr;
+        dev->rbuf_bytes = 0;
+        dev->count_known = false;
+        hipError_t e = hipMalloc(&dev->rbuf, need);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(render buffer)");
+        dev->rbuf_bytes = need;
+    }
+    if (!dev->count_known) {
+        int rc = launch_zero_f32((float*)dev->rbuf, 2, dev->stream);
+        if (rc) return fail(rc, "counter reset");
+        dev->count_base = 0;
+        dev->count_known = true;
+    }
+    *cnt = (unsigned long long*)dev->rbuf;
+    *img = (float*)((char*)dev->rbuf + 256);
+    return BIRP_OK;
+}
+
 static int render_exact_into(birp_volume* vol, const birp_view* view, float* dimg, unsigned long long* counter) {
     birp_device* dev = vol->dev;
     if (vol->dv.kind == SK_CUBE && !vol->dv.linear) return fail(BIRP_ERR_UNSUPPORTED, "exact renders need an FP32 volume");
@@ -594,9 +621,9 @@ extern "C" int birp_render(birp_volume* vol, const birp_view* view, float* image
     birp_device* dev = vol->dev;
     (void)hipSetDevice(dev->ordinal);
     const size_t npix = (size_t)view->width * view->height;
-    void* buf = nullptr;
-    if ((rc = device_scratch(dev, npix * sizeof(float), &buf))) return rc;
-    float* dimg = (float*)buf;
+    float* dimg = nullptr;
+    unsigned long long* cnt = nullptr;
+    if ((rc = render_buffer(dev, npix, &dimg, &cnt))) return rc;
     hipError_t e;
     birp_view vv = *view;
     if (vv.flags & BIRP_RENDER_ACCUMULATE) {
@@ -609,31 +636,32 @@ extern "C" int birp_render(birp_volume* vol, const birp_view* view, float* image
             return hip_fail(e, "hipMemcpy(image)");
     }
     view = &vv;
-    unsigned long long* cnt = (unsigned long long*)dev->d_counter + 1;
-    if ((rc = launch_zero_f32((float*)cnt, 2, dev->stream))) return fail(rc, "counter reset");
     (void)hipEventRecord(dev->ev0, dev->stream);
     if (view->math == BIRP_MATH_EXACT) rc = render_exact_into(vol, view, dimg, cnt);
     else rc = render_fast_into(vol, view, dimg);
     if (rc) {
         (void)hipStreamSynchronize(dev->stream);
+        dev->count_known = false;
         return rc;
     }
     (void)hipEventRecord(dev->ev1, dev->stream);
-    // device -> pinned staging -> caller: one DMA at full link speed, one host copy
-    if ((rc = pinned_staging(dev, npix * 4 + 64))) return rc;
-    unsigned long long* hcount = (unsigned long long*)((char*)dev->pinned + npix * 4 + 32);
-    if ((e = hipMemcpyAsync(dev->pinned, dimg, npix * 4, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess)
-        return hip_fail(e, "hipMemcpy(image)");
-    if ((e = hipMemcpyAsync(hcount, cnt, 8, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess) return hip_fail(e, "hipMemcpy(count)");
-    if ((e = hipStreamSynchronize(dev->stream)) != hipSuccess) return hip_fail(e, "render");
-    memcpy(image, dev->pinned, npix * 4);
-    unsigned long long count = *hcount;
+    // device -> pinned staging -> caller: one DMA of the counter and the image, one host copy
+    if ((rc = pinned_staging(dev, 256 + npix * 4))) return rc;
+    if ((e = hipMemcpyAsync(dev->pinned, dev->rbuf, 256 + npix * 4, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(dev->stream)) != hipSuccess) {
+        dev->count_known = false;
+        return hip_fail(e, "render");
+    }
+    memcpy(image, (char*)dev->pinned + 256, npix * 4);
+    const unsigned long long count = *(const unsigned long long*)dev->pinned;
+    const unsigned long long added = count - dev->count_base;
+    dev->count_base = count;
     if (stats) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, dev->ev0, dev->ev1);
         stats->kernel_ms = ms;
         stats->samples_nominal = (long long)npix * view->samples;
-        stats->samples_in_cube = (view->math == BIRP_MATH_EXACT) ? (long long)count : -1;
+        stats->samples_in_cube = (view->math == BIRP_MATH_EXACT) ? (long long)added : -1;
     }
     return BIRP_OK;
 }
@@ -701,16 +729,24 @@ extern "C" int birp_count_samples(birp_volume* vol, const birp_view* view, long 
     if (rc) return rc;
     birp_device* dev = vol->dev;
     (void)hipSetDevice(dev->ordinal);
-    void* buf = nullptr;
-    if ((rc = device_scratch(dev, (size_t)v.width * v.height * 4, &buf))) return rc;
-    unsigned long long* cnt = (unsigned long long*)dev->d_counter + 1;
-    if ((rc = launch_zero_f32((float*)cnt, 2, dev->stream))) return rc;
-    if ((rc = render_exact_into(vol, &v, (float*)buf, cnt))) return rc;
-    unsigned long long c = 0;
-    hipError_t e = hipMemcpyAsync(&c, cnt, 8, hipMemcpyDeviceToHost, dev->stream);
+    float* dimg = nullptr;
+    unsigned long long* cnt = nullptr;
+    if ((rc = render_buffer(dev, (size_t)v.width * v.height, &dimg, &cnt))) return rc;
+    if ((rc = render_exact_into(vol, &v, dimg, cnt))) {
+        (void)hipStreamSynchronize(dev->stream);
+        dev->count_known = false;
+        return rc;
+    }
+    if ((rc = pinned_staging(dev, 256))) return rc;
+    hipError_t e = hipMemcpyAsync(dev->pinned, cnt, 8, hipMemcpyDeviceToHost, dev->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
-    if (e != hipSuccess) return hip_fail(e, "count");
-    *in_cube = (long long)c;
+    if (e != hipSuccess) {
+        dev->count_known = false;
+        return hip_fail(e, "count");
+    }
+    const unsigned long long count = *(const unsigned long long*)dev->pinned;
+    *in_cube = (long long)(count - dev->count_base);
+    dev->count_base = count;
     return BIRP_OK;
 }
 

@@ -16,13 +16,17 @@ struct birp_device {
     hipStream_t stream = nullptr;      // render stream (default priority)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;   // per-render timing
     hipEvent_t ev_a = nullptr, ev_b = nullptr; // birp_timer_begin/end
-    void* d_counter = nullptr;         // 64 B: class-scan flags, sample counter
+    void* d_counter = nullptr;         // 64 B: the upload's class-scan flags
     void* scratch = nullptr;           // image / point staging, grown on demand
     size_t scratch_bytes = 0;
     void* scratch2 = nullptr;          // single fast-render descriptors
     size_t scratch2_bytes = 0;
     void* pinned = nullptr;            // pinned host staging for birp_render's image copy
     size_t pinned_bytes = 0;
+    void* rbuf = nullptr;              // birp_render: [u64 sample counter | pad to 256 B | image]
+    size_t rbuf_bytes = 0;
+    unsigned long long count_base = 0; // the counter's value at its last read (it only grows)
+    bool count_known = false;          // false: clear the counter before its next use
     std::atomic<int> inject_failures{0};
 };
 
