@@ -74,7 +74,11 @@ extern "C" int birp_autotune_load(const char* path) {
                    "{\"dtype\": %d, \"size_class\": %d, \"samples_class\": %d, \"tile_w\": %d, \"tile_h\": %d, "
                    "\"lanes_per_ray\": %d, \"unroll\": %d, \"ms\": %lf}",
                    &dt, &sc, &nc, &t.tile_w, &t.tile_h, &t.lanes_per_ray, &t.unroll, &ms) == 8) {
-            if (t.tile_w * t.tile_h * t.lanes_per_ray != 64) continue;
+            const bool ok = (t.lanes_per_ray == 1 || t.lanes_per_ray == 2 || t.lanes_per_ray == 4) &&
+                            (t.unroll == 1 || t.unroll == 2 || t.unroll == 4) && t.tile_w > 0 && t.tile_h > 0 &&
+                            (t.tile_w & (t.tile_w - 1)) == 0 && t.tile_w * t.tile_h * t.lanes_per_ray == 64 &&
+                            (dt == BIRP_DTYPE_F32 || dt == BIRP_DTYPE_BF16 || dt == BIRP_DTYPE_F16);
+            if (!ok) continue;  // the same rule as birp_batch_set_tuning
             std::lock_guard<std::mutex> lk(g_mu);
             g_table[{dt, sc, nc}] = {t, ms};
             loaded++;

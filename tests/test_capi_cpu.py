@@ -179,3 +179,23 @@ def test_ctypes_structs_match_header_layout(tmp_path):
         assert got[(py, "size")] == capi.C.sizeof(cls), py
         for name, _ in cls._fields_:
             assert got[(py, name)] == getattr(cls, name).offset, (py, name)
+
+
+def test_tuning_table_load_validates_entries(tmp_path):
+    """birp_autotune_load keeps only entries that birp_batch_set_tuning would accept (tile_w a
+    power of two, tile_w * tile_h * lanes_per_ray = 64, lanes and unroll in {1, 2, 4}, a known
+    dtype), and fails when none is left. (Host-only: no GPU needed.)"""
+    row = ('{{"dtype": {}, "size_class": 2, "samples_class": 2, "tile_w": {}, "tile_h": {}, "lanes_per_ray": {}, '
+           '"unroll": {}, "ms": 1.0}}')
+    bad = [row.format(1, 8, 1, 8, 4), row.format(1, 6, 4, 1, 4), row.format(1, 8, 8, 1, 3), row.format(7, 8, 8, 1, 4),
+           row.format(1, 16, 8, 1, 4), "not json"]
+    p = tmp_path / "bad.jsonl"
+    p.write_text("\n".join(bad) + "\n")
+    assert capi.lib().birp_autotune_load(str(p).encode()) == capi.ERR_NOT_FOUND
+    # the committed table's own entry for BF16 / 576x336 / 1600 samples, so later tests see no change
+    good = tmp_path / "good.jsonl"
+    good.write_text("\n".join(bad + [row.format(1, 8, 8, 1, 4)]) + "\n")
+    assert capi.lib().birp_autotune_load(str(good).encode()) == capi.OK
+    t = capi.Tuning()
+    assert capi.lib().birp_autotune_lookup(1, 576, 336, 1600, capi.C.byref(t)) == capi.OK
+    assert (t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll) == (8, 8, 1, 4)
