@@ -173,16 +173,17 @@ bool take_injected_failure(birp_device* dev) {
     return false;
 }
 
-// Tuning knobs of the exact path, read once: BIRP_EXACT_LANES (lanes per ray: 1, 2, 4),
-// BIRP_EXACT_PIPE (samples in flight per lane in the positive-cube loop: 1..4) and
-// BIRP_EXACT_LAYOUT ("linear" reads the [z][y][x] copy instead of the quad bricks).
-int default_exact_lanes() {
+// Tuning knobs of the exact path, read once: BIRP_EXACT_LANES (lanes per ray: 1, 2, 4; by
+// default 2 for cubes and 4 for CMEM, the measured best of each), BIRP_EXACT_PIPE (samples in
+// flight per lane in the positive-cube loop: 1..4) and BIRP_EXACT_LAYOUT ("linear" reads the
+// [z][y][x] copy instead of the quad bricks).
+int default_exact_lanes(bool cmem) {
     static int lanes = [] {
         const char* e = getenv("BIRP_EXACT_LANES");
-        int v = e ? atoi(e) : 2;
-        return (v == 1 || v == 2 || v == 4) ? v : 2;
+        int v = e ? atoi(e) : 0;
+        return (v == 1 || v == 2 || v == 4) ? v : 0;
     }();
-    return lanes;
+    return lanes ? lanes : (cmem ? 4 : 2);
 }
 int default_exact_pipe() {
     static int pipe = [] {
@@ -608,7 +609,7 @@ static int render_exact_into(birp_volume* vol, const birp_view* view, float* dim
     if (take_injected_failure(dev)) return fail(BIRP_ERR_INJECTED, "injected failure");
     DevVolume dv = vol->dv;
     if (exact_force_linear() && dv.vclass != VC_NONFINITE && dv.kind == SK_CUBE) dv.vclass = VC_NONFINITE;
-    rc = launch_exact(dv, ev, default_exact_lanes(), default_exact_pipe(), dev->stream);
+    rc = launch_exact(dv, ev, default_exact_lanes(dv.kind == SK_CMEM), default_exact_pipe(), dev->stream);
     return rc ? fail(rc, "exact render launch failed") : BIRP_OK;
 }
 
