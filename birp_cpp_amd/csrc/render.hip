@@ -829,6 +829,27 @@ __device__ __forceinline__ float fast_sample(const DevVolume& v, __amdgpu_buffer
     }
 }
 
+// Keeps the FP32 march x(j) = RN(j d + i0), j = 0 .. last, inside [0, hi]. x(j) is one rounding of
+// a linear function of j, so it is monotone in j and stays inside when x(0) and x(last) do. When
+// either is outside (a ray end within FP32 rounding of a face), the start is clamped and the step
+// recomputed, then shortened an ulp at a time until x(last) fits; the last resort is a zero step.
+__device__ __forceinline__ void fit_march(float& i0, float& d, float hi, int last) {
+    const float fl = (float)last;
+    const float b = __builtin_fmaf(fl, d, i0);
+    if (i0 >= 0.f && i0 <= hi && b >= 0.f && b <= hi) return;
+    const float a = fminf(fmaxf(i0, 0.f), hi);
+    const float bc = fminf(fmaxf(b, 0.f), hi);
+    i0 = a;
+    d = last > 0 ? (bc - a) / fl : 0.f;
+    for (int it = 0; it < 8 && d != 0.f; ++it) {
+        const float e = __builtin_fmaf(fl, d, i0);
+        if (e >= 0.f && e <= hi) return;
+        d = __uint_as_float(__float_as_uint(d) - 1u);  // one ulp toward zero
+    }
+    const float e = __builtin_fmaf(fl, d, i0);
+    if (!(e >= 0.f && e <= hi)) d = 0.f;
+}
+
 // One sample folded into a Kahan sum (sum, nc), nc holding the negated compensation -c:
 // y = x - c; t = sum + y; -c' = y - (t - sum); sum = t. An inactive lane adds x = 0.
 template <int DT>
@@ -935,22 +956,26 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
         const int cnt = (kl >= kf) ? (kl - kf + 1) : 0;
         const int my = (cnt > sub) ? (cnt - sub + S - 1) / S : 0;  // samples of this sub-lane
         const int trips = wave_max_i(my);
-        // the largest floats below n - 1: a sample rounded onto the far face keeps weight ~1
-        // on the last grid point instead of falling into the cell before it
-        const float hx = __uint_as_float(__float_as_uint(V.fnx - 1.0f) - 1u);
-        const float hy = __uint_as_float(__float_as_uint(V.fny - 1.0f) - 1u);
-        const float hz = __uint_as_float(__float_as_uint(V.fnz - 1.0f) - 1u);
+        if (cnt > 0) {
+            // every sample inside [0, nextbelow(n - 1)] per axis, so that the cell index is the
+            // truncation (0 .. n-2) with no clamp per sample; a sample rounded onto the far face
+            // keeps weight ~1 on the last grid point
+            fit_march(ix0, dix, __uint_as_float(__float_as_uint(V.fnx - 1.0f) - 1u), cnt - 1);
+            fit_march(iy0, diy, __uint_as_float(__float_as_uint(V.fny - 1.0f) - 1u), cnt - 1);
+            fit_march(iz0, diz, __uint_as_float(__float_as_uint(V.fnz - 1.0f) - 1u), cnt - 1);
+        }
         float sum = 0.f, nc = 0.f;
         for (int q = 0; q < trips; q += U) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int js = (q + u) * S + sub;  // sample offset from kf
                 const bool act = js < cnt;
-                const float fj = (float)js;
-                // clamp into the cube so that every lane's gather stays in bounds
-                float xd = __builtin_amdgcn_fmed3f(__builtin_fmaf(fj, dix, ix0), 0.f, hx);
-                float yd = __builtin_amdgcn_fmed3f(__builtin_fmaf(fj, diy, iy0), 0.f, hy);
-                float zd = __builtin_amdgcn_fmed3f(__builtin_fmaf(fj, diz, iz0), 0.f, hz);
+                // a lane past its ray's last sample re-reads that sample (its result is masked); an
+                // empty ray has i0 = d = 0 and samples the grid origin
+                const float fj = (float)min(js, cnt - 1);
+                const float xd = __builtin_fmaf(fj, dix, ix0);
+                const float yd = __builtin_fmaf(fj, diy, iy0);
+                const float zd = __builtin_fmaf(fj, diz, iz0);
                 fast_accumulate<DT>(V, rs, tab, xd, yd, zd, act, sum, nc);
             }
         }
