@@ -358,7 +358,11 @@ __device__ void exact_clip(const DevVolume& v, const ExactView& ev, const float*
     int n = ev.n;
     if (!(tin <= tout)) { kf = n; kl = -1; return; }
     double sc = (double)n / (double)ev.L;
-    double klo = ceil(tin * sc - 0.5) - 2.0, khi = floor(tout * sc - 0.5) + 2.0;
+    // The margin (2^-16 of the scene's scale) exceeds every FP32 rounding of a sample's index,
+    // so a sample outside the widened box is outside the FP32 test too: the widened range starts
+    // at or before the first accepted sample and ends at or after the last (FP64 rounding of the
+    // two expressions can only move them outward), and the walks below only move inward.
+    double klo = ceil(tin * sc - 0.5), khi = floor(tout * sc - 0.5);
     klo = fmin(fmax(klo, 0.0), (double)n);
     khi = fmin(fmax(khi, -1.0), (double)(n - 1));
     int a = (int)klo, b = (int)khi;
