@@ -73,6 +73,38 @@ def test_ragged_cubes_fast_paths(device, shape, org, spc, pos):
         v16.free()
 
 
+def test_fast_march_on_faces(device):
+    """Cameras on a face, an edge and a corner of the cube, and rays along a face plane: the fast
+    path's per-ray march fit keeps every sample inside, and the images meet the FP32-storage
+    tolerance (1e-5) and the F16 one (1e-4) against the FP64 render."""
+    n = 64
+    vals = positive_cube((n, n, n), seed=77)
+    org, spc = (-10.0, -15.0, -15.0), (30.0 / (n - 1),) * 3
+    hi = tuple(o + (n - 1) * s for o, s in zip(org, spc))
+    v32 = capi.Volume.upload(device, vals, org, spc)
+    v16 = capi.Volume.upload(device, vals, org, spc, dtype=capi.DTYPE_F16)
+    # (a camera on the x axis looking along it faces the reference's Sun direction, a degenerate
+    # basis that the API refuses; these sit off the axis)
+    poses = [((org[0], 2.0, 1.0), (hi[0], 0.0, 0.0)),            # on the x = min face, looking along +x
+             ((hi[0], 1.0, -2.0), (org[0], 0.0, 0.0)),           # on the far face, looking back
+             ((org[0], org[1], 0.0), (hi[0], 0.0, 0.0)),         # on an edge
+             ((org[0], org[1], org[2]), (hi[0], 0.0, 0.0)),      # on a corner
+             ((0.0, org[1], 0.0), (12.0, 0.0, 0.0))]             # on the y = min face
+    try:
+        for pos, aim in poses:
+            for L, n_s in ((50.0, 500), (80.0, 1600)):
+                v = capi.make_view(64, 48, 40.0, 30.0, pos, aim, L, n_s, capi.SEM_EXTENDED, capi.MATH_FAST)
+                ref = v32.render_f64(v)
+                f32, _ = v32.render(v)
+                f16, _ = v16.render(v)
+                assert np.isfinite(f32).all() and np.isfinite(f16).all()
+                assert rel_l2(f32, ref) <= 1e-5, (pos, L)
+                assert rel_l2(f16, ref) <= 1e-4, (pos, L)
+    finally:
+        v32.free()
+        v16.free()
+
+
 def test_longest_exact_ray_and_beyond(device, oracle):
     """n = 8192 is the exact path's largest ray (its t table lives in LDS): bitwise there, and a
     clear BIRP_ERR_UNSUPPORTED one sample beyond; the fast path takes longer rays."""
