@@ -94,6 +94,21 @@ def test_view_basis_rejects_nonfinite_pose():
     assert st == capi.ERR_INVALID_ARGUMENT
 
 
+def test_view_basis_on_the_sun_earth_line(oracle):
+    """On the x axis the basis is finite looking at the Sun and NaN looking away from it (the Sun
+    projects onto the view axis, Camera.cpp:440-444), in the library as in the oracle."""
+    for pos, aim, finite in (((-10.0, 0, 0), (20.0, 0, 0), True), ((20.0, 0, 0), (-10.0, 0, 0), False)):
+        v = capi.make_view(16, 16, 26.5, 15.5, pos, aim, 80, 160)
+        st, a, rot, rinv = capi.view_basis(v)
+        want = O.orient(pos, aim)
+        assert st == 0
+        assert np.isfinite(a) == np.isfinite(want) == finite
+        if finite:
+            assert np.float32(a) == np.float32(want) == 270.0
+            rc, orot, orinv = O.view_basis(O.make_view(16, 16, 26.5, 15.5, list(pos), list(aim), 80, 160))
+            assert rot.tobytes() == orot.tobytes() and rinv.tobytes() == orinv.tobytes()
+
+
 def test_shell_generator_deterministic_and_shaped():
     a, org, spc, r0 = capi.synth_shell(48, (8.0, 11.0), 0.05, seed=1)
     b, *_ = capi.synth_shell(48, (8.0, 11.0), 0.05, seed=1)

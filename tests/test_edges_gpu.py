@@ -83,9 +83,8 @@ def test_fast_march_on_faces(device):
     hi = tuple(o + (n - 1) * s for o, s in zip(org, spc))
     v32 = capi.Volume.upload(device, vals, org, spc)
     v16 = capi.Volume.upload(device, vals, org, spc, dtype=capi.DTYPE_F16)
-    # (a camera on the x axis looking along it faces the reference's Sun direction, a degenerate
-    # basis that the API refuses; these sit off the axis)
-    poses = [((org[0], 2.0, 1.0), (hi[0], 0.0, 0.0)),            # on the x = min face, looking along +x
+    poses = [((org[0], 0.0, 0.0), (hi[0], 0.0, 0.0)),            # on the x axis, looking at the Sun
+             ((org[0], 2.0, 1.0), (hi[0], 0.0, 0.0)),            # on the x = min face, looking along +x
              ((hi[0], 1.0, -2.0), (org[0], 0.0, 0.0)),           # on the far face, looking back
              ((org[0], org[1], 0.0), (hi[0], 0.0, 0.0)),         # on an edge
              ((org[0], org[1], org[2]), (hi[0], 0.0, 0.0)),      # on a corner
@@ -103,6 +102,19 @@ def test_fast_march_on_faces(device):
     finally:
         v32.free()
         v16.free()
+
+
+def test_sunward_axis_pose_refused_on_both_paths(cfg_small, oracle):
+    """A camera on the Sun-Earth line looking away from the Sun: the Sun projects onto the view
+    axis, the reference's PointToPlane divides 0 by 0 (Camera.cpp:440-444) and its angle and basis are
+    NaN. Exact and fast renders refuse the pose alike, with BIRP_ERR_INVALID_ARGUMENT."""
+    vol, _ = cfg_small
+    assert np.isnan(oracle.orient((20.0, 0.0, 0.0), (-10.0, 0.0, 0.0)))
+    buf = np.zeros(16 * 16, np.float32)
+    for math in (capi.MATH_EXACT, capi.MATH_FAST):
+        v = capi.make_view(16, 16, 26.5, 15.5, (20.0, 0.0, 0.0), (-10.0, 0.0, 0.0), 44.0, 200, capi.SEM_EXTENDED, math)
+        assert capi.lib().birp_render(vol.h, capi.C.byref(v), capi.fptr(buf), None) == capi.ERR_INVALID_ARGUMENT
+    assert not buf.any()
 
 
 def test_longest_exact_ray_and_beyond(device, oracle):
