@@ -318,11 +318,11 @@ def bench_serving(capi, W, dev, num=2000, rate=200.0):
         vals, org, spc, _ = capi.synth_shell(n, (7.0, 12.0), 0.1, seed=500 + k)
         cubes.append(capi.Volume.upload(dev, vals, org, spc, dtype=dt))
         del vals
-    budget = sum(c.nbytes() for c in cubes) + (256 << 20)
     t_arr, which, views = W.serving_requests(num, rate)
     # six launches in flight; each slot reserves a full batch of the largest image at creation,
-    # so no allocation happens while serving
+    # so no allocation happens while serving. The budget holds the cubes and those six buffers.
     max_px = max(v.width * v.height for v in views)
+    budget = sum(c.nbytes() for c in cubes) + 6 * 64 * max_px * 4
     srv = capi.Server(dev, memory_budget=budget, max_batch=64, max_inflight=6, batch_window_ms=0.2, reserve_pixels=64 * max_px)
     idx = [srv.add_volume(c) for c in cubes]
     bufs = _result_buffers(views)

@@ -1,10 +1,12 @@
 // server.cpp -- continuous-batching render server (include/birp.h "serving").
 //
 // One dispatcher thread owns `max_inflight` launch slots, each with its own HIP stream (default
-// priority), descriptor buffers and pinned result staging. Requests queue on submission; whenever
-// a slot is free the dispatcher admits queued requests in arrival order while their in-flight
-// footprint (device image + pinned staging) fits the memory budget left after the resident
-// volumes, and launches them as one persistent-kernel batch (one launch per storage dtype). A
+// priority), descriptor buffers, a device image buffer and pinned result staging of the same size.
+// The memory budget bounds device memory: the resident volumes plus the image buffers every slot
+// holds (they are kept between batches). Requests queue on submission; whenever a slot is free the
+// dispatcher admits queued requests in arrival order while the slot's image buffer can hold them,
+// or can grow to hold them within the budget (releasing idle slots' buffers when that makes the
+// room), and launches them as one persistent-kernel batch (one launch per storage dtype). A
 // request that arrives while other batches are on the GPU is launched in the next free slot, so
 // its work overlaps theirs on the device. When a slot's device work is done, a completion thread
 // copies its images from pinned staging into the callers' buffers (split over a pool of helper
@@ -44,7 +46,6 @@ struct Request {
     birp_volume* vol = nullptr;  // captured at submission
     float* result = nullptr;
     size_t pixels = 0;
-    size_t bytes = 0;  // in-flight footprint
     double submit_s = 0, start_s = 0, done_s = 0;
     int batch_size = 0;
     int status = BIRP_OK;
@@ -117,10 +118,10 @@ struct Slot {
     unsigned* d_work = nullptr;   // one counter per dtype group
     float* d_images = nullptr;
     float* h_images = nullptr;    // pinned
-    size_t image_cap = 0;
+    size_t image_cap = 0;         // pixels allocated (dispatcher thread)
+    size_t held = 0;              // device bytes charged to the budget for the images (under mu)
     std::vector<Request*> reqs;
     std::vector<size_t> offsets;
-    size_t bytes = 0;
     std::atomic<bool> busy{false};     // launched and not yet released
     std::atomic<bool> copying{false};  // device work done; handed to the completion thread
 };
@@ -133,8 +134,8 @@ struct birp_server {
     std::vector<birp_volume*> vols;
     DevVolume* d_vols = nullptr;
     int vol_cap = 64;
-    size_t resident = 0;
-    size_t inflight_bytes = 0;
+    size_t resident = 0;  // device bytes of the registered volumes (under mu)
+    size_t held = 0;      // device bytes of the slots' image buffers, allocated or promised (under mu)
     birp_tuning tuning{8, 8, 1, 4};
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -154,24 +155,30 @@ struct birp_server {
 
 namespace {
 
-// Grows slot s to hold `pixels` image pixels; on demand with headroom (hipFree synchronises the
-// device, so regrowth should be rare), or exactly when reserving at creation.
-int slot_reserve_images(Slot& s, size_t pixels, bool exact = false) {
-    if (s.image_cap >= pixels) return BIRP_OK;
+void slot_release_images(Slot& s) {
     if (s.d_images) (void)hipFree(s.d_images);
     if (s.h_images) (void)hipHostFree(s.h_images);
     s.d_images = nullptr;
     s.h_images = nullptr;
     s.image_cap = 0;
-    size_t cap = exact ? pixels : std::max(pixels + pixels / 2, (size_t)1 << 22);
-    hipError_t e = hipMalloc(&s.d_images, cap * 4);
+}
+
+// Reallocates slot s's image buffers to exactly `pixels` when they hold fewer (hipFree
+// synchronises the device, so growth is made rare by the headroom chosen at admission).
+int slot_alloc_images(Slot& s, size_t pixels) {
+    if (s.image_cap >= pixels) return BIRP_OK;
+    slot_release_images(s);
+    hipError_t e = hipMalloc(&s.d_images, pixels * 4);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc(server images)");
-    e = hipHostMalloc(&s.h_images, cap * 4, hipHostMallocDefault);
-    if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(server images)");
+    e = hipHostMalloc(&s.h_images, pixels * 4, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        slot_release_images(s);
+        return hip_fail(e, "hipHostMalloc(server images)");
+    }
     // touch the staging now: the first host access of each page of a fresh pinned allocation
     // faults it into the process, which would otherwise land on the first batches' copy-out
-    memset(s.h_images, 0, cap * 4);
-    s.image_cap = cap;
+    memset(s.h_images, 0, pixels * 4);
+    s.image_cap = pixels;
     return BIRP_OK;
 }
 
@@ -184,7 +191,7 @@ int launch_slot(birp_server* S, Slot& s) {
         s.offsets.push_back(total);
         total += r->pixels;
     }
-    int rc = slot_reserve_images(s, total);
+    int rc = slot_alloc_images(s, s.held / 4);  // admission charged held >= 4 * total
     if (rc) return rc;
     const birp_tuning& t = S->tuning;
     // group by kernel variant: poses of group g occupy [g*nb, g*nb + count_g)
@@ -265,8 +272,6 @@ void finish_slot(birp_server* S, Slot& s, int status) {
         r->done_s = now;
         r->finished = true;
     }
-    S->inflight_bytes -= s.bytes;
-    s.bytes = 0;
     s.reqs.clear();
     s.copying = false;
     s.busy = false;
@@ -288,15 +293,25 @@ void completer(birp_server* S) {
     }
 }
 
+// Device bytes to charge slot s for a batch of `pixels` image pixels: its current charge when that
+// holds them, else the batch plus headroom (half again, at least 4 Mi pixels) as far as `room`
+// (the budget left for slot s) allows, or 0 when even the batch does not fit.
+size_t slot_charge_for(const Slot& s, size_t pixels, size_t room) {
+    const size_t need = pixels * 4;
+    if (need <= s.held) return s.held;
+    if (need > room) return 0;
+    return std::min(std::max({need + need / 2, (size_t)16 << 20}), room);
+}
+
 void dispatcher(birp_server* S) {
     (void)hipSetDevice(S->dev->ordinal);
     const size_t budget = S->cfg.memory_budget;
     while (true) {
         // 1. hand finished launches to the completion thread
-        bool any_busy = false, any_held = false;
+        bool any_busy = false, any_launched = false;
         for (Slot& s : S->slots) {
             if (!s.busy) continue;
-            any_held = true;
+            any_launched = true;
             if (s.copying) continue;
             hipError_t q = hipEventQuery(s.done);
             if (q == hipSuccess) {
@@ -312,14 +327,15 @@ void dispatcher(birp_server* S) {
                 any_busy = true;
             }
         }
-        // 2. admit into a free slot
+        // 2. admit into the free slot with the largest image buffer
         Slot* free_slot = nullptr;
         for (Slot& s : S->slots)
-            if (!s.busy) { free_slot = &s; break; }
+            if (!s.busy && (!free_slot || s.held > free_slot->held)) free_slot = &s;
         std::vector<Request*> failed;
+        bool release_idle = false;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            if (S->stop && S->pending.empty() && !any_held) break;
+            if (S->stop && S->pending.empty() && !any_launched) break;
             if (free_slot && !S->pending.empty()) {
                 // a short window lets a trickle of arrivals share a launch while the GPU is busy
                 if (any_busy && (int)S->pending.size() < S->cfg.max_batch && S->cfg.batch_window_ms > 0) {
@@ -331,16 +347,25 @@ void dispatcher(birp_server* S) {
                     }
                 }
                 double now = birp_clock_s();
+                // device bytes for image buffers: all of it for this slot once the others are released
+                const size_t images = budget > S->resident ? budget - S->resident : 0;
+                const size_t room = images - std::min(images, S->held - free_slot->held);
+                size_t pixels = 0, charge = free_slot->held;
                 while (!S->pending.empty() && (int)free_slot->reqs.size() < S->cfg.max_batch) {
                     Request* r = S->pending.front();
-                    size_t avail = budget > S->resident ? budget - S->resident : 0;
-                    if (r->bytes > avail) {  // can never fit
+                    if (r->pixels * 4 > images) {  // can never fit
                         S->pending.pop_front();
                         r->status = BIRP_ERR_ADMISSION;
                         failed.push_back(r);
                         continue;
                     }
-                    if (S->inflight_bytes + free_slot->bytes + r->bytes > avail) break;  // wait for memory
+                    const size_t c = slot_charge_for(*free_slot, pixels + r->pixels, room);
+                    if (!c) {
+                        // an empty batch waits for the other slots' buffers: released below when
+                        // idle, else when their launches finish
+                        release_idle = free_slot->reqs.empty();
+                        break;
+                    }
                     S->pending.pop_front();
                     if (take_injected_failure(S->dev)) {
                         r->status = BIRP_ERR_INJECTED;
@@ -349,10 +374,12 @@ void dispatcher(birp_server* S) {
                     }
                     r->start_s = now;
                     free_slot->reqs.push_back(r);
-                    free_slot->bytes += r->bytes;
+                    pixels += r->pixels;
+                    charge = c;
                 }
                 for (Request* r : free_slot->reqs) r->batch_size = (int)free_slot->reqs.size();
-                S->inflight_bytes += free_slot->bytes;
+                S->held += charge - free_slot->held;
+                free_slot->held = charge;
                 if (!failed.empty()) {
                     for (Request* r : failed) {
                         r->done_s = now;
@@ -360,7 +387,7 @@ void dispatcher(birp_server* S) {
                     }
                     S->cv_done.notify_all();
                 }
-            } else if (!any_held && S->pending.empty()) {
+            } else if (!any_launched && S->pending.empty()) {
                 S->cv_work.wait(lk, [&] { return S->stop || !S->pending.empty(); });
                 continue;
             }
@@ -371,7 +398,20 @@ void dispatcher(birp_server* S) {
             if (rc) finish_slot(S, *free_slot, rc);
             continue;
         }
-        if (any_held) std::this_thread::sleep_for(std::chrono::microseconds(20));
+        if (release_idle) {
+            // give the budget of the idle slots' buffers back (no work of theirs is in flight)
+            bool released = false;
+            for (Slot& s : S->slots) {
+                if (&s == free_slot || s.busy || !s.held) continue;
+                slot_release_images(s);
+                std::lock_guard<std::mutex> lk(S->mu);
+                S->held -= s.held;
+                s.held = 0;
+                released = true;
+            }
+            if (released) continue;
+        }
+        if (any_launched) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
 }
 
@@ -382,6 +422,8 @@ extern "C" int birp_server_create(birp_device* dev, const birp_server_config* cf
     if (cfg->max_batch < 1 || cfg->max_batch > 4096 || cfg->max_inflight < 1 || cfg->max_inflight > 8)
         return fail(BIRP_ERR_INVALID_ARGUMENT, "max_batch in [1,4096], max_inflight in [1,8]");
     if (cfg->reserve_pixels > ((size_t)1 << 32)) return fail(BIRP_ERR_INVALID_ARGUMENT, "reserve_pixels above 2^32");
+    if (cfg->reserve_pixels * 4 * cfg->max_inflight > cfg->memory_budget)
+        return fail(BIRP_ERR_ADMISSION, "the slots' reserved images exceed the memory budget");
     (void)hipSetDevice(dev->ordinal);
     birp_server* S = new birp_server();
     S->dev = dev;
@@ -402,7 +444,11 @@ extern "C" int birp_server_create(birp_device* dev, const birp_server_config* cf
         if ((e = hipHostMalloc(&s.h_prefix, sizeof(int) * BIRP_KDT_COUNT * (nb + 1), hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
         if ((e = hipMalloc(&s.d_prefix, sizeof(int) * BIRP_KDT_COUNT * (nb + 1))) != hipSuccess) { rc = hip_fail(e, "prefix"); break; }
         if ((e = hipMalloc(&s.d_work, sizeof(unsigned) * BIRP_KDT_COUNT * 256)) != hipSuccess) { rc = hip_fail(e, "counter"); break; }
-        if (cfg->reserve_pixels && (rc = slot_reserve_images(s, cfg->reserve_pixels, true))) break;
+        if (cfg->reserve_pixels) {
+            if ((rc = slot_alloc_images(s, cfg->reserve_pixels))) break;
+            s.held = cfg->reserve_pixels * 4;
+            S->held += s.held;
+        }
     }
     if (rc) {
         birp_server_destroy(S);
@@ -458,7 +504,8 @@ extern "C" int birp_server_add_volume(birp_server* S, birp_volume* vol, int* ind
     if (vol->dv.kind != SK_CUBE) return fail(BIRP_ERR_UNSUPPORTED, "the server renders cube volumes");
     std::lock_guard<std::mutex> lk(S->mu);
     if ((int)S->vols.size() >= S->vol_cap) return fail(BIRP_ERR_OUT_OF_MEMORY, "volume table full");
-    if (S->resident + vol->bytes > S->cfg.memory_budget) return fail(BIRP_ERR_ADMISSION, "volume does not fit the memory budget");
+    if (S->resident + S->held + vol->bytes > S->cfg.memory_budget)
+        return fail(BIRP_ERR_ADMISSION, "volume does not fit the memory budget");
     hipError_t e = hipMemcpy(S->d_vols + S->vols.size(), &vol->dv, sizeof(DevVolume), hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(e, "volume table upload");
     S->resident += vol->bytes;
@@ -479,7 +526,6 @@ extern "C" int birp_server_submit(birp_server* S, const birp_request* req, float
     r->req.view = v;
     r->result = result;
     r->pixels = (size_t)v.width * v.height;
-    r->bytes = r->pixels * 8;  // device image + pinned staging
     r->submit_s = birp_clock_s();
     {
         std::lock_guard<std::mutex> lk(S->mu);

@@ -212,13 +212,18 @@ int birp_autotune_lookup(int dtype, int width, int height, int samples, birp_tun
 
 /* ------------------------------------------------------------------ serving */
 typedef struct birp_server_config {
-    size_t memory_budget;  /* device bytes for resident volumes + in-flight images */
+    size_t memory_budget;  /* device bytes for the resident volumes plus the image buffers the
+                              slots hold (4 bytes per pixel, kept between batches; a slot grows
+                              its buffer within the budget, releasing idle slots' buffers when a
+                              request needs the room). Pinned host staging of the same size per
+                              slot is host memory, outside the budget. */
     int max_batch;         /* most requests one launch may carry */
     int max_inflight;      /* launches in flight at once (each on its own stream) */
     double batch_window_ms;/* wait this long for more arrivals before launching a non-full batch */
     size_t reserve_pixels; /* image pixels each slot allocates at creation (device + pinned
                               staging), so that batches up to that size allocate nothing while
-                              serving; 0: slots grow on demand */
+                              serving; max_inflight * reserve_pixels * 4 must fit the budget
+                              (else BIRP_ERR_ADMISSION); 0: slots grow on demand */
 } birp_server_config;
 
 typedef struct birp_request {
@@ -237,7 +242,9 @@ typedef struct birp_result_info {
 int birp_server_create(birp_device* dev, const birp_server_config* cfg, birp_server** out);
 int birp_server_destroy(birp_server* s);
 /* Registers a resident volume (owned by the caller; must outlive the server). Its device bytes
- * count against the budget; fails with BIRP_ERR_ADMISSION when it does not fit. */
+ * count against the budget; fails with BIRP_ERR_ADMISSION when it does not fit beside the
+ * volumes already registered and the image buffers the slots hold. A request whose image alone
+ * exceeds the budget left after the volumes completes with BIRP_ERR_ADMISSION. */
 int birp_server_add_volume(birp_server* s, birp_volume* vol, int* index);
 /* Queues a request; the result buffer (H*W floats, host) must stay valid until it completes. */
 int birp_server_submit(birp_server* s, const birp_request* req, float* result, uint64_t* id);

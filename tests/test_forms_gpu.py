@@ -264,8 +264,8 @@ def test_server_concurrent_clients(device, cube1):
 def test_server_admission_and_injected_failure(device, cube1):
     vals, org, spc, vols, oc = cube1
     vb = vols[capi.DTYPE_F16].nbytes()
-    # budget: the volume plus room for ~1.5 small images
-    srv = capi.Server(device, memory_budget=vb + int(1.5 * 144 * 84 * 8), max_batch=8, max_inflight=2)
+    # budget: the volume plus device room for ~1.5 small images
+    srv = capi.Server(device, memory_budget=vb + int(1.5 * 144 * 84 * 4), max_batch=8, max_inflight=2)
     try:
         idx = srv.add_volume(vols[capi.DTYPE_F16])
         big = W.view(576, 336, 0.5, math=capi.MATH_FAST)
@@ -289,3 +289,43 @@ def test_server_admission_and_injected_failure(device, cube1):
             tiny.destroy()
     finally:
         srv.destroy()
+
+
+def test_server_budget_counts_held_image_buffers(device, cube1):
+    """The memory budget bounds the resident volumes plus the image buffers the slots hold between
+    batches: reservations beyond it are refused, so is a volume that would exceed it, and a
+    request larger than a slot's buffer gets the room by releasing an idle slot's buffer. Results
+    equal those of a server with ample memory."""
+    vals, org, spc, vols, oc = cube1
+    vb = vols[capi.DTYPE_F16].nbytes()
+    R = 144 * 84
+    with pytest.raises(capi.BirpError) as e:
+        capi.Server(device, memory_budget=2 * R * 4 - 1, max_batch=4, max_inflight=2, reserve_pixels=R)
+    assert e.value.status == capi.ERR_ADMISSION
+    srv = capi.Server(device, memory_budget=vb + 2 * R * 4, max_batch=4, max_inflight=2, reserve_pixels=R)
+    ample = capi.Server(device, memory_budget=4 << 30, max_batch=4, max_inflight=2)
+    try:
+        idx = srv.add_volume(vols[capi.DTYPE_F16])
+        with pytest.raises(capi.BirpError) as e:
+            srv.add_volume(vols[capi.DTYPE_F32])
+        assert e.value.status == capi.ERR_ADMISSION
+        ia = ample.add_volume(vols[capi.DTYPE_F16])
+        views = [W.view(144, 84, 0.5, math=capi.MATH_FAST),           # fits a slot's reservation
+                 W.view(168, 108, 0.5, math=capi.MATH_FAST),          # 1.5 reservations: the other slot's is released
+                 W.view(144, 84, 0.25, pos=(2.0, -20.0, 10.0), math=capi.MATH_FAST)]
+        for v in views:
+            st, img, info = srv.wait(srv.submit(idx, v), 30000.0)
+            st2, want, _ = ample.wait(ample.submit(ia, v), 30000.0)
+            assert st == st2 == capi.OK and info.batch_size == 1
+            assert np.array_equal(img, want)
+        # two of the larger requests at once cannot share the budget: one launch each
+        ids = [srv.submit(idx, views[1]) for _ in range(2)]
+        for rid in ids:
+            st, img, info = srv.wait(rid, 30000.0)
+            assert st == capi.OK and info.batch_size == 1
+        with pytest.raises(capi.BirpError) as e:
+            srv.add_volume(vols[capi.DTYPE_F32])
+        assert e.value.status == capi.ERR_ADMISSION
+    finally:
+        srv.destroy()
+        ample.destroy()
