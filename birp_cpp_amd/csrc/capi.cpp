@@ -540,6 +540,7 @@ extern "C" int birp_volume_bytes(const birp_volume* v, size_t* bytes) {
 
 extern "C" int birp_volume_sample(birp_volume* v, const float* xyz, long long n, float* out) {
     if (!v || (n > 0 && (!xyz || !out)) || n < 0) return fail(BIRP_ERR_INVALID_ARGUMENT, "bad argument");
+    if (n > (1ll << 32)) return fail(BIRP_ERR_INVALID_ARGUMENT, "at most 2^32 points per call");
     if (n == 0) return BIRP_OK;
     if (v->dv.kind == SK_CUBE && !v->dv.linear) return fail(BIRP_ERR_UNSUPPORTED, "point sampling needs an FP32 volume");
     birp_device* dev = v->dev;
@@ -722,7 +723,7 @@ extern "C" int birp_render_async(birp_volume* vol, const birp_view* view, birp_i
 }
 
 extern "C" int birp_count_samples(birp_volume* vol, const birp_view* view, long long* in_cube) {
-    if (!vol || !in_cube) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
+    if (!vol || !view || !in_cube) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
     birp_view v = *view;
     v.math = BIRP_MATH_EXACT;
     v.flags = 0;

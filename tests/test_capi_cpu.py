@@ -37,6 +37,29 @@ def test_version_and_strerror():
     assert capi.lib().birp_strerror(0) == b"ok"
 
 
+def test_null_arguments_are_rejected_before_any_device_work():
+    L, C = capi.lib(), capi.C
+    blank = C.create_string_buffer(4096)  # stands for a handle; never read, the checks come first
+    h = C.cast(blank, C.c_void_p)
+    v = capi.make_view(16, 16, 26.5, 15.5, (15.0, 5.0, 8.0), (0.0, 0.0, 0.0), 44.0, 200)
+    img = np.zeros(256, np.float32)
+    n = C.c_longlong()
+    inv = capi.ERR_INVALID_ARGUMENT
+    assert L.birp_render(None, C.byref(v), capi.fptr(img), None) == inv
+    assert L.birp_render(h, C.byref(v), None, None) == inv
+    assert L.birp_count_samples(h, None, C.byref(n)) == inv
+    assert L.birp_count_samples(None, C.byref(v), C.byref(n)) == inv
+    assert L.birp_render_f64(None, C.byref(v), None) == inv
+    assert L.birp_volume_sample(None, None, 1, None) == inv
+    assert L.birp_volume_sample(h, capi.fptr(img), -1, capi.fptr(img)) == inv
+    assert L.birp_volume_sample(h, capi.fptr(img), (1 << 32) + 1, capi.fptr(img)) == inv
+    assert L.birp_render_async(None, C.byref(v), None) == inv
+    assert L.birp_batch_create(None, C.byref(v), 1, None) == inv
+    assert L.birp_server_submit(None, None, None, None) == inv
+    assert L.birp_server_wait(None, 1, 0.0, None) == inv
+    assert L.birp_device_synchronize(None) == inv
+
+
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is visible")
 def test_no_gpu_is_an_error_not_a_crash():
     h = capi.C.c_void_p()
