@@ -258,7 +258,10 @@ extern "C" int birp_device_close(birp_device* dev) {
     (void)hipSetDevice(dev->ordinal);
     if (dev->stream) (void)hipStreamSynchronize(dev->stream);
     if (dev->scratch) (void)hipFree(dev->scratch);
-    if (dev->scratch2) (void)hipFree(dev->scratch2);
+    if (dev->fast_d) (void)hipFree(dev->fast_d);
+    if (dev->fast_h) (void)hipHostFree(dev->fast_h);
+    for (hipEvent_t ev : dev->fast_ev)
+        if (ev) (void)hipEventDestroy(ev);
     if (dev->pinned) (void)hipHostFree(dev->pinned);
     if (dev->rbuf) (void)hipFree(dev->rbuf);
     if (dev->d_counter) (void)hipFree(dev->d_counter);
@@ -848,7 +851,9 @@ int batch_setup(birp_batch* b) {
     if (e == hipSuccess) e = hipMemcpyAsync(b->d_vols, &b->vol->dv, sizeof(DevVolume), hipMemcpyHostToDevice, dev->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
     if (e != hipSuccess) return hip_fail(e, "batch upload");
-    b->grid = fast_grid_size(fast_kernel_dtype(b->vol->dv), t.lanes_per_ray, t.unroll);
+    // about one tile per wave when the batch is small (every wave visits all work queues before it
+    // exits, so idle waves only add claims); the occupancy-limited persistent grid otherwise
+    b->grid = std::min(fast_grid_size(fast_kernel_dtype(b->vol->dv), t.lanes_per_ray, t.unroll), (b->total_tiles + 3) / 4 + 1);
     // capture: reset the work counter (a kernel, not a memset node), then the persistent kernel
     hipStream_t cs = b->capture_stream;
     if ((e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal)) != hipSuccess) return hip_fail(e, "begin capture");
@@ -988,42 +993,54 @@ extern "C" int birp_batch_stats(birp_batch* b, long long* nominal, double* last_
     return BIRP_OK;
 }
 
-// A single fast render: a one-pose launch of the persistent kernel (no graph).
+// A single fast render: a one-pose launch of the persistent kernel (no graph). Its descriptors go
+// through the device's ring of pinned slots, so the host waits only for the upload issued
+// kFastRing renders earlier, not for the work queued before this render.
 static int render_fast_into(birp_volume* vol, const birp_view* view, float* dimg) {
     birp_device* dev = vol->dev;
     if (vol->dv.kind != SK_CUBE) return fail(BIRP_ERR_UNSUPPORTED, "fast renders take cube volumes");
+    if (view->flags & BIRP_RENDER_ACCUMULATE) return fail(BIRP_ERR_UNSUPPORTED, "fast renders do not accumulate");
     std::vector<birp_view> vv(1, *view);
     birp_tuning t;
     choose_tuning(vol, vv, t);
     FastPose fp;
     int rc = fast_pose(vol, view, 0, dimg, t.tile_w, t.tile_h, fp);
     if (rc) return rc;
-    if (view->flags & BIRP_RENDER_ACCUMULATE) return fail(BIRP_ERR_UNSUPPORTED, "fast renders do not accumulate");
-    int prefix[2] = {0, tiles_of(*view, t.tile_w, t.tile_h)};
-    // staging in the device's second scratch buffer: [DevVolume][FastPose][prefix][counter]
-    size_t need = sizeof(DevVolume) + sizeof(FastPose) + 64 + 64;
-    if (dev->scratch2_bytes < need) {
-        if (dev->scratch2) (void)hipFree(dev->scratch2);
-        dev->scratch2 = nullptr;
-        hipError_t e = hipMalloc(&dev->scratch2, 4096);
-        if (e != hipSuccess) return hip_fail(e, "hipMalloc(scratch2)");
-        dev->scratch2_bytes = 4096;
+    const int prefix[2] = {0, tiles_of(*view, t.tile_w, t.tile_h)};
+    constexpr size_t kSlot = 4096;
+    static_assert(sizeof(DevVolume) <= 1024 && sizeof(FastPose) <= 1024, "fast render descriptor layout");
+    hipError_t e;
+    if (!dev->fast_h) {
+        if ((e = hipHostMalloc(&dev->fast_h, kSlot * birp_device::kFastRing, hipHostMallocDefault)) != hipSuccess)
+            return hip_fail(e, "hipHostMalloc(fast descriptors)");
+        if ((e = hipMalloc(&dev->fast_d, kSlot * birp_device::kFastRing)) != hipSuccess) {
+            (void)hipHostFree(dev->fast_h);
+            dev->fast_h = nullptr;
+            return hip_fail(e, "hipMalloc(fast descriptors)");
+        }
+        for (hipEvent_t& ev : dev->fast_ev)
+            if (!ev && (e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return hip_fail(e, "event");
     }
-    static_assert(sizeof(DevVolume) <= 1024 && sizeof(FastPose) <= 1024, "fast render staging layout");
-    char* base = (char*)dev->scratch2;
-    DevVolume* dv = (DevVolume*)base;
-    FastPose* dp = (FastPose*)(base + 1024);
-    int* dpre = (int*)(base + 2048);
-    unsigned* dwork = (unsigned*)(base + 3072);
-    hipError_t e = hipMemcpyAsync(dv, &vol->dv, sizeof(DevVolume), hipMemcpyHostToDevice, dev->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(dp, &fp, sizeof(FastPose), hipMemcpyHostToDevice, dev->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(dpre, prefix, sizeof(prefix), hipMemcpyHostToDevice, dev->stream);
-    // the sources are this frame's locals: wait for the copies before returning
-    if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
-    if (e != hipSuccess) return hip_fail(e, "fast render upload");
+    const int slot = dev->fast_next;
+    if (dev->fast_pending[slot]) {
+        if ((e = hipEventSynchronize(dev->fast_ev[slot])) != hipSuccess) return hip_fail(e, "fast descriptor slot");
+        dev->fast_pending[slot] = false;
+    }
+    char* hb = (char*)dev->fast_h + slot * kSlot;
+    char* db = (char*)dev->fast_d + slot * kSlot;
+    memcpy(hb, &vol->dv, sizeof(DevVolume));
+    memcpy(hb + 1024, &fp, sizeof(FastPose));
+    memcpy(hb + 2048, prefix, sizeof(prefix));
     if (take_injected_failure(dev)) return fail(BIRP_ERR_INJECTED, "injected failure");
+    if ((e = hipMemcpyAsync(db, hb, 3072, hipMemcpyHostToDevice, dev->stream)) != hipSuccess ||
+        (e = hipEventRecord(dev->fast_ev[slot], dev->stream)) != hipSuccess)
+        return hip_fail(e, "fast render upload");
+    dev->fast_pending[slot] = true;
+    dev->fast_next = (slot + 1) % birp_device::kFastRing;
+    unsigned* dwork = (unsigned*)(db + 3072);
     if ((rc = launch_reset_counter(dwork, dev->stream))) return rc;
-    int grid = std::min(fast_grid_size(fast_kernel_dtype(vol->dv), t.lanes_per_ray, t.unroll), (prefix[1] + 3) / 4 + 1);
-    return launch_fast_batch(dv, fast_kernel_dtype(vol->dv), dp, dpre, 1, prefix[1], dwork, t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll,
-                             grid, dev->stream);
+    const int g = fast_kernel_dtype(vol->dv);
+    const int grid = std::min(fast_grid_size(g, t.lanes_per_ray, t.unroll), (prefix[1] + 3) / 4 + 1);
+    return launch_fast_batch((const DevVolume*)db, g, (const FastPose*)(db + 1024), (const int*)(db + 2048), 1, prefix[1], dwork,
+                             t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll, grid, dev->stream);
 }

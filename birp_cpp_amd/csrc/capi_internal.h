@@ -19,8 +19,15 @@ struct birp_device {
     void* d_counter = nullptr;         // 64 B: the upload's class-scan flags
     void* scratch = nullptr;           // image / point staging, grown on demand
     size_t scratch_bytes = 0;
-    void* scratch2 = nullptr;          // single fast-render descriptors
-    size_t scratch2_bytes = 0;
+    // single fast renders: a ring of descriptor slots [DevVolume | FastPose | tile prefix | work
+    // counter], 1 KiB each part, in pinned host memory and on the device; slot k is reused once
+    // the event recorded after its upload has completed
+    static constexpr int kFastRing = 8;
+    void* fast_h = nullptr;
+    void* fast_d = nullptr;
+    hipEvent_t fast_ev[kFastRing] = {};
+    bool fast_pending[kFastRing] = {};
+    int fast_next = 0;
     void* pinned = nullptr;            // pinned host staging for birp_render's image copy
     size_t pinned_bytes = 0;
     void* rbuf = nullptr;              // birp_render: [u64 sample counter | pad to 256 B | image]

@@ -24,6 +24,8 @@
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
+#include <atomic>
+
 #include "../../include/birp.h"
 #include "birp_internal.h"
 #include "exact_math.h"
@@ -1145,6 +1147,13 @@ int launch_fast_batch(const DevVolume* vols_dev, int dtype, const FastPose* pose
 int fast_grid_size(int dtype, int lanes_per_ray, int unroll) {
     int dev = 0;
     (void)hipGetDevice(&dev);
+    // cached per device and kernel variant: the occupancy query costs microseconds per launch
+    static std::atomic<int> cache[16][BIRP_KDT_COUNT][5][5];
+    const bool cacheable = dev >= 0 && dev < 16 && dtype >= 0 && dtype < BIRP_KDT_COUNT && lanes_per_ray <= 4 && unroll <= 4;
+    if (cacheable) {
+        const int c = cache[dev][dtype][lanes_per_ray][unroll].load(std::memory_order_relaxed);
+        if (c > 0) return c;
+    }
     int cus = 0;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (cus <= 0) cus = 256;
@@ -1160,6 +1169,7 @@ int fast_grid_size(int dtype, int lanes_per_ray, int unroll) {
     else { BIRP_FK(BIRP_DTYPE_F32) }
 #undef BIRP_FK
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BIRP_BLOCK, 0) != hipSuccess || per_cu <= 0) per_cu = 4;
+    if (cacheable) cache[dev][dtype][lanes_per_ray][unroll].store(cus * per_cu, std::memory_order_relaxed);
     return cus * per_cu;
 }
 

@@ -122,8 +122,7 @@ struct Slot {
     size_t held = 0;              // device bytes charged to the budget for the images (under mu)
     std::vector<Request*> reqs;
     std::vector<size_t> offsets;
-    std::atomic<bool> busy{false};     // launched and not yet released
-    std::atomic<bool> copying{false};  // device work done; handed to the completion thread
+    std::atomic<bool> busy{false};     // admitted or launched, and not yet released
 };
 
 }  // namespace
@@ -146,8 +145,9 @@ struct birp_server {
     std::thread worker;
     std::thread completer;
     CopyPool copy_pool;  // helpers of the completion thread
-    std::deque<Slot*> to_copy;  // guarded by mu
-    std::condition_variable cv_copy;
+    std::deque<Slot*> launched;  // guarded by mu: launched slots not yet taken by the completion thread
+    std::condition_variable cv_launched;
+    uint64_t wake_gen = 0;  // bumped (under mu) by submissions, released slots and shutdown
     bool stop = false;
     bool stop_copy = false;
     int setup_error = BIRP_OK;
@@ -273,23 +273,45 @@ void finish_slot(birp_server* S, Slot& s, int status) {
         r->finished = true;
     }
     s.reqs.clear();
-    s.copying = false;
     s.busy = false;
     S->cv_done.notify_all();
+    S->wake_gen++;  // a slot is free
+    S->cv_work.notify_one();
 }
 
+// The completion thread polls the events of the launched slots while any is on the GPU (it
+// spins, yielding, only then), copies each finished slot out and releases it.
 void completer(birp_server* S) {
+    (void)hipSetDevice(S->dev->ordinal);
+    std::vector<Slot*> mine;
     while (true) {
-        Slot* s = nullptr;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            S->cv_copy.wait(lk, [&] { return S->stop_copy || !S->to_copy.empty(); });
-            if (S->to_copy.empty()) return;  // stop_copy with nothing left
-            s = S->to_copy.front();
-            S->to_copy.pop_front();
+            if (mine.empty()) S->cv_launched.wait(lk, [&] { return S->stop_copy || !S->launched.empty(); });
+            while (!S->launched.empty()) {
+                mine.push_back(S->launched.front());
+                S->launched.pop_front();
+            }
+            if (mine.empty()) return;  // stop_copy with nothing in flight
         }
-        copy_out(S->copy_pool, *s);
-        finish_slot(S, *s, BIRP_OK);
+        bool progressed = false;
+        for (size_t k = 0; k < mine.size();) {
+            Slot* s = mine[k];
+            const hipError_t q = hipEventQuery(s->done);
+            if (q == hipErrorNotReady) {
+                k++;
+                continue;
+            }
+            mine.erase(mine.begin() + (long)k);
+            if (q == hipSuccess) {
+                copy_out(S->copy_pool, *s);
+                finish_slot(S, *s, BIRP_OK);
+            } else {
+                finish_slot(S, *s, hip_fail(q, "server batch"));
+            }
+            progressed = true;
+        }
+        if (!progressed) std::this_thread::yield();
     }
 }
 
@@ -303,115 +325,104 @@ size_t slot_charge_for(const Slot& s, size_t pixels, size_t room) {
     return std::min(std::max({need + need / 2, (size_t)16 << 20}), room);
 }
 
+// The dispatcher sleeps on cv_work until a submission, a released slot or shutdown bumps
+// wake_gen, admits queued requests into the free slot with the largest image buffer and launches
+// them; launched slots go to the completion thread.
 void dispatcher(birp_server* S) {
     (void)hipSetDevice(S->dev->ordinal);
     const size_t budget = S->cfg.memory_budget;
+    std::unique_lock<std::mutex> lk(S->mu);
     while (true) {
-        // 1. hand finished launches to the completion thread
-        bool any_busy = false, any_launched = false;
-        for (Slot& s : S->slots) {
-            if (!s.busy) continue;
-            any_launched = true;
-            if (s.copying) continue;
-            hipError_t q = hipEventQuery(s.done);
-            if (q == hipSuccess) {
-                s.copying = true;
-                {
-                    std::lock_guard<std::mutex> lk(S->mu);
-                    S->to_copy.push_back(&s);
-                }
-                S->cv_copy.notify_one();
-            } else if (q != hipErrorNotReady) {
-                finish_slot(S, s, hip_fail(q, "server batch"));
-            } else {
-                any_busy = true;
-            }
-        }
-        // 2. admit into the free slot with the largest image buffer
         Slot* free_slot = nullptr;
-        for (Slot& s : S->slots)
-            if (!s.busy && (!free_slot || s.held > free_slot->held)) free_slot = &s;
-        std::vector<Request*> failed;
-        bool release_idle = false;
-        {
-            std::unique_lock<std::mutex> lk(S->mu);
-            if (S->stop && S->pending.empty() && !any_launched) break;
-            if (free_slot && !S->pending.empty()) {
-                // a short window lets a trickle of arrivals share a launch while the GPU is busy
-                if (any_busy && (int)S->pending.size() < S->cfg.max_batch && S->cfg.batch_window_ms > 0) {
-                    double oldest = S->pending.front()->submit_s;
-                    if (birp_clock_s() - oldest < S->cfg.batch_window_ms * 1e-3) {
-                        lk.unlock();
-                        std::this_thread::sleep_for(std::chrono::microseconds(20));
-                        continue;
-                    }
-                }
-                double now = birp_clock_s();
-                // device bytes for image buffers: all of it for this slot once the others are released
-                const size_t images = budget > S->resident ? budget - S->resident : 0;
-                const size_t room = images - std::min(images, S->held - free_slot->held);
-                size_t pixels = 0, charge = free_slot->held;
-                while (!S->pending.empty() && (int)free_slot->reqs.size() < S->cfg.max_batch) {
-                    Request* r = S->pending.front();
-                    if (r->pixels * 4 > images) {  // can never fit
-                        S->pending.pop_front();
-                        r->status = BIRP_ERR_ADMISSION;
-                        failed.push_back(r);
-                        continue;
-                    }
-                    const size_t c = slot_charge_for(*free_slot, pixels + r->pixels, room);
-                    if (!c) {
-                        // an empty batch waits for the other slots' buffers: released below when
-                        // idle, else when their launches finish
-                        release_idle = free_slot->reqs.empty();
-                        break;
-                    }
-                    S->pending.pop_front();
-                    if (take_injected_failure(S->dev)) {
-                        r->status = BIRP_ERR_INJECTED;
-                        failed.push_back(r);
-                        continue;
-                    }
-                    r->start_s = now;
-                    free_slot->reqs.push_back(r);
-                    pixels += r->pixels;
-                    charge = c;
-                }
-                for (Request* r : free_slot->reqs) r->batch_size = (int)free_slot->reqs.size();
-                S->held += charge - free_slot->held;
-                free_slot->held = charge;
-                if (!failed.empty()) {
-                    for (Request* r : failed) {
-                        r->done_s = now;
-                        r->finished = true;
-                    }
-                    S->cv_done.notify_all();
-                }
-            } else if (!any_launched && S->pending.empty()) {
-                S->cv_work.wait(lk, [&] { return S->stop || !S->pending.empty(); });
+        bool any_busy = false;
+        for (Slot& s : S->slots) {
+            if (s.busy) any_busy = true;
+            else if (!free_slot || s.held > free_slot->held) free_slot = &s;
+        }
+        if (S->stop && S->pending.empty() && !any_busy) break;
+        const uint64_t seen = S->wake_gen;
+        if (S->pending.empty() || !free_slot) {
+            S->cv_work.wait(lk, [&] { return S->wake_gen != seen; });
+            continue;
+        }
+        // a short window lets a trickle of arrivals share a launch while the GPU is busy
+        if (any_busy && (int)S->pending.size() < S->cfg.max_batch && S->cfg.batch_window_ms > 0 && !S->stop) {
+            const double left = S->pending.front()->submit_s + S->cfg.batch_window_ms * 1e-3 - birp_clock_s();
+            if (left > 0) {
+                S->cv_work.wait_for(lk, std::chrono::duration<double>(left), [&] { return S->wake_gen != seen; });
                 continue;
             }
         }
-        if (free_slot && !free_slot->reqs.empty()) {
+        const double now = birp_clock_s();
+        // device bytes for image buffers: all of it for this slot once the others are released
+        const size_t images = budget > S->resident ? budget - S->resident : 0;
+        const size_t room = images - std::min(images, S->held - free_slot->held);
+        size_t pixels = 0, charge = free_slot->held;
+        bool release_idle = false, failed = false;
+        while (!S->pending.empty() && (int)free_slot->reqs.size() < S->cfg.max_batch) {
+            Request* r = S->pending.front();
+            if (r->pixels * 4 > images) {  // can never fit
+                S->pending.pop_front();
+                r->status = BIRP_ERR_ADMISSION;
+                r->done_s = now;
+                r->finished = failed = true;
+                continue;
+            }
+            const size_t c = slot_charge_for(*free_slot, pixels + r->pixels, room);
+            if (!c) {
+                // an empty batch waits for the other slots' buffers: released below when idle,
+                // else when their launches finish
+                release_idle = free_slot->reqs.empty();
+                break;
+            }
+            S->pending.pop_front();
+            if (take_injected_failure(S->dev)) {
+                r->status = BIRP_ERR_INJECTED;
+                r->done_s = now;
+                r->finished = failed = true;
+                continue;
+            }
+            r->start_s = now;
+            free_slot->reqs.push_back(r);
+            pixels += r->pixels;
+            charge = c;
+        }
+        for (Request* r : free_slot->reqs) r->batch_size = (int)free_slot->reqs.size();
+        S->held += charge - free_slot->held;
+        free_slot->held = charge;
+        if (failed) S->cv_done.notify_all();
+        if (!free_slot->reqs.empty()) {
             free_slot->busy = true;
-            int rc = launch_slot(S, *free_slot);
-            if (rc) finish_slot(S, *free_slot, rc);
+            lk.unlock();
+            const int rc = launch_slot(S, *free_slot);
+            if (rc) {
+                finish_slot(S, *free_slot, rc);
+                lk.lock();
+            } else {
+                lk.lock();
+                S->launched.push_back(free_slot);
+                S->cv_launched.notify_one();
+            }
             continue;
         }
         if (release_idle) {
             // give the budget of the idle slots' buffers back (no work of theirs is in flight)
-            bool released = false;
-            for (Slot& s : S->slots) {
-                if (&s == free_slot || s.busy || !s.held) continue;
-                slot_release_images(s);
-                std::lock_guard<std::mutex> lk(S->mu);
-                S->held -= s.held;
-                s.held = 0;
-                released = true;
+            std::vector<Slot*> idle;
+            for (Slot& s : S->slots)
+                if (&s != free_slot && !s.busy && s.held) idle.push_back(&s);
+            if (!idle.empty()) {
+                lk.unlock();
+                for (Slot* s : idle) slot_release_images(*s);
+                lk.lock();
+                for (Slot* s : idle) {
+                    S->held -= s->held;
+                    s->held = 0;
+                }
+                continue;
             }
-            if (released) continue;
+            // the room is held by running launches: wait for one to finish
+            S->cv_work.wait(lk, [&] { return S->wake_gen != seen; });
         }
-        if (any_launched) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
 }
 
@@ -467,6 +478,7 @@ extern "C" int birp_server_destroy(birp_server* S) {
         {
             std::lock_guard<std::mutex> lk(S->mu);
             S->stop = true;
+            S->wake_gen++;
         }
         S->cv_work.notify_all();
         S->worker.join();
@@ -476,7 +488,7 @@ extern "C" int birp_server_destroy(birp_server* S) {
             std::lock_guard<std::mutex> lk(S->mu);
             S->stop_copy = true;
         }
-        S->cv_copy.notify_all();
+        S->cv_launched.notify_all();
         S->completer.join();
     }
     S->copy_pool.stop();
@@ -537,6 +549,7 @@ extern "C" int birp_server_submit(birp_server* S, const birp_request* req, float
         r->id = S->next_id++;
         S->reqs[r->id] = r;
         S->pending.push_back(r);
+        S->wake_gen++;
     }
     S->cv_work.notify_one();
     *id = r->id;

@@ -265,3 +265,23 @@ def test_render_async_into_device_image(device, shell128, oracle):
         img.free()
     ref, _ = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT)
     assert_bitwise(got, ref)
+
+
+def test_fast_async_renders_queue_without_waiting(device, shell128):
+    """Twenty fast renders queued back to back with no wait between them (more than the device's
+    ring of descriptor slots, so slots are reused while earlier renders may still be queued): each
+    image equals the same view rendered on its own."""
+    vol, _ = shell128
+    views = [W.view(144 + 16 * (k % 3), 84 + 8 * (k % 2), 0.5, pos=(0.5 * k, -20.0, 10.0), math=capi.MATH_FAST)
+             for k in range(20)]
+    imgs = [capi.Image(device, v.width, v.height) for v in views]
+    try:
+        for v, img in zip(views, imgs):
+            capi.render_async(vol, v, img)
+        got = [img.download() for img in imgs]
+    finally:
+        for img in imgs:
+            img.free()
+    for v, g in zip(views, got):
+        want, _ = vol.render(v)
+        assert_bitwise(g, want)
