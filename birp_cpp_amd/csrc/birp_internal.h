@@ -134,4 +134,7 @@ inline int fast_kernel_dtype(const DevVolume& d) {
     return (d.dtype == BIRP_DTYPE_F16 && d.f16_uniform) ? BIRP_KDT_F16_UNIFORM : d.dtype;
 }
 int fast_grid_size(int dtype, int lanes_per_ray, int unroll);
+// The grid of one persistent launch over `tiles` tiles: the occupancy-limited grid, or about one
+// wave per tile when that is smaller (idle waves cost serialized work-queue claims on exit).
+int fast_grid_for(int dtype, int lanes_per_ray, int unroll, int tiles);
 }  // namespace birp

@@ -851,9 +851,7 @@ int batch_setup(birp_batch* b) {
     if (e == hipSuccess) e = hipMemcpyAsync(b->d_vols, &b->vol->dv, sizeof(DevVolume), hipMemcpyHostToDevice, dev->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
     if (e != hipSuccess) return hip_fail(e, "batch upload");
-    // about one tile per wave when the batch is small (every wave visits all work queues before it
-    // exits, so idle waves only add claims); the occupancy-limited persistent grid otherwise
-    b->grid = std::min(fast_grid_size(fast_kernel_dtype(b->vol->dv), t.lanes_per_ray, t.unroll), (b->total_tiles + 3) / 4 + 1);
+    b->grid = fast_grid_for(fast_kernel_dtype(b->vol->dv), t.lanes_per_ray, t.unroll, b->total_tiles);
     // capture: reset the work counter (a kernel, not a memset node), then the persistent kernel
     hipStream_t cs = b->capture_stream;
     if ((e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal)) != hipSuccess) return hip_fail(e, "begin capture");
@@ -1040,7 +1038,7 @@ static int render_fast_into(birp_volume* vol, const birp_view* view, float* dimg
     unsigned* dwork = (unsigned*)(db + 3072);
     if ((rc = launch_reset_counter(dwork, dev->stream))) return rc;
     const int g = fast_kernel_dtype(vol->dv);
-    const int grid = std::min(fast_grid_size(g, t.lanes_per_ray, t.unroll), (prefix[1] + 3) / 4 + 1);
+    const int grid = fast_grid_for(g, t.lanes_per_ray, t.unroll, prefix[1]);
     return launch_fast_batch((const DevVolume*)db, g, (const FastPose*)(db + 1024), (const int*)(db + 2048), 1, prefix[1], dwork,
                              t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll, grid, dev->stream);
 }

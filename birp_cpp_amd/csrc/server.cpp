@@ -223,7 +223,7 @@ int launch_slot(birp_server* S, Slot& s) {
         if (!counts[g]) continue;
         int total_tiles = s.h_prefix[g * (nb + 1) + counts[g]];
         if ((rc = launch_reset_counter(s.d_work + g * 256, s.stream))) return rc;
-        int grid = std::min(fast_grid_size(g, t.lanes_per_ray, t.unroll), (total_tiles + 3) / 4 + 1);
+        int grid = fast_grid_for(g, t.lanes_per_ray, t.unroll, total_tiles);
         rc = launch_fast_batch(S->d_vols, g, s.d_poses + g * nb, s.d_prefix + g * (nb + 1), counts[g], total_tiles, s.d_work + g * 256,
                                t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll, grid, s.stream);
         if (rc) return rc;
