@@ -816,6 +816,25 @@ int choose_tuning(const birp_volume* vol, const std::vector<birp_view>& views, b
     return BIRP_OK;
 }
 
+// The tuning of a fast render launched alone or in a server batch, chosen for latency from the
+// view itself (so a request's result never depends on what it is batched with): an image with
+// few rays splits each ray over 2 or 4 lanes, shortening every lane's chain of dependent sample
+// gathers. Measured on single launches (256^3 cube, F32 and F16): 144x84 at n = 800 took 0.036
+// ms with 4 lanes against 0.06 ms with 1, at n = 160-320 2 lanes were best; 288x168 at n = 800
+// 0.053 ms with 2 lanes against 0.06 ms; 576x336 was fastest with 1.
+birp_tuning latency_tuning(const birp_view& v) {
+    const long long rays = (long long)v.width * v.height;
+    int lanes = 1;
+    if (rays <= 20000) lanes = v.samples >= 600 ? 4 : 2;
+    else if (rays <= 60000 && v.samples >= 600) lanes = 2;
+    birp_tuning t;
+    t.lanes_per_ray = lanes;
+    t.unroll = 4;
+    t.tile_w = 8;
+    t.tile_h = 64 / (8 * lanes);
+    return t;
+}
+
 void batch_release(birp_batch* b) {
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -998,9 +1017,7 @@ static int render_fast_into(birp_volume* vol, const birp_view* view, float* dimg
     birp_device* dev = vol->dev;
     if (vol->dv.kind != SK_CUBE) return fail(BIRP_ERR_UNSUPPORTED, "fast renders take cube volumes");
     if (view->flags & BIRP_RENDER_ACCUMULATE) return fail(BIRP_ERR_UNSUPPORTED, "fast renders do not accumulate");
-    std::vector<birp_view> vv(1, *view);
-    birp_tuning t;
-    choose_tuning(vol, vv, t);
+    const birp_tuning t = latency_tuning(*view);
     FastPose fp;
     int rc = fast_pose(vol, view, 0, dimg, t.tile_w, t.tile_h, fp);
     if (rc) return rc;

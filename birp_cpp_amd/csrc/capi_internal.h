@@ -87,4 +87,5 @@ int fast_pose(const birp_volume* vol, const birp_view* v, int vol_index, float* 
 int tiles_of(const birp_view& v, int tile_w, int tile_h);
 bool take_injected_failure(birp_device* dev);
 int choose_tuning(const birp_volume* vol, const std::vector<birp_view>& views, birp_tuning& t);
+birp_tuning latency_tuning(const birp_view& v);
 }  // namespace birp

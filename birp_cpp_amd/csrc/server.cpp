@@ -6,13 +6,14 @@
 // holds (they are kept between batches). Requests queue on submission; whenever a slot is free the
 // dispatcher admits queued requests in arrival order while the slot's image buffer can hold them,
 // or can grow to hold them within the budget (releasing idle slots' buffers when that makes the
-// room), and launches them as one persistent-kernel batch (one launch per storage dtype). A
-// request that arrives while other batches are on the GPU is launched in the next free slot, so
-// its work overlaps theirs on the device. When a slot's device work is done, a completion thread
+// room), and launches them as one persistent-kernel batch (one launch per storage type and lanes
+// per ray). A request that arrives while other batches are on the GPU is launched in the next
+// free slot, so its work overlaps theirs on the device. When a slot's device work is done, a completion thread
 // copies its images from pinned staging into the callers' buffers (split over a pool of helper
-// threads for large batches) and releases the slot, so the dispatcher keeps launching meanwhile. Every
-// request is rendered by the same kernel tuning and the same per-tile arithmetic whatever it is
-// batched with, so results do not depend on the mix.
+// threads for large batches) and releases the slot, so the dispatcher keeps launching meanwhile.
+// Every request is rendered with the tuning its own view selects (the one fast single renders
+// use) and the same per-tile arithmetic whatever it is batched with, so results do not depend on
+// the mix and equal birp_render's.
 // No reference counterpart: the reference renders one pose per process run (main.cpp:266-316).
 #include <hip/hip_runtime_api.h>
 
@@ -111,11 +112,11 @@ private:
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    FastPose* h_poses = nullptr;  // pinned
+    FastPose* h_poses = nullptr;  // pinned, max_batch poses packed by kernel group
     FastPose* d_poses = nullptr;
-    int* h_prefix = nullptr;      // pinned, 3 groups x (max_batch + 1)
+    int* h_prefix = nullptr;      // pinned, max_batch + kGroups tile prefix sums
     int* d_prefix = nullptr;
-    unsigned* d_work = nullptr;   // one counter per dtype group
+    unsigned* d_work = nullptr;   // work-queue counters, 256 per kernel group
     float* d_images = nullptr;
     float* h_images = nullptr;    // pinned
     size_t image_cap = 0;         // pixels allocated (dispatcher thread)
@@ -135,7 +136,6 @@ struct birp_server {
     int vol_cap = 64;
     size_t resident = 0;  // device bytes of the registered volumes (under mu)
     size_t held = 0;      // device bytes of the slots' image buffers, allocated or promised (under mu)
-    birp_tuning tuning{8, 8, 1, 4};
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> pending;
@@ -182,9 +182,19 @@ int slot_alloc_images(Slot& s, size_t pixels) {
     return BIRP_OK;
 }
 
-// Launch the admitted requests of slot s (dispatcher thread, lock not held).
+// Kernel groups of a batch: storage kernel x lanes per ray (1, 2, 4). Every request's tuning comes
+// from its own view (latency_tuning), so a group has one tuning and a request renders the same
+// whatever shares its launch.
+constexpr int kGroups = BIRP_KDT_COUNT * 3;
+int lane_index(int lanes) { return lanes == 4 ? 2 : (lanes == 2 ? 1 : 0); }
+int request_group(const Request* r, birp_tuning* t) {
+    *t = latency_tuning(r->req.view);
+    return fast_kernel_dtype(r->vol->dv) * 3 + lane_index(t->lanes_per_ray);
+}
+
+// Launch the admitted requests of slot s (dispatcher thread, lock not held). Group g's poses are
+// h_poses[base[g], base[g+1]) and its tile prefix sums h_prefix[base[g] + g, ... + count_g].
 int launch_slot(birp_server* S, Slot& s) {
-    const int nb = S->cfg.max_batch;
     size_t total = 0;
     s.offsets.clear();
     for (Request* r : s.reqs) {
@@ -193,38 +203,48 @@ int launch_slot(birp_server* S, Slot& s) {
     }
     int rc = slot_alloc_images(s, s.held / 4);  // admission charged held >= 4 * total
     if (rc) return rc;
-    const birp_tuning& t = S->tuning;
-    // group by kernel variant: poses of group g occupy [g*nb, g*nb + count_g)
-    int counts[BIRP_KDT_COUNT] = {0, 0, 0, 0};
+    int base[kGroups + 1] = {}, fill[kGroups];
+    birp_tuning gt[kGroups];
+    for (Request* r : s.reqs) {
+        birp_tuning t;
+        const int g = request_group(r, &t);
+        base[g + 1]++;
+        gt[g] = t;
+    }
+    for (int g = 0; g < kGroups; g++) {
+        base[g + 1] += base[g];
+        fill[g] = base[g];
+    }
     for (size_t k = 0; k < s.reqs.size(); k++) {
         Request* r = s.reqs[k];
-        birp_volume* vol = r->vol;
-        int g = fast_kernel_dtype(vol->dv);
-        FastPose& fp = s.h_poses[g * nb + counts[g]];
-        rc = fast_pose(vol, &r->req.view, r->req.volume, s.d_images + s.offsets[k], t.tile_w, t.tile_h, fp);
+        birp_tuning t;
+        const int g = request_group(r, &t);
+        rc = fast_pose(r->vol, &r->req.view, r->req.volume, s.d_images + s.offsets[k], t.tile_w, t.tile_h, s.h_poses[fill[g]++]);
         if (rc) return rc;
-        counts[g]++;
     }
-    for (int g = 0; g < BIRP_KDT_COUNT; g++) {
-        int* pre = s.h_prefix + g * (nb + 1);
+    for (int g = 0; g < kGroups; g++) {
+        int* pre = s.h_prefix + base[g] + g;
         pre[0] = 0;
-        for (int k = 0; k < counts[g]; k++) {
-            const FastPose& fp = s.h_poses[g * nb + k];
+        for (int k = base[g]; k < base[g + 1]; k++) {
             birp_view tmp{};
-            tmp.width = fp.W;
-            tmp.height = fp.H;
-            pre[k + 1] = pre[k] + tiles_of(tmp, t.tile_w, t.tile_h);
+            tmp.width = s.h_poses[k].W;
+            tmp.height = s.h_poses[k].H;
+            pre[k - base[g] + 1] = pre[k - base[g]] + tiles_of(tmp, gt[g].tile_w, gt[g].tile_h);
         }
     }
-    hipError_t e = hipMemcpyAsync(s.d_poses, s.h_poses, sizeof(FastPose) * BIRP_KDT_COUNT * nb, hipMemcpyHostToDevice, s.stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(s.d_prefix, s.h_prefix, sizeof(int) * BIRP_KDT_COUNT * (nb + 1), hipMemcpyHostToDevice, s.stream);
+    const int n = base[kGroups];
+    hipError_t e = hipMemcpyAsync(s.d_poses, s.h_poses, sizeof(FastPose) * n, hipMemcpyHostToDevice, s.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s.d_prefix, s.h_prefix, sizeof(int) * (n + kGroups), hipMemcpyHostToDevice, s.stream);
     if (e != hipSuccess) return hip_fail(e, "server descriptor upload");
-    for (int g = 0; g < BIRP_KDT_COUNT; g++) {
-        if (!counts[g]) continue;
-        int total_tiles = s.h_prefix[g * (nb + 1) + counts[g]];
+    for (int g = 0; g < kGroups; g++) {
+        const int count = base[g + 1] - base[g];
+        if (!count) continue;
+        const birp_tuning& t = gt[g];
+        const int kdt = g / 3;
+        const int total_tiles = s.h_prefix[base[g] + g + count];
         if ((rc = launch_reset_counter(s.d_work + g * 256, s.stream))) return rc;
-        int grid = fast_grid_for(g, t.lanes_per_ray, t.unroll, total_tiles);
-        rc = launch_fast_batch(S->d_vols, g, s.d_poses + g * nb, s.d_prefix + g * (nb + 1), counts[g], total_tiles, s.d_work + g * 256,
+        const int grid = fast_grid_for(kdt, t.lanes_per_ray, t.unroll, total_tiles);
+        rc = launch_fast_batch(S->d_vols, kdt, s.d_poses + base[g], s.d_prefix + base[g] + g, count, total_tiles, s.d_work + g * 256,
                                t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll, grid, s.stream);
         if (rc) return rc;
     }
@@ -439,8 +459,6 @@ extern "C" int birp_server_create(birp_device* dev, const birp_server_config* cf
     birp_server* S = new birp_server();
     S->dev = dev;
     S->cfg = *cfg;
-    birp_tuning t;
-    if (birp_autotune_lookup(BIRP_DTYPE_F16, 576, 336, 800, &t) == BIRP_OK) S->tuning = t;
     hipError_t e = hipMalloc(&S->d_vols, sizeof(DevVolume) * S->vol_cap);
     int rc = BIRP_OK;
     if (e != hipSuccess) rc = hip_fail(e, "hipMalloc(server volumes)");
@@ -450,11 +468,11 @@ extern "C" int birp_server_create(birp_device* dev, const birp_server_config* cf
         if (rc) break;
         if ((e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking)) != hipSuccess) { rc = hip_fail(e, "stream"); break; }
         if ((e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming)) != hipSuccess) { rc = hip_fail(e, "event"); break; }
-        if ((e = hipHostMalloc(&s.h_poses, sizeof(FastPose) * BIRP_KDT_COUNT * nb, hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
-        if ((e = hipMalloc(&s.d_poses, sizeof(FastPose) * BIRP_KDT_COUNT * nb)) != hipSuccess) { rc = hip_fail(e, "poses"); break; }
-        if ((e = hipHostMalloc(&s.h_prefix, sizeof(int) * BIRP_KDT_COUNT * (nb + 1), hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
-        if ((e = hipMalloc(&s.d_prefix, sizeof(int) * BIRP_KDT_COUNT * (nb + 1))) != hipSuccess) { rc = hip_fail(e, "prefix"); break; }
-        if ((e = hipMalloc(&s.d_work, sizeof(unsigned) * BIRP_KDT_COUNT * 256)) != hipSuccess) { rc = hip_fail(e, "counter"); break; }
+        if ((e = hipHostMalloc(&s.h_poses, sizeof(FastPose) * nb, hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
+        if ((e = hipMalloc(&s.d_poses, sizeof(FastPose) * nb)) != hipSuccess) { rc = hip_fail(e, "poses"); break; }
+        if ((e = hipHostMalloc(&s.h_prefix, sizeof(int) * (nb + kGroups), hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
+        if ((e = hipMalloc(&s.d_prefix, sizeof(int) * (nb + kGroups))) != hipSuccess) { rc = hip_fail(e, "prefix"); break; }
+        if ((e = hipMalloc(&s.d_work, sizeof(unsigned) * kGroups * 256)) != hipSuccess) { rc = hip_fail(e, "counter"); break; }
         if (cfg->reserve_pixels) {
             if ((rc = slot_alloc_images(s, cfg->reserve_pixels))) break;
             s.held = cfg->reserve_pixels * 4;

@@ -329,3 +329,32 @@ def test_server_budget_counts_held_image_buffers(device, cube1):
     finally:
         srv.destroy()
         ample.destroy()
+
+
+def test_served_requests_equal_single_fast_renders(device, cube1):
+    """A served request is rendered with the tuning its own view selects, the one birp_render
+    uses for a fast render: every request of a mixed burst (three image sizes, three sample
+    counts, three storage types, so every lanes-per-ray group) equals birp_render of its view,
+    bit for bit."""
+    vals, org, spc, vols, oc = cube1
+    srv = capi.Server(device, memory_budget=4 << 30, max_batch=32, max_inflight=3)
+    try:
+        dts = (capi.DTYPE_F32, capi.DTYPE_BF16, capi.DTYPE_F16)
+        idx = {dt: srv.add_volume(vols[dt]) for dt in dts}
+        pos = capi.synth_orbit(27, 21)
+        reqs = []
+        for k, p in enumerate(pos):
+            w, h = W.SIZES[k % 3]
+            step = (0.1, 0.25, 0.5)[(k // 3) % 3]
+            reqs.append((dts[(k // 9) % 3], W.view(w, h, step, pos=tuple(float(c) for c in p), math=capi.MATH_FAST)))
+        ids = [srv.submit(idx[dt], v) for dt, v in reqs]
+        sizes = []
+        for (dt, v), rid in zip(reqs, ids):
+            st, img, info = srv.wait(rid, 30000.0)
+            assert st == capi.OK
+            sizes.append(info.batch_size)
+            want, _ = vols[dt].render(v)
+            assert np.array_equal(img.view(np.uint32), want.view(np.uint32)), (dt, v.width, v.samples)
+        assert max(sizes) > 1
+    finally:
+        srv.destroy()
