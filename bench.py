@@ -310,9 +310,9 @@ def _serve(capi, srv, idx, which, views, t_arr, bufs):
 
 def bench_serving(capi, W, dev, num=2000, rate=200.0):
     """Config-5 serving: 8 resident cubes, Poisson arrivals at `rate`, then the same requests as
-    one burst (the server saturated; timed after one untimed burst), then the same requests
-    through the plain path (one birp_render and host sync per request) for the renders/s
-    comparison."""
+    one burst (the server saturated; timed after one untimed burst), both again with pinned
+    response buffers (birp_host_alloc), then the same requests through the plain path (one
+    birp_render and host sync per request) for the renders/s comparison."""
     cubes = []
     for k, (n, dt) in enumerate(W.serving_cubes()):
         vals, org, spc, _ = capi.synth_shell(n, (7.0, 12.0), 0.1, seed=500 + k)
@@ -332,7 +332,16 @@ def bench_serving(capi, W, dev, num=2000, rate=200.0):
     # first burst after the light Poisson load still runs at ramping clocks
     _serve(capi, srv, idx, which, views, None, bufs)
     out["saturated"] = _serve(capi, srv, idx, which, views, None, bufs)
+    # the same with response buffers from birp_host_alloc: each image arrives by one DMA
+    # straight into its buffer (no staging copy)
+    pinned = [capi.pinned_empty((v.height, v.width)) for v in views]
+    for b in pinned:
+        b.fill(1.0)
+    out["poisson_pinned_results"] = _serve(capi, srv, idx, which, views, t_arr, pinned)
+    _serve(capi, srv, idx, which, views, None, pinned)
+    out["saturated_pinned_results"] = _serve(capi, srv, idx, which, views, None, pinned)
     srv.destroy()
+    del pinned
     t0 = time.perf_counter()
     for k in range(num):
         cubes[which[k]].render(views[k], out=bufs[k])

@@ -126,6 +126,8 @@ SIGNATURES = [
     ("birp_server_submit", _i, [_P, C.POINTER(Request), _FP, C.POINTER(C.c_uint64)]),
     ("birp_server_wait", _i, [_P, C.c_uint64, _d, C.POINTER(ResultInfo)]),
     ("birp_clock_s", _d, []),
+    ("birp_host_alloc", _i, [C.c_size_t, _PP]),
+    ("birp_host_free", _i, [_P]),
     ("birp_synth_shell", _i, [C.POINTER(ShellParams), _FP, _FP, _FP, _FP]),
     ("birp_synth_orbit", _i, [_i, _f, _f, _f, C.c_uint64, _FP]),
     ("birp_debug_inject_failure", _i, [_P, _i]),
@@ -376,6 +378,27 @@ class Server:
         if self.h:
             lib().birp_server_destroy(self.h)
             self.h = C.c_void_p()
+
+
+def pinned_empty(shape, dtype=np.float32) -> np.ndarray:
+    """An uninitialised array in pinned host memory from birp_host_alloc (freed with the array):
+    a server request whose result is such an array gets its image by one DMA, no staging copy."""
+    n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+    p = C.c_void_p()
+    check(lib().birp_host_alloc(max(n, 1), C.byref(p)), "birp_host_alloc")
+    buf = (C.c_char * max(n, 1)).from_address(p.value)
+    buf._owner = _PinnedBlock(p.value)  # keeps the allocation alive as long as any view of it
+    return np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
+
+
+class _PinnedBlock:
+    def __init__(self, addr: int):
+        self.addr = addr
+
+    def __del__(self):
+        if _lib is not None and self.addr:
+            _lib.birp_host_free(C.c_void_p(self.addr))
+            self.addr = 0
 
 
 def clock_s() -> float:

@@ -24,6 +24,7 @@
 #include <cstring>
 #include <deque>
 #include <functional>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -40,6 +41,47 @@ extern "C" double birp_clock_s(void) {
 }
 
 namespace {
+// Pinned response buffers handed out by birp_host_alloc: base address -> size.
+std::mutex g_pinned_mu;
+std::map<uintptr_t, size_t> g_pinned;
+
+bool pinned_range(const void* p, size_t bytes) {
+    const uintptr_t a = (uintptr_t)p;
+    std::lock_guard<std::mutex> lk(g_pinned_mu);
+    auto it = g_pinned.upper_bound(a);
+    if (it == g_pinned.begin()) return false;
+    --it;
+    return a >= it->first && a - it->first <= it->second && bytes <= it->second - (a - it->first);
+}
+}  // namespace
+
+extern "C" int birp_host_alloc(size_t bytes, void** out) {
+    if (!out || bytes == 0 || bytes > ((size_t)1 << 40)) return fail(BIRP_ERR_INVALID_ARGUMENT, "bytes in [1, 2^40] and out required");
+    *out = nullptr;
+    void* p = nullptr;
+    hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(response buffer)");
+    {
+        std::lock_guard<std::mutex> lk(g_pinned_mu);
+        g_pinned[(uintptr_t)p] = bytes;
+    }
+    *out = p;
+    return BIRP_OK;
+}
+
+extern "C" int birp_host_free(void* p) {
+    if (!p) return BIRP_OK;
+    {
+        std::lock_guard<std::mutex> lk(g_pinned_mu);
+        auto it = g_pinned.find((uintptr_t)p);
+        if (it == g_pinned.end()) return fail(BIRP_ERR_NOT_FOUND, "not a birp_host_alloc pointer");
+        g_pinned.erase(it);
+    }
+    hipError_t e = hipHostFree(p);
+    return e == hipSuccess ? BIRP_OK : hip_fail(e, "hipHostFree(response buffer)");
+}
+
+namespace {
 
 struct Request {
     uint64_t id = 0;
@@ -47,6 +89,7 @@ struct Request {
     birp_volume* vol = nullptr;  // captured at submission
     float* result = nullptr;
     size_t pixels = 0;
+    bool direct = false;  // result lies in a birp_host_alloc buffer: DMA straight into it
     double submit_s = 0, start_s = 0, done_s = 0;
     int batch_size = 0;
     int status = BIRP_OK;
@@ -123,6 +166,7 @@ struct Slot {
     size_t held = 0;              // device bytes charged to the budget for the images (under mu)
     std::vector<Request*> reqs;
     std::vector<size_t> offsets;
+    bool direct = false;          // this launch DMAs pinned results straight into their buffers
     std::atomic<bool> busy{false};     // admitted or launched, and not yet released
 };
 
@@ -186,6 +230,7 @@ int slot_alloc_images(Slot& s, size_t pixels) {
 // from its own view (latency_tuning), so a group has one tuning and a request renders the same
 // whatever shares its launch.
 constexpr int kGroups = BIRP_KDT_COUNT * 3;
+constexpr size_t kDirectBatch = 8;  // most requests of a launch that get per-request result DMAs
 int lane_index(int lanes) { return lanes == 4 ? 2 : (lanes == 2 ? 1 : 0); }
 int request_group(const Request* r, birp_tuning* t) {
     *t = latency_tuning(r->req.view);
@@ -248,7 +293,21 @@ int launch_slot(birp_server* S, Slot& s) {
                                t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll, grid, s.stream);
         if (rc) return rc;
     }
-    e = hipMemcpyAsync(s.h_images, s.d_images, total * 4, hipMemcpyDeviceToHost, s.stream);
+    // results: in a small batch one DMA straight into each pinned result buffer; the other images
+    // (all of them in a large batch, where per-request DMAs cost more than the pool's host copy) in
+    // one DMA to the slot's staging, from where the completion thread copies them out
+    s.direct = s.reqs.size() <= kDirectBatch;
+    size_t lo = total, hi = 0;
+    for (size_t k = 0; k < s.reqs.size() && e == hipSuccess; k++) {
+        const Request* r = s.reqs[k];
+        if (s.direct && r->direct) {
+            e = hipMemcpyAsync(r->result, s.d_images + s.offsets[k], r->pixels * 4, hipMemcpyDeviceToHost, s.stream);
+        } else {
+            lo = std::min(lo, s.offsets[k]);
+            hi = std::max(hi, s.offsets[k] + r->pixels);
+        }
+    }
+    if (e == hipSuccess && hi > lo) e = hipMemcpyAsync(s.h_images + lo, s.d_images + lo, (hi - lo) * 4, hipMemcpyDeviceToHost, s.stream);
     if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
     if (e != hipSuccess) return hip_fail(e, "server launch");
     return BIRP_OK;
@@ -258,10 +317,13 @@ int launch_slot(birp_server* S, Slot& s) {
 // the copy pool's threads.
 void copy_out(CopyPool& pool, Slot& s) {
     size_t total = 0;
-    for (Request* r : s.reqs) total += r->pixels * 4;
+    auto staged = [&s](size_t k) { return !(s.direct && s.reqs[k]->direct); };
     const size_t n = s.reqs.size();
-    auto run = [&s](size_t a, size_t b) {
-        for (size_t k = a; k < b; k++) memcpy(s.reqs[k]->result, s.h_images + s.offsets[k], s.reqs[k]->pixels * 4);
+    for (size_t k = 0; k < n; k++) total += staged(k) ? s.reqs[k]->pixels * 4 : 0;
+    if (!total) return;
+    auto run = [&s, &staged](size_t a, size_t b) {
+        for (size_t k = a; k < b; k++)
+            if (staged(k)) memcpy(s.reqs[k]->result, s.h_images + s.offsets[k], s.reqs[k]->pixels * 4);
     };
     const size_t parts = std::min<size_t>({n, (size_t)pool.helpers() + 1, std::max<size_t>(1, total >> 21)});
     if (parts <= 1) {
@@ -271,7 +333,7 @@ void copy_out(CopyPool& pool, Slot& s) {
     std::vector<size_t> cut(1, 0);
     size_t acc = 0, next = total / parts;
     for (size_t k = 0; k < n && cut.size() < parts; k++) {
-        acc += s.reqs[k]->pixels * 4;
+        acc += staged(k) ? s.reqs[k]->pixels * 4 : 0;
         if (acc >= next) {
             cut.push_back(k + 1);
             next += total / parts;
@@ -556,6 +618,7 @@ extern "C" int birp_server_submit(birp_server* S, const birp_request* req, float
     r->req.view = v;
     r->result = result;
     r->pixels = (size_t)v.width * v.height;
+    r->direct = pinned_range(result, r->pixels * 4);
     r->submit_s = birp_clock_s();
     {
         std::lock_guard<std::mutex> lk(S->mu);

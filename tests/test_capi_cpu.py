@@ -58,6 +58,10 @@ def test_null_arguments_are_rejected_before_any_device_work():
     assert L.birp_server_submit(None, None, None, None) == inv
     assert L.birp_server_wait(None, 1, 0.0, None) == inv
     assert L.birp_device_synchronize(None) == inv
+    p = C.c_void_p()
+    assert L.birp_host_alloc(0, C.byref(p)) == inv and not p.value
+    assert L.birp_host_free(None) == capi.OK
+    assert L.birp_host_free(h) == capi.ERR_NOT_FOUND  # not from birp_host_alloc: refused, not freed
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is visible")

@@ -358,3 +358,37 @@ def test_served_requests_equal_single_fast_renders(device, cube1):
         assert max(sizes) > 1
     finally:
         srv.destroy()
+
+
+def test_server_results_into_pinned_buffers(device, cube1):
+    """Requests whose result arrays come from birp_host_alloc get their images by DMA straight
+    into them in small launches (no staging copy), and through staging in large ones; one at a
+    time and in a burst that mixes pinned and ordinary result buffers, every image equals the
+    ordinary path's."""
+    vals, org, spc, vols, oc = cube1
+    srv = capi.Server(device, memory_budget=4 << 30, max_batch=16, max_inflight=3)
+    try:
+        idx = srv.add_volume(vols[capi.DTYPE_F16])
+        views = W.orbit_views_mixed(count=24, seed=5)
+        want = []
+        for v in views:
+            st, img, _ = srv.wait(srv.submit(idx, v), 30000.0)
+            assert st == capi.OK
+            want.append(img)
+        for k, v in enumerate(views[:6]):  # alone: launches of one request, direct DMA
+            out = capi.pinned_empty((v.height, v.width))
+            out.fill(np.nan)
+            st, img, info = srv.wait(srv.submit(idx, v, out=out), 30000.0)
+            assert st == capi.OK and info.batch_size == 1
+            assert np.array_equal(img.view(np.uint32), want[k].view(np.uint32)), k
+        outs = [capi.pinned_empty((v.height, v.width)) if k % 2 == 0 else np.empty((v.height, v.width), np.float32)
+                for k, v in enumerate(views)]
+        for o in outs:
+            o.fill(np.nan)
+        ids = [srv.submit(idx, v, out=o) for v, o in zip(views, outs)]
+        for k, rid in enumerate(ids):
+            st, img, _ = srv.wait(rid, 30000.0)
+            assert st == capi.OK and img is outs[k]
+            assert np.array_equal(img.view(np.uint32), want[k].view(np.uint32)), k
+    finally:
+        srv.destroy()
