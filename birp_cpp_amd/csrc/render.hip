@@ -856,13 +856,10 @@ __device__ __forceinline__ void fit_march(float& i0, float& d, float hi, int las
     if (!(e >= 0.f && e <= hi)) d = 0.f;
 }
 
-// One sample folded into a Kahan sum (sum, nc), nc holding the negated compensation -c:
-// y = x - c; t = sum + y; -c' = y - (t - sum); sum = t. An inactive lane adds x = 0.
-template <int DT>
-__device__ __forceinline__ void fast_accumulate(const DevVolume& v, __amdgpu_buffer_rsrc_t rs, const unsigned* tab,
-                                                float xd, float yd, float zd, bool act, float& sum, float& nc) {
-    const float x = fast_sample<DT>(v, rs, tab, xd, yd, zd);
-    const float y = (act ? x : 0.f) + nc;
+// x folded into a Kahan sum (sum, nc), nc holding the negated compensation -c:
+// y = x - c; t = sum + y; -c' = y - (t - sum); sum = t.
+__device__ __forceinline__ void kahan_add(float x, float& sum, float& nc) {
+    const float y = x + nc;
     const float t = sum + y;
     nc = y - (t - sum);
     sum = t;
@@ -970,8 +967,11 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
             fit_march(iy0, diy, __uint_as_float(__float_as_uint(V.fny - 1.0f) - 1u), cnt - 1);
             fit_march(iz0, diz, __uint_as_float(__float_as_uint(V.fnz - 1.0f) - 1u), cnt - 1);
         }
+        // the U samples of an iteration (neighbours on the ray, of similar size) are added plainly,
+        // and their partial sum enters the running sum by one Kahan step
         float sum = 0.f, nc = 0.f;
         for (int q = 0; q < trips; q += U) {
+            float part = 0.f;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int js = (q + u) * S + sub;  // sample offset from kf
@@ -982,8 +982,11 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
                 const float xd = __builtin_fmaf(fj, dix, ix0);
                 const float yd = __builtin_fmaf(fj, diy, iy0);
                 const float zd = __builtin_fmaf(fj, diz, iz0);
-                fast_accumulate<DT>(V, rs, tab, xd, yd, zd, act, sum, nc);
+                const float x = fast_sample<DT>(V, rs, tab, xd, yd, zd);
+                const float xa = act ? x : 0.f;
+                part = (u == 0) ? xa : part + xa;
             }
+            kahan_add(part, sum, nc);
         }
         // each sub-lane's Kahan total, then the S partials combined in sub order; the step weight
         // and the scale constant are applied once per ray (all in FP32)
