@@ -117,9 +117,18 @@ def bench_plain(capi, W, dev, steps, warmup):
     for _ in range(k2):
         vol.render(v, out=host)
     sync_rps = k2 / (time.perf_counter() - t1)
+    # the same into a birp_host_alloc (pinned) image, which the render fills by DMA
+    pin = capi.pinned_empty((v.height, v.width))
+    for _ in range(5):
+        vol.render(v, out=pin)
+    t1 = time.perf_counter()
+    for _ in range(k2):
+        vol.render(v, out=pin)
+    sync_rps_pinned = k2 / (time.perf_counter() - t1)
+    del pin
     img.free()
     return dict(vol=vol, vals=vals, org=org, spc=spc, view=v, n_in=n_in, wall=wall, dev_ms=dev_ms, sync_rps=sync_rps,
-                image=host)
+                sync_rps_pinned=sync_rps_pinned, image=host)
 
 
 def cpu_baseline(O, W, capi, vals, org, spc, v, n_in):
@@ -402,6 +411,7 @@ def main():
                      "kernel_ms": round(kern_ms, 5), "algorithmic_bytes": int(alg_bytes),
                      "valu": valu_roofline(kern_ms)},
         "forms": {"plain_renders_per_s": round(p["sync_rps"], 1),
+                  "plain_renders_per_s_pinned_image": round(p["sync_rps_pinned"], 1),
                   "plain_renders_per_s_async": round(args.steps / p["wall"], 1)},
     }
     # every result carries its error against the FP64 render of the same sample grid

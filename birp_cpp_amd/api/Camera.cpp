@@ -41,6 +41,23 @@ Camera::Camera(Space& cube, int width, int height, float plot_fov_h, float plot_
 
 Camera::~Camera() {}
 
+Camera::PinnedImage::~PinnedImage() {
+    if (p_) birp_host_free(p_);
+}
+
+float* Camera::PinnedImage::get(std::size_t n) {
+    if (n_ < n) {
+        if (p_) birp_host_free(p_);
+        p_ = nullptr;
+        n_ = 0;
+        void* q = nullptr;
+        birp_api::check(birp_host_alloc(n * sizeof(float), &q), "Camera::Render staging");
+        p_ = static_cast<float*>(q);
+        n_ = n;
+    }
+    return p_;
+}
+
 void Camera::SetPosition(float& x, float& y, float& z) { position = {x, y, z}; }
 void Camera::SetAim(float x, float y, float z) { aim = {x, y, z}; }
 
@@ -77,8 +94,7 @@ void Camera::Render() {
     if ((int)image.size() != H) throw std::invalid_argument("Camera::Render: image has the wrong number of rows");
     for (const auto& row : image)
         if ((int)row.size() != W) throw std::invalid_argument("Camera::Render: image row of the wrong length");
-    staging_.resize((size_t)W * H);
-    float* buf = staging_.data();
+    float* buf = staging_.get((size_t)W * H);
     v.flags = 0;
     if (math_ == BIRP_MATH_EXACT) {
         // the exact render adds to the image on the device (Camera.cpp:223); an image of +0.0

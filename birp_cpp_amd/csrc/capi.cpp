@@ -650,14 +650,25 @@ extern "C" int birp_render(birp_volume* vol, const birp_view* view, float* image
         return rc;
     }
     (void)hipEventRecord(dev->ev1, dev->stream);
-    // device -> pinned staging -> caller: one DMA of the counter and the image, one host copy
-    if ((rc = pinned_staging(dev, 256 + npix * 4))) return rc;
-    if ((e = hipMemcpyAsync(dev->pinned, dev->rbuf, 256 + npix * 4, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess ||
-        (e = hipStreamSynchronize(dev->stream)) != hipSuccess) {
-        dev->count_known = false;
-        return hip_fail(e, "render");
+    if (pinned_range(image, npix * 4)) {
+        // a birp_host_alloc image: the image by one DMA straight into it, the counter into staging
+        if ((rc = pinned_staging(dev, 256))) return rc;
+        if ((e = hipMemcpyAsync(dev->pinned, dev->rbuf, 8, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess ||
+            (e = hipMemcpyAsync(image, dimg, npix * 4, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess ||
+            (e = hipStreamSynchronize(dev->stream)) != hipSuccess) {
+            dev->count_known = false;
+            return hip_fail(e, "render");
+        }
+    } else {
+        // device -> pinned staging -> caller: one DMA of the counter and the image, one host copy
+        if ((rc = pinned_staging(dev, 256 + npix * 4))) return rc;
+        if ((e = hipMemcpyAsync(dev->pinned, dev->rbuf, 256 + npix * 4, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess ||
+            (e = hipStreamSynchronize(dev->stream)) != hipSuccess) {
+            dev->count_known = false;
+            return hip_fail(e, "render");
+        }
+        memcpy(image, (char*)dev->pinned + 256, npix * 4);
     }
-    memcpy(image, (char*)dev->pinned + 256, npix * 4);
     const unsigned long long count = *(const unsigned long long*)dev->pinned;
     const unsigned long long added = count - dev->count_base;
     dev->count_base = count;

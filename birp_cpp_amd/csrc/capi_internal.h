@@ -88,4 +88,6 @@ int tiles_of(const birp_view& v, int tile_w, int tile_h);
 bool take_injected_failure(birp_device* dev);
 int choose_tuning(const birp_volume* vol, const std::vector<birp_view>& views, birp_tuning& t);
 birp_tuning latency_tuning(const birp_view& v);
+// true when [p, p + bytes) lies inside one birp_host_alloc allocation (pinned host memory)
+bool pinned_range(const void* p, size_t bytes);
 }  // namespace birp

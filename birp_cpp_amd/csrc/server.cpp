@@ -44,8 +44,9 @@ namespace {
 // Pinned response buffers handed out by birp_host_alloc: base address -> size.
 std::mutex g_pinned_mu;
 std::map<uintptr_t, size_t> g_pinned;
+}  // namespace
 
-bool pinned_range(const void* p, size_t bytes) {
+bool birp::pinned_range(const void* p, size_t bytes) {
     const uintptr_t a = (uintptr_t)p;
     std::lock_guard<std::mutex> lk(g_pinned_mu);
     auto it = g_pinned.upper_bound(a);
@@ -53,7 +54,6 @@ bool pinned_range(const void* p, size_t bytes) {
     --it;
     return a >= it->first && a - it->first <= it->second && bytes <= it->second - (a - it->first);
 }
-}  // namespace
 
 extern "C" int birp_host_alloc(size_t bytes, void** out) {
     if (!out || bytes == 0 || bytes > ((size_t)1 << 40)) return fail(BIRP_ERR_INVALID_ARGUMENT, "bytes in [1, 2^40] and out required");

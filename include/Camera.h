@@ -1,6 +1,7 @@
 // Camera.h -- the SXI camera: ray generation, march and integration into an image
 // (reference include/Camera.h:20-63). Render runs on the GPU for DataCube and CMEM spaces.
 #pragma once
+#include <cstddef>
 #include <string>
 #include <vector>
 
@@ -58,5 +59,18 @@ private:
     int math_ = 0;
     long long last_in_cube_ = 0;
     double last_ms_ = 0.0;
-    std::vector<float> staging_;  // the flat H x W image handed to the C ABI, kept between renders
+    // The flat H x W image handed to the C ABI, kept between renders, in pinned host memory
+    // (birp_host_alloc) so that a render arrives in it by DMA. A copied Camera starts without one.
+    class PinnedImage {
+    public:
+        PinnedImage() = default;
+        PinnedImage(const PinnedImage&) {}
+        PinnedImage& operator=(const PinnedImage&) { return *this; }
+        ~PinnedImage();
+        float* get(std::size_t n);  // at least n floats
+    private:
+        float* p_ = nullptr;
+        std::size_t n_ = 0;
+    };
+    PinnedImage staging_;
 };

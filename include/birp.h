@@ -157,7 +157,8 @@ int birp_volume_sample(birp_volume* vol, const float* xyz, long long n, float* o
 
 /* ------------------------------------------------------------------ render (Camera) */
 /* Replaces Camera::Render (Camera.cpp:63-233): ray generation, march, integration. Host image
- * (H*W floats): read first when BIRP_RENDER_ACCUMULATE is set, always written. Synchronous. */
+ * (H*W floats): read first when BIRP_RENDER_ACCUMULATE is set, always written. Synchronous. An
+ * image inside a birp_host_alloc allocation is written by DMA, with no host-side copy. */
 int birp_render(birp_volume* vol, const birp_view* view, float* image, birp_render_stats* stats);
 
 /* Device-resident images for pipelines that keep results on the GPU. */

@@ -254,6 +254,29 @@ def test_camera_at_aim_point_and_invalid_views(shell128, oracle):
     assert e.value.status == capi.ERR_INVALID_ARGUMENT
 
 
+def test_render_into_pinned_image(device, shell128, oracle):
+    """birp_render into a birp_host_alloc image (filled by DMA, no host copy): bitwise the oracle's
+    image and sample count, fresh and accumulating, exact and fast."""
+    vol, oc = shell128
+    v = W.view_144x84()
+    pin = capi.pinned_empty((v.height, v.width))
+    pin.fill(0.0)
+    ref, cnt = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT)
+    img, st = vol.render(v, out=pin)
+    assert img is pin and st.samples_in_cube == cnt
+    assert_bitwise(pin, ref)
+    v.flags = capi.RENDER_ACCUMULATE
+    img, st = vol.render(v, out=pin)  # adds a second render onto the first
+    ref2, _ = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT, init=ref)
+    assert st.samples_in_cube == cnt
+    assert_bitwise(pin, ref2)
+    f = W.view_144x84()
+    f.math = capi.MATH_FAST
+    want, _ = vol.render(f)
+    img, _ = vol.render(f, out=pin)
+    assert_bitwise(pin, want)
+
+
 def test_render_async_into_device_image(device, shell128, oracle):
     vol, oc = shell128
     v = W.view_144x84()
