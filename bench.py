@@ -341,8 +341,8 @@ def bench_serving(capi, W, dev, num=2000, rate=200.0):
     # first burst after the light Poisson load still runs at ramping clocks
     _serve(capi, srv, idx, which, views, None, bufs)
     out["saturated"] = _serve(capi, srv, idx, which, views, None, bufs)
-    # the same with response buffers from birp_host_alloc: each image arrives by one DMA
-    # straight into its buffer (no staging copy)
+    # the same with response buffers from birp_host_alloc: the kernel writes each image straight
+    # into its buffer (no device image, DMA or host copy)
     pinned = [capi.pinned_empty((v.height, v.width)) for v in views]
     for b in pinned:
         b.fill(1.0)

@@ -382,7 +382,7 @@ class Server:
 
 def pinned_empty(shape, dtype=np.float32) -> np.ndarray:
     """An uninitialised array in pinned host memory from birp_host_alloc (freed with the array):
-    a server request whose result is such an array gets its image by one DMA, no staging copy."""
+    a server request whose result is such an array has its image written there by the kernel."""
     n = int(np.prod(shape)) * np.dtype(dtype).itemsize
     p = C.c_void_p()
     check(lib().birp_host_alloc(max(n, 1), C.byref(p)), "birp_host_alloc")

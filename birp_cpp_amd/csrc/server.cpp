@@ -43,7 +43,20 @@ extern "C" double birp_clock_s(void) {
 namespace {
 // Pinned response buffers handed out by birp_host_alloc: base address -> size.
 std::mutex g_pinned_mu;
-std::map<uintptr_t, size_t> g_pinned;
+std::map<uintptr_t, std::pair<size_t, char*>> g_pinned;  // base -> (size, device address of base)
+
+// The device address of [p, p + bytes) when it lies inside one birp_host_alloc allocation that
+// the device maps, else null.
+float* pinned_device_ptr(const void* p, size_t bytes) {
+    const uintptr_t a = (uintptr_t)p;
+    std::lock_guard<std::mutex> lk(g_pinned_mu);
+    auto it = g_pinned.upper_bound(a);
+    if (it == g_pinned.begin()) return nullptr;
+    --it;
+    const size_t size = it->second.first;
+    if (!it->second.second || a - it->first > size || bytes > size - (a - it->first)) return nullptr;
+    return (float*)(it->second.second + (a - it->first));
+}
 }  // namespace
 
 bool birp::pinned_range(const void* p, size_t bytes) {
@@ -52,7 +65,7 @@ bool birp::pinned_range(const void* p, size_t bytes) {
     auto it = g_pinned.upper_bound(a);
     if (it == g_pinned.begin()) return false;
     --it;
-    return a >= it->first && a - it->first <= it->second && bytes <= it->second - (a - it->first);
+    return a >= it->first && a - it->first <= it->second.first && bytes <= it->second.first - (a - it->first);
 }
 
 extern "C" int birp_host_alloc(size_t bytes, void** out) {
@@ -61,9 +74,15 @@ extern "C" int birp_host_alloc(size_t bytes, void** out) {
     void* p = nullptr;
     hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
     if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(response buffer)");
+    // the device-side address of the allocation, for kernels that write results into it
+    void* d = nullptr;
+    if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        d = nullptr;  // not mapped: requests into it take the staging path
+    }
     {
         std::lock_guard<std::mutex> lk(g_pinned_mu);
-        g_pinned[(uintptr_t)p] = bytes;
+        g_pinned[(uintptr_t)p] = {bytes, (char*)d};
     }
     *out = p;
     return BIRP_OK;
@@ -89,7 +108,7 @@ struct Request {
     birp_volume* vol = nullptr;  // captured at submission
     float* result = nullptr;
     size_t pixels = 0;
-    bool direct = false;  // result lies in a birp_host_alloc buffer: DMA straight into it
+    float* dev_out = nullptr;  // result lies in a birp_host_alloc buffer: its device address
     double submit_s = 0, start_s = 0, done_s = 0;
     int batch_size = 0;
     int status = BIRP_OK;
@@ -166,7 +185,6 @@ struct Slot {
     size_t held = 0;              // device bytes charged to the budget for the images (under mu)
     std::vector<Request*> reqs;
     std::vector<size_t> offsets;
-    bool direct = false;          // this launch DMAs pinned results straight into their buffers
     std::atomic<bool> busy{false};     // admitted or launched, and not yet released
 };
 
@@ -230,7 +248,6 @@ int slot_alloc_images(Slot& s, size_t pixels) {
 // from its own view (latency_tuning), so a group has one tuning and a request renders the same
 // whatever shares its launch.
 constexpr int kGroups = BIRP_KDT_COUNT * 3;
-constexpr size_t kDirectBatch = 8;  // most requests of a launch that get per-request result DMAs
 int lane_index(int lanes) { return lanes == 4 ? 2 : (lanes == 2 ? 1 : 0); }
 int request_group(const Request* r, birp_tuning* t) {
     *t = latency_tuning(r->req.view);
@@ -264,7 +281,9 @@ int launch_slot(birp_server* S, Slot& s) {
         Request* r = s.reqs[k];
         birp_tuning t;
         const int g = request_group(r, &t);
-        rc = fast_pose(r->vol, &r->req.view, r->req.volume, s.d_images + s.offsets[k], t.tile_w, t.tile_h, s.h_poses[fill[g]++]);
+        // a result in a birp_host_alloc buffer is written there by the kernel (zero-copy)
+        float* out = r->dev_out ? r->dev_out : s.d_images + s.offsets[k];
+        rc = fast_pose(r->vol, &r->req.view, r->req.volume, out, t.tile_w, t.tile_h, s.h_poses[fill[g]++]);
         if (rc) return rc;
     }
     for (int g = 0; g < kGroups; g++) {
@@ -293,19 +312,13 @@ int launch_slot(birp_server* S, Slot& s) {
                                t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll, grid, s.stream);
         if (rc) return rc;
     }
-    // results: in a small batch one DMA straight into each pinned result buffer; the other images
-    // (all of them in a large batch, where per-request DMAs cost more than the pool's host copy) in
-    // one DMA to the slot's staging, from where the completion thread copies them out
-    s.direct = s.reqs.size() <= kDirectBatch;
+    // the other results: one DMA to the slot's staging, from where the completion thread copies
+    // them out
     size_t lo = total, hi = 0;
-    for (size_t k = 0; k < s.reqs.size() && e == hipSuccess; k++) {
-        const Request* r = s.reqs[k];
-        if (s.direct && r->direct) {
-            e = hipMemcpyAsync(r->result, s.d_images + s.offsets[k], r->pixels * 4, hipMemcpyDeviceToHost, s.stream);
-        } else {
-            lo = std::min(lo, s.offsets[k]);
-            hi = std::max(hi, s.offsets[k] + r->pixels);
-        }
+    for (size_t k = 0; k < s.reqs.size(); k++) {
+        if (s.reqs[k]->dev_out) continue;
+        lo = std::min(lo, s.offsets[k]);
+        hi = std::max(hi, s.offsets[k] + s.reqs[k]->pixels);
     }
     if (e == hipSuccess && hi > lo) e = hipMemcpyAsync(s.h_images + lo, s.d_images + lo, (hi - lo) * 4, hipMemcpyDeviceToHost, s.stream);
     if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
@@ -317,7 +330,7 @@ int launch_slot(birp_server* S, Slot& s) {
 // the copy pool's threads.
 void copy_out(CopyPool& pool, Slot& s) {
     size_t total = 0;
-    auto staged = [&s](size_t k) { return !(s.direct && s.reqs[k]->direct); };
+    auto staged = [&s](size_t k) { return !s.reqs[k]->dev_out; };
     const size_t n = s.reqs.size();
     for (size_t k = 0; k < n; k++) total += staged(k) ? s.reqs[k]->pixels * 4 : 0;
     if (!total) return;
@@ -618,7 +631,7 @@ extern "C" int birp_server_submit(birp_server* S, const birp_request* req, float
     r->req.view = v;
     r->result = result;
     r->pixels = (size_t)v.width * v.height;
-    r->direct = pinned_range(result, r->pixels * 4);
+    r->dev_out = pinned_device_ptr(result, r->pixels * 4);
     r->submit_s = birp_clock_s();
     {
         std::lock_guard<std::mutex> lk(S->mu);

@@ -255,10 +255,11 @@ int birp_server_submit(birp_server* s, const birp_request* req, float* result, u
 int birp_server_wait(birp_server* s, uint64_t id, double timeout_ms, birp_result_info* info);
 /* Steady-clock seconds as used in birp_result_info. */
 double birp_clock_s(void);
-/* Pinned (page-locked) host memory for response buffers. A request whose result buffer lies
- * inside one of these allocations, launched in a batch of at most 8 requests, receives its image
- * by one DMA straight from the device with no staging copy (larger batches stage their images
- * as usual). Free only after every request writing into it has been waited for. */
+/* Pinned (page-locked) host memory for response buffers, mapped into the device's address space.
+ * A served request whose result buffer lies inside one of these allocations has its image
+ * written there by the render kernel itself (no device image, DMA or host copy); birp_render
+ * into one DMAs the image straight into it. Free only after every request writing into it has
+ * been waited for. */
 int birp_host_alloc(size_t bytes, void** out);
 int birp_host_free(void* p);
 

@@ -361,10 +361,9 @@ def test_served_requests_equal_single_fast_renders(device, cube1):
 
 
 def test_server_results_into_pinned_buffers(device, cube1):
-    """Requests whose result arrays come from birp_host_alloc get their images by DMA straight
-    into them in small launches (no staging copy), and through staging in large ones; one at a
-    time and in a burst that mixes pinned and ordinary result buffers, every image equals the
-    ordinary path's."""
+    """Requests whose result arrays come from birp_host_alloc get their images written there by the
+    render kernel (zero-copy through the device mapping); one at a time and in a burst that mixes
+    pinned and ordinary result buffers, every image equals the ordinary (staged) path's."""
     vals, org, spc, vols, oc = cube1
     srv = capi.Server(device, memory_budget=4 << 30, max_batch=16, max_inflight=3)
     try:
@@ -375,7 +374,7 @@ def test_server_results_into_pinned_buffers(device, cube1):
             st, img, _ = srv.wait(srv.submit(idx, v), 30000.0)
             assert st == capi.OK
             want.append(img)
-        for k, v in enumerate(views[:6]):  # alone: launches of one request, direct DMA
+        for k, v in enumerate(views[:6]):  # alone: launches of one request
             out = capi.pinned_empty((v.height, v.width))
             out.fill(np.nan)
             st, img, info = srv.wait(srv.submit(idx, v, out=out), 30000.0)
