@@ -629,6 +629,9 @@ extern "C" int birp_render(birp_volume* vol, const birp_view* view, float* image
     float* dimg = nullptr;
     unsigned long long* cnt = nullptr;
     if ((rc = render_buffer(dev, npix, &dimg, &cnt))) return rc;
+    // a birp_host_alloc image is read and written by the kernel in place, through its device address
+    float* mapped = pinned_device_ptr(image, npix * 4);
+    if (mapped) dimg = mapped;
     hipError_t e;
     birp_view vv = *view;
     if (vv.flags & BIRP_RENDER_ACCUMULATE) {
@@ -637,7 +640,7 @@ extern "C" int birp_render(birp_volume* vol, const birp_view* view, float* image
         size_t k = 0;
         while (k < npix && bits[k] == 0u) k++;
         if (k == npix) vv.flags &= ~BIRP_RENDER_ACCUMULATE;
-        else if ((e = hipMemcpyAsync(dimg, image, npix * 4, hipMemcpyHostToDevice, dev->stream)) != hipSuccess)
+        else if (!mapped && (e = hipMemcpyAsync(dimg, image, npix * 4, hipMemcpyHostToDevice, dev->stream)) != hipSuccess)
             return hip_fail(e, "hipMemcpy(image)");
     }
     view = &vv;
@@ -650,11 +653,12 @@ extern "C" int birp_render(birp_volume* vol, const birp_view* view, float* image
         return rc;
     }
     (void)hipEventRecord(dev->ev1, dev->stream);
-    if (pinned_range(image, npix * 4)) {
-        // a birp_host_alloc image: the image by one DMA straight into it, the counter into staging
+    if (mapped || pinned_range(image, npix * 4)) {
+        // a birp_host_alloc image: written in place by the kernel, or else by one DMA straight into
+        // it; the counter comes back through staging
         if ((rc = pinned_staging(dev, 256))) return rc;
         if ((e = hipMemcpyAsync(dev->pinned, dev->rbuf, 8, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess ||
-            (e = hipMemcpyAsync(image, dimg, npix * 4, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess ||
+            (!mapped && (e = hipMemcpyAsync(image, dimg, npix * 4, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess) ||
             (e = hipStreamSynchronize(dev->stream)) != hipSuccess) {
             dev->count_known = false;
             return hip_fail(e, "render");

@@ -90,4 +90,6 @@ int choose_tuning(const birp_volume* vol, const std::vector<birp_view>& views, b
 birp_tuning latency_tuning(const birp_view& v);
 // true when [p, p + bytes) lies inside one birp_host_alloc allocation (pinned host memory)
 bool pinned_range(const void* p, size_t bytes);
+// its device address when the allocation is mapped into the device's address space, else null
+float* pinned_device_ptr(const void* p, size_t bytes);
 }  // namespace birp

@@ -45,9 +45,11 @@ namespace {
 std::mutex g_pinned_mu;
 std::map<uintptr_t, std::pair<size_t, char*>> g_pinned;  // base -> (size, device address of base)
 
+}  // namespace
+
 // The device address of [p, p + bytes) when it lies inside one birp_host_alloc allocation that
 // the device maps, else null.
-float* pinned_device_ptr(const void* p, size_t bytes) {
+float* birp::pinned_device_ptr(const void* p, size_t bytes) {
     const uintptr_t a = (uintptr_t)p;
     std::lock_guard<std::mutex> lk(g_pinned_mu);
     auto it = g_pinned.upper_bound(a);
@@ -57,7 +59,6 @@ float* pinned_device_ptr(const void* p, size_t bytes) {
     if (!it->second.second || a - it->first > size || bytes > size - (a - it->first)) return nullptr;
     return (float*)(it->second.second + (a - it->first));
 }
-}  // namespace
 
 bool birp::pinned_range(const void* p, size_t bytes) {
     const uintptr_t a = (uintptr_t)p;

@@ -158,7 +158,7 @@ int birp_volume_sample(birp_volume* vol, const float* xyz, long long n, float* o
 /* ------------------------------------------------------------------ render (Camera) */
 /* Replaces Camera::Render (Camera.cpp:63-233): ray generation, march, integration. Host image
  * (H*W floats): read first when BIRP_RENDER_ACCUMULATE is set, always written. Synchronous. An
- * image inside a birp_host_alloc allocation is written by DMA, with no host-side copy. */
+ * image inside a birp_host_alloc allocation is read and written by the kernel in place. */
 int birp_render(birp_volume* vol, const birp_view* view, float* image, birp_render_stats* stats);
 
 /* Device-resident images for pipelines that keep results on the GPU. */
@@ -257,9 +257,9 @@ int birp_server_wait(birp_server* s, uint64_t id, double timeout_ms, birp_result
 double birp_clock_s(void);
 /* Pinned (page-locked) host memory for response buffers, mapped into the device's address space.
  * A served request whose result buffer lies inside one of these allocations has its image
- * written there by the render kernel itself (no device image, DMA or host copy); birp_render
- * into one DMAs the image straight into it. Free only after every request writing into it has
- * been waited for. */
+ * written there by the render kernel itself (no device image, DMA or host copy), and
+ * birp_render reads and writes such an image in place the same way. Free only after every
+ * request writing into it has been waited for. */
 int birp_host_alloc(size_t bytes, void** out);
 int birp_host_free(void* p);
 
