@@ -8,9 +8,11 @@
 // or can grow to hold them within the budget (releasing idle slots' buffers when that makes the
 // room), and launches them as one persistent-kernel batch (one launch per storage type and lanes
 // per ray). A request that arrives while other batches are on the GPU is launched in the next
-// free slot, so its work overlaps theirs on the device. When a slot's device work is done, a completion thread
-// copies its images from pinned staging into the callers' buffers (split over a pool of helper
-// threads for large batches) and releases the slot, so the dispatcher keeps launching meanwhile.
+// free slot, so its work overlaps theirs on the device. A result buffer from birp_host_alloc is
+// written by the kernel itself through its device address; the other images come back to the
+// slot's pinned staging in one DMA, and when a slot's device work is done a completion thread
+// copies them into the callers' buffers (split over a pool of helper threads for large batches)
+// and releases the slot, so the dispatcher keeps launching meanwhile.
 // Every request is rendered with the tuning its own view selects (the one fast single renders
 // use) and the same per-tile arithmetic whatever it is batched with, so results do not depend on
 // the mix and equal birp_render's.
@@ -213,7 +215,6 @@ struct birp_server {
     uint64_t wake_gen = 0;  // bumped (under mu) by submissions, released slots and shutdown
     bool stop = false;
     bool stop_copy = false;
-    int setup_error = BIRP_OK;
 };
 
 namespace {
@@ -403,7 +404,9 @@ void completer(birp_server* S) {
                 copy_out(S->copy_pool, *s);
                 finish_slot(S, *s, BIRP_OK);
             } else {
-                finish_slot(S, *s, hip_fail(q, "server batch"));
+                const int rc = hip_fail(q, "server batch");
+                (void)hipStreamSynchronize(s->stream);
+                finish_slot(S, *s, rc);
             }
             progressed = true;
         }
@@ -492,6 +495,9 @@ void dispatcher(birp_server* S) {
             lk.unlock();
             const int rc = launch_slot(S, *free_slot);
             if (rc) {
+                // work queued before the failure may still write the callers' pinned buffers:
+                // let it drain before the requests are reported failed
+                (void)hipStreamSynchronize(free_slot->stream);
                 finish_slot(S, *free_slot, rc);
                 lk.lock();
             } else {
