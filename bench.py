@@ -202,14 +202,29 @@ def bench_batched(capi, W, dev, vals, org, spc, fp32_vol):
     plain_el = time.perf_counter() - t
     # accuracy of the batch against the FP64 render (a sample of images)
     errs = [rel_l2(b.image(k), fp32_vol.render_f64(views[k])) for k in range(0, 256, 16)]
+    # the same batch with its images written by the kernel into pinned host memory
+    # (birp_batch_set_output): a finished launch has all 256 images on the host
+    pixels = sum(x.width * x.height for x in views)
+    arena = capi.pinned_empty((pixels,))
+    b.set_output(arena)
+    for _ in range(2):
+        b.launch()
+        b.wait()
+    t = time.perf_counter()
+    for _ in range(reps):
+        b.launch()
+        b.wait()
+    el_host = (time.perf_counter() - t) / reps
+    b.set_output(None)
     # in-cube samples of the 256 views (the exact path's count; the fast path's FP64 entry can
     # differ by a sample at a face) and the batch's algorithmic bytes: 16 B per F16 sample + 4 B/px
     in_cube = sum(fp32_vol.count_samples(x) for x in ex)
-    pixels = sum(x.width * x.height for x in views)
     alg = 16.0 * in_cube + 4.0 * pixels
     b.free()
+    del arena
     vol16.free()
     return {"poses": 256, "storage": "f16", "renders_per_s": round(256 / el, 1), "ms_per_batch": round(el * 1e3, 3),
+            "renders_per_s_images_in_host_memory": round(256 / el_host, 1),
             "device_ms_per_batch": round(last_ms, 3), "nominal_gsamples_per_s": round(nominal / el / 1e9, 2),
             "plain_renders_per_s_same_views": round(256 / plain_el, 1), "speedup_vs_plain": round(plain_el / el, 2),
             "max_rel_l2_vs_fp64": float(f"{max(errs):.3e}"), "in_cube_samples": int(in_cube),

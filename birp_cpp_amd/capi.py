@@ -117,6 +117,7 @@ SIGNATURES = [
     ("birp_batch_wait", _i, [_P]),
     ("birp_batch_download", _i, [_P, _i, _FP]),
     ("birp_batch_stats", _i, [_P, C.POINTER(C.c_longlong), _DP]),
+    ("birp_batch_set_output", _i, [_P, _FP]),
     ("birp_autotune", _i, [_P, C.POINTER(View), _i, C.c_char_p]),
     ("birp_autotune_load", _i, [C.c_char_p]),
     ("birp_autotune_lookup", _i, [_i, _i, _i, _i, C.POINTER(Tuning)]),
@@ -328,6 +329,16 @@ class Batch:
         out = np.zeros((v.height, v.width), np.float32)
         check(lib().birp_batch_download(self.h, k, fptr(out)), "birp_batch_download")
         return out
+
+    def set_output(self, host: np.ndarray | None):
+        """Have the kernel write every image into `host` (a pinned_empty float32 array of the
+        batch's total pixels, images in view order), or back into device memory for None."""
+        if host is not None:
+            total = sum(v.width * v.height for v in self.views)
+            if host.dtype != np.float32 or host.size < total or not host.flags.c_contiguous:
+                raise ValueError("host must be a C-contiguous float32 array of the batch's total pixels")
+        check(lib().birp_batch_set_output(self.h, fptr(host) if host is not None else None), "birp_batch_set_output")
+        self._out = host
 
     def stats(self):
         n = C.c_longlong()

@@ -871,7 +871,8 @@ int batch_setup(birp_batch* b) {
     std::vector<FastPose> poses(n);
     std::vector<int> prefix(n + 1, 0);
     for (int k = 0; k < n; k++) {
-        int rc = fast_pose(b->vol, &b->views[k], 0, b->d_images + b->offsets[k], t.tile_w, t.tile_h, poses[k]);
+        float* out = (b->host_out_dev ? b->host_out_dev : b->d_images) + b->offsets[k];
+        int rc = fast_pose(b->vol, &b->views[k], 0, out, t.tile_w, t.tile_h, poses[k]);
         if (rc) return rc;
         prefix[k + 1] = prefix[k] + tiles_of(b->views[k], t.tile_w, t.tile_h);
     }
@@ -1012,10 +1013,29 @@ extern "C" int birp_batch_wait(birp_batch* b) {
 extern "C" int birp_batch_download(birp_batch* b, int index, float* host) {
     if (!b || !host || index < 0 || index >= (int)b->views.size()) return fail(BIRP_ERR_INVALID_ARGUMENT, "bad argument");
     const birp_view& v = b->views[index];
-    hipError_t e = hipMemcpyAsync(host, b->d_images + b->offsets[index], (size_t)v.width * v.height * 4, hipMemcpyDeviceToHost,
-                                  b->vol->dev->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(b->vol->dev->stream);
+    const size_t bytes = (size_t)v.width * v.height * 4;
+    hipError_t e = hipSuccess;
+    if (b->host_out) {  // already in host memory once the launch is done
+        e = hipStreamSynchronize(b->vol->dev->stream);
+        if (e == hipSuccess && host != b->host_out + b->offsets[index]) memcpy(host, b->host_out + b->offsets[index], bytes);
+    } else {
+        e = hipMemcpyAsync(host, b->d_images + b->offsets[index], bytes, hipMemcpyDeviceToHost, b->vol->dev->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->vol->dev->stream);
+    }
     return e == hipSuccess ? BIRP_OK : hip_fail(e, "batch download");
+}
+
+extern "C" int birp_batch_set_output(birp_batch* b, float* host) {
+    if (!b) return fail(BIRP_ERR_INVALID_ARGUMENT, "null batch");
+    float* dev_addr = nullptr;
+    if (host) {
+        dev_addr = pinned_device_ptr(host, b->total_pixels * 4);
+        if (!dev_addr) return fail(BIRP_ERR_INVALID_ARGUMENT, "output must be a device-mapped birp_host_alloc buffer of the batch's pixels");
+    }
+    (void)hipStreamSynchronize(b->vol->dev->stream);
+    b->host_out = host;
+    b->host_out_dev = dev_addr;
+    return batch_setup(b);
 }
 
 extern "C" int birp_batch_stats(birp_batch* b, long long* nominal, double* last_ms) {

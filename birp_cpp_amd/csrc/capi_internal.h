@@ -64,6 +64,8 @@ struct birp_batch {
     size_t total_pixels = 0;
     long long nominal = 0;
     float* d_images = nullptr;
+    float* host_out = nullptr;      // birp_batch_set_output: images go to this pinned host arena
+    float* host_out_dev = nullptr;  // its device address
     birp::FastPose* d_poses = nullptr;
     int* d_prefix = nullptr;
     birp::DevVolume* d_vols = nullptr;

@@ -202,6 +202,11 @@ int birp_batch_launch_eager(birp_batch* b);
 int birp_batch_wait(birp_batch* b);
 /* Copy image i (views[i].height * views[i].width floats) to host memory. */
 int birp_batch_download(birp_batch* b, int index, float* host);
+/* Directs the batch's images into `host`, a birp_host_alloc buffer of at least the batch's total
+ * pixels (image i at the sum of the earlier images' W*H floats): the kernel writes them there
+ * itself, so a finished launch has every image in host memory. NULL returns the images to the
+ * device buffer. Re-captures the launch. */
+int birp_batch_set_output(birp_batch* b, float* host);
 int birp_batch_stats(birp_batch* b, long long* samples_nominal, double* last_ms);
 
 /* Autotuner: times every candidate birp_tuning on the given views (few launches each), keeps

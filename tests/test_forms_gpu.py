@@ -166,6 +166,42 @@ def test_batch_matches_fp64_and_replays(cube1):
         b.free()
 
 
+def test_batch_output_into_pinned_host_memory(cube1):
+    """birp_batch_set_output: the kernel writes every image of the batch into a pinned host arena
+    (graph replays and eager launches), bit for bit the images it writes to device memory; the
+    output must be a birp_host_alloc buffer of the batch's pixels, and None restores device
+    images."""
+    vals, org, spc, vols, oc = cube1
+    views = W.orbit_views_mixed(count=24, seed=9)
+    b = capi.Batch(vols[capi.DTYPE_F16], views)
+    try:
+        b.launch()
+        b.wait()
+        want = [b.image(k) for k in range(len(views))]
+        offs = np.cumsum([0] + [v.width * v.height for v in views])
+        with pytest.raises(capi.BirpError) as e:
+            b.set_output(np.zeros(offs[-1], np.float32))  # pageable memory is refused
+        assert e.value.status == capi.ERR_INVALID_ARGUMENT
+        arena = capi.pinned_empty((int(offs[-1]),))
+        b.set_output(arena)
+        for launch in (b.launch, b.launch_eager, b.launch):
+            arena.fill(np.nan)
+            launch()
+            b.wait()
+            for k, v in enumerate(views):
+                img = arena[offs[k]:offs[k + 1]].reshape(v.height, v.width)
+                assert np.array_equal(img.view(np.uint32), want[k].view(np.uint32)), k
+                assert np.array_equal(b.image(k), want[k])
+        b.set_output(None)
+        arena.fill(np.nan)
+        b.launch()
+        b.wait()
+        assert np.isnan(arena).all()  # back to device images
+        assert np.array_equal(b.image(3), want[3])
+    finally:
+        b.free()
+
+
 @pytest.mark.parametrize("tuning", [(8, 8, 1, 1), (16, 4, 1, 2), (8, 4, 2, 4), (4, 8, 2, 1), (4, 4, 4, 2), (8, 2, 4, 4)])
 def test_batch_tunings_agree(cube1, tuning):
     vals, org, spc, vols, oc = cube1
