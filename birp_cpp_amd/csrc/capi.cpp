@@ -156,11 +156,16 @@ int pinned_staging(birp_device* dev, size_t bytes) {
     if (dev->pinned_bytes < bytes) {
         if (dev->pinned) (void)hipHostFree(dev->pinned);
         dev->pinned = nullptr;
+        dev->pinned_dev = nullptr;
         dev->pinned_bytes = 0;
         hipError_t e = hipHostMalloc(&dev->pinned, bytes, hipHostMallocDefault);
         if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(staging)");
         memset(dev->pinned, 0, bytes);  // fault the pages in now, not on the first render's copy
         dev->pinned_bytes = bytes;
+        if (hipHostGetDevicePointer(&dev->pinned_dev, dev->pinned, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            dev->pinned_dev = nullptr;  // not mapped: images come back by DMA
+        }
     }
     return BIRP_OK;
 }
@@ -629,9 +634,20 @@ extern "C" int birp_render(birp_volume* vol, const birp_view* view, float* image
     float* dimg = nullptr;
     unsigned long long* cnt = nullptr;
     if ((rc = render_buffer(dev, npix, &dimg, &cnt))) return rc;
-    // a birp_host_alloc image is read and written by the kernel in place, through its device address
+    // a birp_host_alloc image is read and written by the kernel in place, through its device
+    // address; any other image goes through the device's pinned staging, which the kernel also
+    // writes in place when the device maps it (then only the sample counter comes back by DMA)
     float* mapped = pinned_device_ptr(image, npix * 4);
-    if (mapped) dimg = mapped;
+    float* staged = nullptr;  // the image's place in the pinned staging, when the kernel writes it
+    if (mapped) {
+        dimg = mapped;
+    } else {
+        if ((rc = pinned_staging(dev, 256 + npix * 4))) return rc;
+        if (dev->pinned_dev) {
+            dimg = (float*)((char*)dev->pinned_dev + 256);
+            staged = (float*)((char*)dev->pinned + 256);
+        }
+    }
     hipError_t e;
     birp_view vv = *view;
     if (vv.flags & BIRP_RENDER_ACCUMULATE) {
@@ -640,6 +656,7 @@ extern "C" int birp_render(birp_volume* vol, const birp_view* view, float* image
         size_t k = 0;
         while (k < npix && bits[k] == 0u) k++;
         if (k == npix) vv.flags &= ~BIRP_RENDER_ACCUMULATE;
+        else if (staged) memcpy(staged, image, npix * 4);
         else if (!mapped && (e = hipMemcpyAsync(dimg, image, npix * 4, hipMemcpyHostToDevice, dev->stream)) != hipSuccess)
             return hip_fail(e, "hipMemcpy(image)");
     }
@@ -653,16 +670,17 @@ extern "C" int birp_render(birp_volume* vol, const birp_view* view, float* image
         return rc;
     }
     (void)hipEventRecord(dev->ev1, dev->stream);
-    if (mapped || pinned_range(image, npix * 4)) {
-        // a birp_host_alloc image: written in place by the kernel, or else by one DMA straight into
-        // it; the counter comes back through staging
+    if (mapped || staged || pinned_range(image, npix * 4)) {
+        // the image is in place (in the caller's pinned buffer or in the staging), or goes by one DMA
+        // straight into a pinned caller buffer; the counter comes back through staging
         if ((rc = pinned_staging(dev, 256))) return rc;
         if ((e = hipMemcpyAsync(dev->pinned, dev->rbuf, 8, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess ||
-            (!mapped && (e = hipMemcpyAsync(image, dimg, npix * 4, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess) ||
+            (!mapped && !staged && (e = hipMemcpyAsync(image, dimg, npix * 4, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess) ||
             (e = hipStreamSynchronize(dev->stream)) != hipSuccess) {
             dev->count_known = false;
             return hip_fail(e, "render");
         }
+        if (staged) memcpy(image, staged, npix * 4);
     } else {
         // device -> pinned staging -> caller: one DMA of the counter and the image, one host copy
         if ((rc = pinned_staging(dev, 256 + npix * 4))) return rc;

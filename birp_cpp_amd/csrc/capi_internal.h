@@ -29,6 +29,7 @@ struct birp_device {
     bool fast_pending[kFastRing] = {};
     int fast_next = 0;
     void* pinned = nullptr;            // pinned host staging for birp_render's image copy
+    void* pinned_dev = nullptr;        // its device address (null when the device does not map it)
     size_t pinned_bytes = 0;
     void* rbuf = nullptr;              // birp_render: [u64 sample counter | pad to 256 B | image]
     size_t rbuf_bytes = 0;
