@@ -9,10 +9,10 @@
 // room), and launches them as one persistent-kernel batch (one launch per storage type and lanes
 // per ray). A request that arrives while other batches are on the GPU is launched in the next
 // free slot, so its work overlaps theirs on the device. A result buffer from birp_host_alloc is
-// written by the kernel itself through its device address; the other images come back to the
-// slot's pinned staging in one DMA, and when a slot's device work is done a completion thread
-// copies them into the callers' buffers (split over a pool of helper threads for large batches)
-// and releases the slot, so the dispatcher keeps launching meanwhile.
+// written by the kernel itself through its device address, the other images into the slot's
+// pinned staging the same way, and when a slot's device work is done a completion thread copies
+// those into the callers' buffers (split over a pool of helper threads for large batches) and
+// releases the slot, so the dispatcher keeps launching meanwhile.
 // Every request is rendered with the tuning its own view selects (the one fast single renders
 // use) and the same per-tile arithmetic whatever it is batched with, so results do not depend on
 // the mix and equal birp_render's.
@@ -184,6 +184,7 @@ struct Slot {
     unsigned* d_work = nullptr;   // work-queue counters, 256 per kernel group
     float* d_images = nullptr;
     float* h_images = nullptr;    // pinned
+    float* h_images_dev = nullptr;  // its device address: kernels write the staged images there
     size_t image_cap = 0;         // pixels allocated (dispatcher thread)
     size_t held = 0;              // device bytes charged to the budget for the images (under mu)
     std::vector<Request*> reqs;
@@ -224,6 +225,7 @@ void slot_release_images(Slot& s) {
     if (s.h_images) (void)hipHostFree(s.h_images);
     s.d_images = nullptr;
     s.h_images = nullptr;
+    s.h_images_dev = nullptr;
     s.image_cap = 0;
 }
 
@@ -242,6 +244,12 @@ int slot_alloc_images(Slot& s, size_t pixels) {
     // touch the staging now: the first host access of each page of a fresh pinned allocation
     // faults it into the process, which would otherwise land on the first batches' copy-out
     memset(s.h_images, 0, pixels * 4);
+    void* d = nullptr;
+    if (hipHostGetDevicePointer(&d, s.h_images, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        d = nullptr;  // not mapped: the images come back by DMA from d_images
+    }
+    s.h_images_dev = (float*)d;
     s.image_cap = pixels;
     return BIRP_OK;
 }
@@ -283,8 +291,9 @@ int launch_slot(birp_server* S, Slot& s) {
         Request* r = s.reqs[k];
         birp_tuning t;
         const int g = request_group(r, &t);
-        // a result in a birp_host_alloc buffer is written there by the kernel (zero-copy)
-        float* out = r->dev_out ? r->dev_out : s.d_images + s.offsets[k];
+        // a result in a birp_host_alloc buffer is written there by the kernel (zero-copy), the others
+        // into the slot's pinned staging (or its device images when the staging is not mapped)
+        float* out = r->dev_out ? r->dev_out : (s.h_images_dev ? s.h_images_dev : s.d_images) + s.offsets[k];
         rc = fast_pose(r->vol, &r->req.view, r->req.volume, out, t.tile_w, t.tile_h, s.h_poses[fill[g]++]);
         if (rc) return rc;
     }
@@ -314,10 +323,10 @@ int launch_slot(birp_server* S, Slot& s) {
                                t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll, grid, s.stream);
         if (rc) return rc;
     }
-    // the other results: one DMA to the slot's staging, from where the completion thread copies
-    // them out
+    // without a mapped staging, the other results come back to it in one DMA; the completion thread
+    // copies them out
     size_t lo = total, hi = 0;
-    for (size_t k = 0; k < s.reqs.size(); k++) {
+    for (size_t k = 0; k < s.reqs.size() && !s.h_images_dev; k++) {
         if (s.reqs[k]->dev_out) continue;
         lo = std::min(lo, s.offsets[k]);
         hi = std::max(hi, s.offsets[k] + s.reqs[k]->pixels);
