@@ -1,9 +1,10 @@
 // server.cpp -- continuous-batching render server (include/birp.h "serving").
 //
 // One dispatcher thread owns `max_inflight` launch slots, each with its own HIP stream (default
-// priority), descriptor buffers, a device image buffer and pinned result staging of the same size.
-// The memory budget bounds device memory: the resident volumes plus the image buffers every slot
-// holds (they are kept between batches). Requests queue on submission; whenever a slot is free the
+// priority), descriptor buffers and an image buffer: pinned staging that the kernels write through
+// its device address (plus device images where the device does not map it). The memory budget
+// bounds the resident volumes plus the image buffers every slot holds (they are kept between
+// batches). Requests queue on submission; whenever a slot is free the
 // dispatcher admits queued requests in arrival order while the slot's image buffer can hold them,
 // or can grow to hold them within the budget (releasing idle slots' buffers when that makes the
 // room), and launches them as one persistent-kernel batch (one launch per storage type and lanes
@@ -230,26 +231,30 @@ void slot_release_images(Slot& s) {
 }
 
 // Reallocates slot s's image buffers to exactly `pixels` when they hold fewer (hipFree
-// synchronises the device, so growth is made rare by the headroom chosen at admission).
+// synchronises the device, so growth is made rare by the headroom chosen at admission): the
+// pinned staging, written by the kernels through its device address, plus device images only
+// when the device does not map the staging.
 int slot_alloc_images(Slot& s, size_t pixels) {
     if (s.image_cap >= pixels) return BIRP_OK;
     slot_release_images(s);
-    hipError_t e = hipMalloc(&s.d_images, pixels * 4);
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc(server images)");
-    e = hipHostMalloc(&s.h_images, pixels * 4, hipHostMallocDefault);
-    if (e != hipSuccess) {
-        slot_release_images(s);
-        return hip_fail(e, "hipHostMalloc(server images)");
-    }
+    hipError_t e = hipHostMalloc(&s.h_images, pixels * 4, hipHostMallocDefault);
+    if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(server images)");
     // touch the staging now: the first host access of each page of a fresh pinned allocation
     // faults it into the process, which would otherwise land on the first batches' copy-out
     memset(s.h_images, 0, pixels * 4);
     void* d = nullptr;
     if (hipHostGetDevicePointer(&d, s.h_images, 0) != hipSuccess) {
         (void)hipGetLastError();
-        d = nullptr;  // not mapped: the images come back by DMA from d_images
+        d = nullptr;
     }
     s.h_images_dev = (float*)d;
+    if (!s.h_images_dev) {  // not mapped: the kernels write device images, which come back by DMA
+        e = hipMalloc(&s.d_images, pixels * 4);
+        if (e != hipSuccess) {
+            slot_release_images(s);
+            return hip_fail(e, "hipMalloc(server images)");
+        }
+    }
     s.image_cap = pixels;
     return BIRP_OK;
 }

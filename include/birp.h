@@ -218,11 +218,12 @@ int birp_autotune_lookup(int dtype, int width, int height, int samples, birp_tun
 
 /* ------------------------------------------------------------------ serving */
 typedef struct birp_server_config {
-    size_t memory_budget;  /* device bytes for the resident volumes plus the image buffers the
-                              slots hold (4 bytes per pixel, kept between batches; a slot grows
-                              its buffer within the budget, releasing idle slots' buffers when a
-                              request needs the room). Pinned host staging of the same size per
-                              slot is host memory, outside the budget. */
+    size_t memory_budget;  /* bytes for the resident volumes (device memory) plus the image
+                              buffers the slots hold (4 bytes per pixel, kept between batches:
+                              pinned host staging that the kernels write through the device
+                              mapping, plus device images where the device does not map it; a
+                              slot grows its buffer within the budget, releasing idle slots'
+                              buffers when a request needs the room). */
     int max_batch;         /* most requests one launch may carry */
     int max_inflight;      /* launches in flight at once (each on its own stream) */
     double batch_window_ms;/* wait this long for more arrivals before launching a non-full batch */
