@@ -58,6 +58,7 @@ def test_null_arguments_are_rejected_before_any_device_work():
     assert L.birp_server_submit(None, None, None, None) == inv
     assert L.birp_server_wait(None, 1, 0.0, None) == inv
     assert L.birp_device_synchronize(None) == inv
+    assert L.birp_batch_set_output(None, None) == inv
     p = C.c_void_p()
     assert L.birp_host_alloc(0, C.byref(p)) == inv and not p.value
     assert L.birp_host_free(None) == capi.OK
