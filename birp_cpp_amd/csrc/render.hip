@@ -290,19 +290,22 @@ __device__ __forceinline__ float sample_linear_nearest(const DevVolume& v, const
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ float cmem_sample(const CmemParams& m, float x, float y, float z) {
     float r = sqrtf(x * x + y * y + z * z);
+    const float ryz = sqrtf(y * y + z * z);
     float theta = (x != 0) ? acosf(x / r) : 0.f;
-    float phi = (y != 0) ? acosf(y / sqrtf(y * y + z * z)) : 0.f;
+    // phi = acos(y / ryz) (0 when y == 0) enters only through its cosine and sine, and theta's
+    // cosine and sine are needed too: all four from the ratios that are their arc cosines' arguments
+    // (theta, phi in [0, pi], so both sines are >= 0; x == 0 and y == 0 give the reference's angle
+    // 0), within an ulp or two of cos(acos(u)) and sin(acos(u)), with no trigonometric call
+    const float ct = (x != 0) ? x / r : 1.f, st = (x != 0) ? ryz / r : 0.f;
+    const float cos_phi = (y != 0) ? y / ryz : 1.f, sin_phi = (y != 0) ? fabsf(z) / ryz : 0.f;
     // LinScaled (Cmem.cpp:188-222)
     // cos(phi - pi/2) and cos(phi - 3 pi/2) in double (Cmem.cpp:205, 209) are sin(phi) and
-    // -sin(phi): one double sine serves both (they differ from the reference's by ~1e-16
-    // absolute, far below the float rounding of phi_n and phi_s)
-    float ct = cosf(theta), st = sinf(theta);
-    const double sphi = sin((double)phi);
+    // -sin(phi), formed here in double from the coordinates
+    const double sphi = (y != 0) ? (double)fabsf(z) / sqrt((double)y * y + (double)z * z) : 0.0;
     float phi_n = (float)acos((double)(ct * m.cos_theta_n) + (double)(st * m.sin_theta_n) * sphi);
     float phi_s = (float)acos((double)(ct * m.cos_theta_s) + (double)(st * m.sin_theta_s) * -sphi);
-    float sin_phi = sinf(phi);
     float f = powf(cosf(theta / 2) + m.a5 * sinf(2 * theta) * (1 - expf(-theta)),
-                   (m.p1 * (m.beta_c[0] + m.beta_c[1] * cosf(phi) + m.beta_c[3] * (sin_phi * sin_phi))));
+                   (m.p1 * (m.beta_c[0] + m.beta_c[1] * cos_phi + m.beta_c[3] * (sin_phi * sin_phi))));
     float q = m.p2 * m.charlie * expf(m.p3 * m.dn * (powf(phi_n, m.a21))) +
               m.p2 * m.charlie * expf(m.p3 * m.ds * (powf(phi_s, m.a21)));
     float radius_mp = m.p0 * m.r0_lin * f + q;
@@ -312,7 +315,7 @@ __device__ __forceinline__ float cmem_sample(const CmemParams& m, float x, float
     float base = 2 / (1 + ct);
     float ry = m.bs * powf(base, m.ay_bs);
     float rz = (m.az_bs == m.ay_bs) ? ry : m.bs * powf(base, m.az_bs);  // equal unless set apart
-    float rzc = rz * cosf(phi), rys = ry * sin_phi;
+    float rzc = rz * cos_phi, rys = ry * sin_phi;
     float radius_bs = (ry * rz) / sqrtf((rzc * rzc) + (rys * rys));
     // the reference's zone tests in its order (Cmem.cpp:136-176); the sheath emissivity is
     // evaluated only for the three zones that use it
