@@ -288,6 +288,16 @@ __device__ __forceinline__ float sample_linear_nearest(const DevVolume& v, const
 // CMEM::GetSample (Cmem.cpp:117-288) on the device: the reference's float/double types and
 // order; the transcendental functions are ROCm's device libm (within ~1 ulp of glibc's).
 // ------------------------------------------------------------------------------------------
+// (float)acos(c) for a double c, to float accuracy without an FP64 arc cosine: acos(c) =
+// 2 asin(sqrt((1 - c) / 2)) for c >= 0 and pi - 2 asin(sqrt((1 + c) / 2)) below, with 1 -/+ c formed
+// in double (exact near the ends, where acos is steep) and the rest in float; within a few float
+// ulps of the double arc cosine rounded to float. |c| > 1 gives NaN (a negative square root), as
+// acos does.
+__device__ __forceinline__ float acos_to_float(double c) {
+    if (c >= 0.0) return 2.0f * asinf(sqrtf((float)((1.0 - c) * 0.5)));
+    return 3.14159265358979323846f - 2.0f * asinf(sqrtf((float)((1.0 + c) * 0.5)));
+}
+
 __device__ __forceinline__ float cmem_sample(const CmemParams& m, float x, float y, float z) {
     float r = sqrtf(x * x + y * y + z * z);
     const float ryz = sqrtf(y * y + z * z);
@@ -302,8 +312,8 @@ __device__ __forceinline__ float cmem_sample(const CmemParams& m, float x, float
     // cos(phi - pi/2) and cos(phi - 3 pi/2) in double (Cmem.cpp:205, 209) are sin(phi) and
     // -sin(phi), formed here in double from the coordinates
     const double sphi = (y != 0) ? (double)fabsf(z) / sqrt((double)y * y + (double)z * z) : 0.0;
-    float phi_n = (float)acos((double)(ct * m.cos_theta_n) + (double)(st * m.sin_theta_n) * sphi);
-    float phi_s = (float)acos((double)(ct * m.cos_theta_s) + (double)(st * m.sin_theta_s) * -sphi);
+    float phi_n = acos_to_float((double)(ct * m.cos_theta_n) + (double)(st * m.sin_theta_n) * sphi);
+    float phi_s = acos_to_float((double)(ct * m.cos_theta_s) + (double)(st * m.sin_theta_s) * -sphi);
     float f = powf(cosf(theta / 2) + m.a5 * sinf(2 * theta) * (1 - expf(-theta)),
                    (m.p1 * (m.beta_c[0] + m.beta_c[1] * cos_phi + m.beta_c[3] * (sin_phi * sin_phi))));
     float q = m.p2 * m.charlie * expf(m.p3 * m.dn * (powf(phi_n, m.a21))) +
