@@ -427,3 +427,31 @@ def test_server_results_into_pinned_buffers(device, cube1):
             assert np.array_equal(img.view(np.uint32), want[k].view(np.uint32)), k
     finally:
         srv.destroy()
+
+
+def test_server_destroy_with_requests_in_flight(device, cube1):
+    """birp_server_destroy with requests still queued and launched (none waited for) finishes
+    their work, writes every result buffer, then frees the server; nothing hangs."""
+    vals, org, spc, vols, oc = cube1
+    views = W.orbit_views_mixed(count=40, seed=13)
+    ref = capi.Server(device, memory_budget=4 << 30, max_batch=8, max_inflight=2)
+    try:
+        i = ref.add_volume(vols[capi.DTYPE_F16])
+        want = []
+        for v in views:
+            st, img, _ = ref.wait(ref.submit(i, v), 30000.0)
+            assert st == capi.OK
+            want.append(img)
+    finally:
+        ref.destroy()
+    srv = capi.Server(device, memory_budget=4 << 30, max_batch=8, max_inflight=2)
+    idx = srv.add_volume(vols[capi.DTYPE_F16])
+    outs = [capi.pinned_empty((v.height, v.width)) if k % 2 else np.full((v.height, v.width), np.nan, np.float32)
+            for k, v in enumerate(views)]
+    for o in outs:
+        o.fill(np.nan)
+    for v, o in zip(views, outs):
+        srv.submit(idx, v, out=o)
+    srv.destroy()
+    for k, o in enumerate(outs):
+        assert np.array_equal(o.view(np.uint32), want[k].view(np.uint32)), k
