@@ -425,6 +425,17 @@ def test_server_results_into_pinned_buffers(device, cube1):
             st, img, _ = srv.wait(rid, 30000.0)
             assert st == capi.OK and img is outs[k]
             assert np.array_equal(img.view(np.uint32), want[k].view(np.uint32)), k
+        # results at offsets inside one shared pinned arena (device address = base + offset)
+        offs = np.cumsum([0] + [v.width * v.height for v in views])
+        arena = capi.pinned_empty((int(offs[-1]) + 7,))
+        arena.fill(np.nan)
+        ids = [srv.submit(idx, v, out=arena[7 + offs[k]:7 + offs[k + 1]].reshape(v.height, v.width))
+               for k, v in enumerate(views)]
+        for k, rid in enumerate(ids):
+            st, img, _ = srv.wait(rid, 30000.0)
+            assert st == capi.OK
+            assert np.array_equal(img.view(np.uint32), want[k].view(np.uint32)), k
+        assert np.isnan(arena[:7]).all()  # nothing written before the first result
     finally:
         srv.destroy()
 
