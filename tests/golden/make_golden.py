@@ -1,6 +1,6 @@
 """Write the golden fixtures of tests/golden/ from the reference's own code.
 
-Runs only where the reference sources exist (this build's development container): it builds
+Runs only on a machine that holds the reference's sources under /root/reference: it builds
 oracle/_ref/libbirp_ref.so from the reference's src/DataCube.cpp, src/Cmem.cpp and src/Helper.cpp
 (oracle/Makefile) and records inputs and the reference's outputs as JSON:
 
