@@ -892,7 +892,9 @@ int batch_setup(birp_batch* b) {
         float* out = (b->host_out_dev ? b->host_out_dev : b->d_images) + b->offsets[k];
         int rc = fast_pose(b->vol, &b->views[k], 0, out, t.tile_w, t.tile_h, poses[k]);
         if (rc) return rc;
-        prefix[k + 1] = prefix[k] + tiles_of(b->views[k], t.tile_w, t.tile_h);
+        const long long next = (long long)prefix[k] + tiles_of(b->views[k], t.tile_w, t.tile_h);
+        if (next > INT32_MAX) return fail(BIRP_ERR_UNSUPPORTED, "batch of more than 2^31 - 1 tiles: split it");
+        prefix[k + 1] = (int)next;
     }
     b->total_tiles = prefix[n];
     hipError_t e;

@@ -309,7 +309,9 @@ int launch_slot(birp_server* S, Slot& s) {
             birp_view tmp{};
             tmp.width = s.h_poses[k].W;
             tmp.height = s.h_poses[k].H;
-            pre[k - base[g] + 1] = pre[k - base[g]] + tiles_of(tmp, gt[g].tile_w, gt[g].tile_h);
+            const long long next = (long long)pre[k - base[g]] + tiles_of(tmp, gt[g].tile_w, gt[g].tile_h);
+            if (next > INT32_MAX) return fail(BIRP_ERR_UNSUPPORTED, "launch of more than 2^31 - 1 tiles");
+            pre[k - base[g] + 1] = (int)next;
         }
     }
     const int n = base[kGroups];
