@@ -936,13 +936,15 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
             // FP64 ray (the reference's formulas, in double)
             double sx = (2 * ((i + 0.5) / P.W) - 1) * P.tanx_aspect;
             double sy = (1 - 2 * ((j + 0.5) / P.H)) * P.tany;
-            double nd = sqrt(sx * sx + sy * sy + 1.0);
-            double u0 = sx / nd, u1 = sy / nd, u2 = -1.0 / nd;
+            // (divisions by a norm or a spacing as products with its FP64 reciprocal: a relative
+            // change of ~1e-16, far below the FP32 march)
+            const double ind = 1.0 / sqrt(sx * sx + sy * sy + 1.0);
+            double u0 = sx * ind, u1 = sy * ind, u2 = -ind;
             double w0 = P.rinv[0] * u0 + P.rinv[1] * u1 + P.rinv[2] * u2;
             double w1 = P.rinv[3] * u0 + P.rinv[4] * u1 + P.rinv[5] * u2;
             double w2 = P.rinv[6] * u0 + P.rinv[7] * u1 + P.rinv[8] * u2;
-            double wd = sqrt(w0 * w0 + w1 * w1 + w2 * w2);
-            double dx = w0 / wd, dy = w1 / wd, dz = w2 / wd;
+            const double iwd = 1.0 / sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+            double dx = w0 * iwd, dy = w1 * iwd, dz = w2 * iwd;
             // FP64 cube entry/exit: inside iff 0 <= (x - o)/s < n - 1 per axis
             double tin = -1e300, tout = 1e300;
             slab_axis(P.px, dx, (double)V.ox, (double)V.ox + (double)(V.nx - 1) * V.sx, tin, tout);
@@ -961,12 +963,12 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
             if (P.semantics == BIRP_SEM_REFERENCE && kf != 0) kl = -1;
             if (kf <= kl) {
                 double t0 = P.L * ((kf + 0.5) / P.n), dt = P.L / P.n;
-                ix0 = (float)((P.px + dx * t0 - V.ox) / V.sx);
-                iy0 = (float)((P.py + dy * t0 - V.oy) / V.sy);
-                iz0 = (float)((P.pz + dz * t0 - V.oz) / V.sz);
-                dix = (float)(dx * dt / V.sx);
-                diy = (float)(dy * dt / V.sy);
-                diz = (float)(dz * dt / V.sz);
+                ix0 = (float)((P.px + dx * t0 - V.ox) * V.rdx);
+                iy0 = (float)((P.py + dy * t0 - V.oy) * V.rdy);
+                iz0 = (float)((P.pz + dz * t0 - V.oz) * V.rdz);
+                dix = (float)(dx * dt * V.rdx);
+                diy = (float)(dy * dt * V.rdy);
+                diz = (float)(dz * dt * V.rdz);
             }
         }
         const int cnt = (kl >= kf) ? (kl - kf + 1) : 0;
