@@ -93,6 +93,8 @@ struct FastPose {
     double px, py, pz;
     double tanx_aspect, tany;  // tan(fov_w/2)*aspect, tan(fov_h/2), in double
     double L;
+    double inv_w, inv_h;  // 1/W, 1/H
+    double sc, dt;        // n/L (samples per unit length), L/n (the step)
     int W, H, n;
     int semantics;
     int vol;          // index into the volume table

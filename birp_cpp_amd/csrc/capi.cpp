@@ -121,6 +121,10 @@ int fast_pose(const birp_volume* vol, const birp_view* v, int vol_index, float* 
     fp.tanx_aspect = std::tan(to_rad * (v->fov_w / 2.0)) * ((double)v->width / v->height);
     fp.tany = std::tan(to_rad * (v->fov_h / 2.0));
     fp.L = v->ray_length;
+    fp.inv_w = 1.0 / v->width;
+    fp.inv_h = 1.0 / v->height;
+    fp.sc = (double)v->samples / v->ray_length;
+    fp.dt = (double)v->ray_length / v->samples;
     fp.W = v->width;
     fp.H = v->height;
     fp.n = v->samples;

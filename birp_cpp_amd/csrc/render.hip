@@ -934,8 +934,8 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
         float ix0 = 0.f, iy0 = 0.f, iz0 = 0.f, dix = 0.f, diy = 0.f, diz = 0.f;
         if (valid) {
             // FP64 ray (the reference's formulas, in double)
-            double sx = (2 * ((i + 0.5) / P.W) - 1) * P.tanx_aspect;
-            double sy = (1 - 2 * ((j + 0.5) / P.H)) * P.tany;
+            double sx = (2 * ((i + 0.5) * P.inv_w) - 1) * P.tanx_aspect;
+            double sy = (1 - 2 * ((j + 0.5) * P.inv_h)) * P.tany;
             // (divisions by a norm or a spacing as products with its FP64 reciprocal: a relative
             // change of ~1e-16, far below the FP32 march)
             const double ind = 1.0 / sqrt(sx * sx + sy * sy + 1.0);
@@ -951,7 +951,7 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
             slab_axis(P.py, dy, (double)V.oy, (double)V.oy + (double)(V.ny - 1) * V.sy, tin, tout);
             slab_axis(P.pz, dz, (double)V.oz, (double)V.oz + (double)(V.nz - 1) * V.sz, tin, tout);
             if (tin <= tout) {
-                double sc = (double)P.n / P.L;
+                const double sc = P.sc;
                 double a = ceil(tin * sc - 0.5), b = ceil(tout * sc - 0.5) - 1.0;
                 a = fmax(a, 0.0);
                 b = fmin(b, (double)(P.n - 1));
@@ -962,7 +962,7 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
             }
             if (P.semantics == BIRP_SEM_REFERENCE && kf != 0) kl = -1;
             if (kf <= kl) {
-                double t0 = P.L * ((kf + 0.5) / P.n), dt = P.L / P.n;
+                const double dt = P.dt, t0 = (kf + 0.5) * dt;
                 ix0 = (float)((P.px + dx * t0 - V.ox) * V.rdx);
                 iy0 = (float)((P.py + dy * t0 - V.oy) * V.rdy);
                 iz0 = (float)((P.pz + dz * t0 - V.oz) * V.rdz);
