@@ -261,7 +261,8 @@ int birp_server_submit(birp_server* s, const birp_request* req, float* result, u
 int birp_server_wait(birp_server* s, uint64_t id, double timeout_ms, birp_result_info* info);
 /* Steady-clock seconds as used in birp_result_info. */
 double birp_clock_s(void);
-/* Pinned (page-locked) host memory for response buffers, mapped into the device's address space.
+/* Pinned (page-locked) host memory for response buffers, mapped into the device's address space
+ * (no reference counterpart: the reference keeps its image in a std::vector, Camera.h:53).
  * A served request whose result buffer lies inside one of these allocations has its image
  * written there by the render kernel itself (no device image, DMA or host copy), and
  * birp_render reads and writes such an image in place the same way. Free only after every
