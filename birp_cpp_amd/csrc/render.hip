@@ -983,9 +983,26 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
             fit_march(iz0, diz, __uint_as_float(__float_as_uint(V.fnz - 1.0f) - 1u), cnt - 1);
         }
         // the U samples of an iteration (neighbours on the ray, of similar size) are added plainly,
-        // and their partial sum enters the running sum by one Kahan step
+        // and their partial sum enters the running sum by one Kahan step. The trips that every lane
+        // of the wave fills completely need no clamp and no mask; the rest of the trips run below
+        // with both (the same samples in the same order and grouping either way).
         float sum = 0.f, nc = 0.f;
-        for (int q = 0; q < trips; q += U) {
+        const int tfull = wave_min_i(my) / U * U;
+        int q = 0;
+        for (; q < tfull; q += U) {
+            float part = 0.f;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const float fj = (float)((q + u) * S + sub);
+                const float xd = __builtin_fmaf(fj, dix, ix0);
+                const float yd = __builtin_fmaf(fj, diy, iy0);
+                const float zd = __builtin_fmaf(fj, diz, iz0);
+                const float x = fast_sample<DT>(V, rs, tab, xd, yd, zd);
+                part = (u == 0) ? x : part + x;
+            }
+            kahan_add(part, sum, nc);
+        }
+        for (; q < trips; q += U) {
             float part = 0.f;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
