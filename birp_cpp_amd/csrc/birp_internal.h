@@ -84,7 +84,9 @@ struct ExactView {
     int fast_guards;  // positive cube whose scaled samples stay normal floats: only the
                       // scale's midpoint guard is needed
     float* out;       // device image H*W
-    unsigned long long* counter;  // optional: samples added
+    unsigned long long* counter;  // optional: samples added (one atomic per wave)
+    unsigned* wave_counts;  // optional, instead of counter: each wave stores its samples added at
+                            // [block * 4 + wave] (exact_wave_slots(W, H, lanes) slots)
 };
 
 // Per-pose constants of the fast path: FP64 basis and origin, FP32 march.
@@ -117,6 +119,8 @@ struct F64View {
 // Launch wrappers implemented in render.hip (host side of the HIP translation unit).
 namespace birp {
 int launch_exact(const DevVolume& v, const ExactView& ev, int lanes_per_ray, int pipe, void* stream);
+// Slots of ExactView::wave_counts for a W x H image at `lanes_per_ray` (4 waves per block).
+int exact_wave_slots(int W, int H, int lanes_per_ray);
 int launch_f64(const DevVolume& v, const F64View& fv, void* stream);
 int launch_sample_points(const DevVolume& v, const float* xyz, long long n, float* out, void* stream);
 int launch_fast_batch(const DevVolume* vols_dev, int dtype, const FastPose* poses_dev, const int* tile_prefix_dev,

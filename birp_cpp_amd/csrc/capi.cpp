@@ -100,6 +100,7 @@ int exact_view(const birp_volume* vol, const birp_view* v, float* out_dev, Exact
     ev.fast_guards = (d.vclass == VC_POSITIVE && (double)d.vmin * ev.w * kf >= 0x1p-124 && (double)d.vmax * ev.w * kf <= 0x1p126) ? 1 : 0;
     ev.out = out_dev;
     ev.counter = nullptr;
+    ev.wave_counts = nullptr;
     return BIRP_OK;
 }
 
@@ -612,13 +613,17 @@ static int render_buffer(birp_device* dev, size_t npix, float** img, unsigned lo
     return BIRP_OK;
 }
 
-static int render_exact_into(birp_volume* vol, const birp_view* view, float* dimg, unsigned long long* counter) {
+// `counter` (a device u64 that each wave adds its in-cube samples to) or `wave_counts` (one slot
+// per wave, exact_wave_slots of the image, each stored by its wave), or neither.
+static int render_exact_into(birp_volume* vol, const birp_view* view, float* dimg, unsigned long long* counter,
+                             unsigned* wave_counts = nullptr) {
     birp_device* dev = vol->dev;
     if (vol->dv.kind == SK_CUBE && !vol->dv.linear) return fail(BIRP_ERR_UNSUPPORTED, "exact renders need an FP32 volume");
     ExactView ev;
     int rc = exact_view(vol, view, dimg, ev);
     if (rc) return rc;
-    ev.counter = counter;
+    ev.counter = wave_counts ? nullptr : counter;
+    ev.wave_counts = wave_counts;
     if (take_injected_failure(dev)) return fail(BIRP_ERR_INJECTED, "injected failure");
     DevVolume dv = vol->dv;
     if (exact_force_linear() && dv.vclass != VC_NONFINITE && dv.kind == SK_CUBE) dv.vclass = VC_NONFINITE;
@@ -640,17 +645,26 @@ extern "C" int birp_render(birp_volume* vol, const birp_view* view, float* image
     if ((rc = render_buffer(dev, npix, &dimg, &cnt))) return rc;
     // a birp_host_alloc image is read and written by the kernel in place, through its device
     // address; any other image goes through the device's pinned staging, which the kernel also
-    // writes in place when the device maps it (then only the sample counter comes back by DMA)
+    // writes in place when the device maps it. With a mapped staging an exact render's waves also
+    // store their in-cube sample counts there (after the image's place), and nothing comes back by
+    // DMA: the host sums the counts after the stream synchronises.
+    const bool exact = view->math == BIRP_MATH_EXACT;
     float* mapped = pinned_device_ptr(image, npix * 4);
+    const size_t counts_at = mapped ? 256 : 256 + ((npix * 4 + 255) & ~(size_t)255);
+    const int nslots = exact ? exact_wave_slots(view->width, view->height, default_exact_lanes(vol->dv.kind == SK_CMEM)) : 0;
+    if ((rc = pinned_staging(dev, counts_at + (size_t)nslots * 4))) return rc;
     float* staged = nullptr;  // the image's place in the pinned staging, when the kernel writes it
+    unsigned* counts_dev = nullptr;
+    const unsigned* counts_host = nullptr;
     if (mapped) {
         dimg = mapped;
-    } else {
-        if ((rc = pinned_staging(dev, 256 + npix * 4))) return rc;
-        if (dev->pinned_dev) {
-            dimg = (float*)((char*)dev->pinned_dev + 256);
-            staged = (float*)((char*)dev->pinned + 256);
-        }
+    } else if (dev->pinned_dev) {
+        dimg = (float*)((char*)dev->pinned_dev + 256);
+        staged = (float*)((char*)dev->pinned + 256);
+    }
+    if (exact && dev->pinned_dev) {
+        counts_dev = (unsigned*)((char*)dev->pinned_dev + counts_at);
+        counts_host = (const unsigned*)((const char*)dev->pinned + counts_at);
     }
     hipError_t e;
     birp_view vv = *view;
@@ -666,7 +680,7 @@ extern "C" int birp_render(birp_volume* vol, const birp_view* view, float* image
     }
     view = &vv;
     (void)hipEventRecord(dev->ev0, dev->stream);
-    if (view->math == BIRP_MATH_EXACT) rc = render_exact_into(vol, view, dimg, cnt);
+    if (exact) rc = render_exact_into(vol, view, dimg, cnt, counts_dev);
     else rc = render_fast_into(vol, view, dimg);
     if (rc) {
         (void)hipStreamSynchronize(dev->stream);
@@ -674,11 +688,13 @@ extern "C" int birp_render(birp_volume* vol, const birp_view* view, float* image
         return rc;
     }
     (void)hipEventRecord(dev->ev1, dev->stream);
+    bool counter_back = true;  // the device counter's value is at the staging's offset 0
     if (mapped || staged || pinned_range(image, npix * 4)) {
         // the image is in place (in the caller's pinned buffer or in the staging), or goes by one DMA
-        // straight into a pinned caller buffer; the counter comes back through staging
-        if ((rc = pinned_staging(dev, 256))) return rc;
-        if ((e = hipMemcpyAsync(dev->pinned, dev->rbuf, 8, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess ||
+        // straight into a pinned caller buffer; an exact render's counter, unless its waves stored
+        // their counts in the staging, comes back by an 8-byte DMA
+        counter_back = exact && !counts_host;
+        if ((counter_back && (e = hipMemcpyAsync(dev->pinned, dev->rbuf, 8, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess) ||
             (!mapped && !staged && (e = hipMemcpyAsync(image, dimg, npix * 4, hipMemcpyDeviceToHost, dev->stream)) != hipSuccess) ||
             (e = hipStreamSynchronize(dev->stream)) != hipSuccess) {
             dev->count_known = false;
@@ -695,9 +711,14 @@ extern "C" int birp_render(birp_volume* vol, const birp_view* view, float* image
         }
         memcpy(image, (char*)dev->pinned + 256, npix * 4);
     }
-    const unsigned long long count = *(const unsigned long long*)dev->pinned;
-    const unsigned long long added = count - dev->count_base;
-    dev->count_base = count;
+    unsigned long long added = 0;
+    if (counts_host) {
+        for (int k = 0; k < nslots; k++) added += counts_host[k];
+    } else if (counter_back) {
+        const unsigned long long count = *(const unsigned long long*)dev->pinned;
+        added = count - dev->count_base;
+        dev->count_base = count;
+    }
     if (stats) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, dev->ev0, dev->ev1);

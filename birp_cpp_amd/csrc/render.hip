@@ -545,9 +545,12 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact(DevVolume v, ExactV
     }
     if constexpr (POS) added = (unsigned)len;
     if (valid && sub == 0) ev.out[(size_t)j * ev.W + i] = acc;
-    if (ev.counter) {
+    if (ev.counter || ev.wave_counts) {
         unsigned tot = wave_sum_u((valid && sub == 0) ? added : 0u);
-        if (lane == 0 && tot) atomicAdd(ev.counter, (unsigned long long)tot);
+        if (lane == 0) {
+            if (ev.wave_counts) ev.wave_counts[lb * 4 + wave] = tot;  // a plain store, every wave
+            else if (tot) atomicAdd(ev.counter, (unsigned long long)tot);
+        }
     }
 }
 
@@ -1070,6 +1073,17 @@ static int launch_exact_s(const DevVolume& v, const ExactView& ev, int pipe, hip
         default: hipLaunchKernelGGL((k_render_exact<S, SM_CMEM>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks); break;
     }
     return hip_status(hipGetLastError());
+}
+
+template <int S>
+static int exact_blocks(int W, int H) {
+    constexpr int TW = ExactTile<S>::TW, TH = ExactTile<S>::TH;
+    return ((W + 2 * TW - 1) / (2 * TW)) * ((H + 2 * TH - 1) / (2 * TH));
+}
+
+int exact_wave_slots(int W, int H, int lanes_per_ray) {
+    const int b = lanes_per_ray == 4 ? exact_blocks<4>(W, H) : (lanes_per_ray == 1 ? exact_blocks<1>(W, H) : exact_blocks<2>(W, H));
+    return 4 * b;
 }
 
 int launch_exact(const DevVolume& v, const ExactView& ev, int lanes_per_ray, int pipe, void* stream) {

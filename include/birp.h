@@ -115,7 +115,8 @@ typedef struct birp_view {
 } birp_view;
 
 typedef struct birp_render_stats {
-    long long samples_in_cube; /* samples added into the image (cube renders), or n per ray (CMEM) */
+    long long samples_in_cube; /* exact renders: samples added into the image (cube), or n per ray (CMEM);
+                                  fast renders: -1 (not counted) */
     long long samples_nominal; /* W * H * n */
     double kernel_ms;          /* device time of the render kernel(s), from HIP events */
 } birp_render_stats;

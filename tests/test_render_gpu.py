@@ -277,6 +277,33 @@ def test_render_into_pinned_image(device, shell128, oracle):
     assert_bitwise(pin, want)
 
 
+def test_sample_counts_on_every_image_path(device, shell128, oracle):
+    """In-cube sample counts stay the oracle's whichever way the count comes back: per-wave counts in
+    the mapped staging (plain and pinned images), the device counter (birp_count_samples), with
+    fast renders, which count nothing, and image sizes that regrow the staging in between."""
+    vol, oc = shell128
+    small = capi.make_view(40, 24, 26.5, 15.5, (15.0, 5.0, 8.0), (0.0, 0.0, 0.0), 44.0, 300)
+    big = W.view_144x84()
+    want = {}
+    for name, v in (("small", small), ("big", big)):
+        want[name] = oracle.render_f32(oc, oview(oracle, v), oracle.SEM_EXTENDED, nthreads=NT)
+    fast = W.view_144x84()
+    fast.math = capi.MATH_FAST
+    pin = capi.pinned_empty((big.height, big.width))
+    for name, v in (("small", small), ("big", big), ("small", small), ("big", big)):
+        ref, cnt = want[name]
+        img, st = vol.render(v)
+        assert st.samples_in_cube == cnt, name
+        assert_bitwise(img, ref)
+        _, fst = vol.render(fast)
+        assert fst.samples_in_cube == -1
+        assert vol.count_samples(v) == cnt, name
+        if name == "big":
+            img, st = vol.render(v, out=pin)
+            assert st.samples_in_cube == cnt
+            assert_bitwise(pin, ref)
+
+
 def test_render_async_into_device_image(device, shell128, oracle):
     vol, oc = shell128
     v = W.view_144x84()
