@@ -658,6 +658,10 @@ extern "C" int birp_server_submit(birp_server* S, const birp_request* req, float
     r->submit_s = birp_clock_s();
     {
         std::lock_guard<std::mutex> lk(S->mu);
+        if (S->stop) {  // destroy has begun: the dispatcher may already have exited
+            delete r;
+            return fail(BIRP_ERR_INVALID_ARGUMENT, "server is being destroyed");
+        }
         if (req->volume < 0 || req->volume >= (int)S->vols.size()) {
             delete r;
             return fail(BIRP_ERR_NOT_FOUND, "unknown volume index");
