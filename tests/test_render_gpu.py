@@ -81,6 +81,32 @@ def test_incube_poses_bitwise(shell128, oracle, pose, semantics):
     assert_bitwise(img, ref)
 
 
+def test_random_poses_bitwise(shell128, oracle):
+    """A seeded sweep of 48 random views of the 128^3 cube (x in [-10, 20], y and z in [-15, 15]):
+    cameras inside and outside it, random aim points, image sizes 1..40 px a side, fields of view
+    2..80 degrees, ray lengths 5..90 and 2..1200 samples, both semantics. Every image is
+    bitwise the oracle's and every in-cube sample count equals its count."""
+    vol, oc = shell128
+    rng = np.random.default_rng(20261202)
+    done = 0
+    while done < 48:
+        pos = rng.uniform((-30.0, -35.0, -35.0), (40.0, 35.0, 35.0))
+        aim = rng.uniform((-10.0, -15.0, -15.0), (20.0, 15.0, 15.0))
+        if np.linalg.norm(pos - aim) < 0.5 or not np.isfinite(oracle.orient(tuple(pos), tuple(aim))):
+            continue
+        w, h = (int(x) for x in rng.integers(1, 41, 2))
+        fw, fh = (float(x) for x in rng.uniform(2.0, 80.0, 2))
+        L = float(rng.uniform(5.0, 90.0))
+        n = int(rng.integers(2, 1201))
+        sem = int(done % 2)
+        v = capi.make_view(w, h, fw, fh, tuple(pos), tuple(aim), L, n, sem, capi.MATH_EXACT)
+        img, st = vol.render(v)
+        ref, cnt = oracle.render_f32(oc, oview(oracle, v), sem, nthreads=NT)
+        assert st.samples_in_cube == cnt, (done, pos, aim, w, h, n, sem)
+        assert_bitwise(img, ref)
+        done += 1
+
+
 def test_reference_semantics_outside_camera_is_zero(shell128, oracle):
     vol, oc = shell128
     v = W.view(144, 84, 0.5, semantics=capi.SEM_REFERENCE)  # camera at y=-20 is outside the cube
