@@ -104,6 +104,43 @@ def test_fast_march_on_faces(device):
         v16.free()
 
 
+@pytest.mark.parametrize("interp", ["trilinear", "nearest"])
+def test_random_views_of_signed_cube_bitwise(device, oracle, interp):
+    """A seeded sweep of 24 random views (cameras inside and outside, image sizes 1..32 px a side,
+    2..900 samples, both semantics) of a signed 40^3 cube with 15% zeros, trilinear (the general
+    brick kernel: std::lerp's straddle branch, early stops on negative samples) and nearest: every
+    image is bitwise the oracle's and every in-cube sample count equals its count."""
+    rng = np.random.default_rng(11)
+    n = 40
+    vals = (rng.standard_normal((n, n, n)) * 1e-5 + 2e-6).astype(np.float32)
+    vals[rng.random(vals.shape) < 0.15] = 0.0
+    org, spc = (-10.0, -15.0, -15.0), (30.0 / (n - 1),) * 3
+    nearest = interp == "nearest"
+    vol = capi.Volume.upload(device, vals, org, spc, interp=capi.INTERP_NEAREST if nearest else capi.INTERP_TRILINEAR)
+    try:
+        oc = oracle.OracleCube(vals, org, spc, trilinear=not nearest)
+        done = total = 0
+        while done < 24:
+            inside = rng.random() < 0.7  # else anywhere within 15 units of the cube
+            pos = rng.uniform((-10.0, -15.0, -15.0), (20.0, 15.0, 15.0)) if inside else rng.uniform((-25.0, -30.0, -30.0), (35.0, 30.0, 30.0))
+            aim = rng.uniform((-10.0, -15.0, -15.0), (20.0, 15.0, 15.0))
+            if np.linalg.norm(pos - aim) < 0.5 or not np.isfinite(oracle.orient(tuple(pos), tuple(aim))):
+                continue
+            w, h = (int(x) for x in rng.integers(1, 33, 2))
+            sem = done % 2
+            v = capi.make_view(w, h, float(rng.uniform(5.0, 70.0)), float(rng.uniform(5.0, 70.0)), tuple(pos), tuple(aim),
+                               float(rng.uniform(5.0, 80.0)), int(rng.integers(2, 901)), sem, capi.MATH_EXACT)
+            img, st = vol.render(v)
+            ref, cnt = oracle.render_f32(oc, oview(oracle, v), sem, nthreads=NT)
+            assert st.samples_in_cube == cnt, (done, pos, aim, w, h, sem)
+            assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), (done, pos, aim, w, h, sem)
+            done += 1
+            total += cnt
+        assert total > 20_000  # the sweep reaches the cube (negative samples end rays early)
+    finally:
+        vol.free()
+
+
 def test_sunward_axis_pose_refused_on_both_paths(cfg_small, oracle):
     """A camera on the Sun-Earth line looking away from the Sun: the Sun projects onto the view
     axis, the reference's PointToPlane divides 0 by 0 (Camera.cpp:440-444) and its angle and basis are

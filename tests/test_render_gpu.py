@@ -88,9 +88,10 @@ def test_random_poses_bitwise(shell128, oracle):
     bitwise the oracle's and every in-cube sample count equals its count."""
     vol, oc = shell128
     rng = np.random.default_rng(20261202)
-    done = 0
+    done = total = 0
     while done < 48:
-        pos = rng.uniform((-30.0, -35.0, -35.0), (40.0, 35.0, 35.0))
+        inside = rng.random() < 0.7  # else anywhere within 20 units of the cube
+        pos = rng.uniform((-10.0, -15.0, -15.0), (20.0, 15.0, 15.0)) if inside else rng.uniform((-30.0, -35.0, -35.0), (40.0, 35.0, 35.0))
         aim = rng.uniform((-10.0, -15.0, -15.0), (20.0, 15.0, 15.0))
         if np.linalg.norm(pos - aim) < 0.5 or not np.isfinite(oracle.orient(tuple(pos), tuple(aim))):
             continue
@@ -105,6 +106,8 @@ def test_random_poses_bitwise(shell128, oracle):
         assert st.samples_in_cube == cnt, (done, pos, aim, w, h, n, sem)
         assert_bitwise(img, ref)
         done += 1
+        total += cnt
+    assert total > 100_000
 
 
 def test_reference_semantics_outside_camera_is_zero(shell128, oracle):
