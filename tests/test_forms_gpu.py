@@ -100,6 +100,34 @@ def test_fast_path_vs_fp64(cube1, oracle, dtype, tol):
         assert err <= tol
 
 
+def test_fast_path_random_views(cube1, oracle):
+    """A seeded sweep of 32 random fast views of the 128^3 cube (cameras inside and outside it,
+    16..48 px a side, 200..1200 samples, both semantics): FP32 storage within 1e-5 and F16 within
+    1e-4 relative L2 of the FP64 render, and a view that sees nothing renders all zeros."""
+    vals, org, spc, vols, oc = cube1
+    rng = np.random.default_rng(4242)
+    done = lit = 0
+    while done < 32:
+        inside = rng.random() < 0.7
+        pos = rng.uniform((-10.0, -15.0, -15.0), (20.0, 15.0, 15.0)) if inside else rng.uniform((-30.0, -35.0, -35.0), (40.0, 35.0, 35.0))
+        aim = rng.uniform((-10.0, -15.0, -15.0), (20.0, 15.0, 15.0))
+        if np.linalg.norm(pos - aim) < 0.5 or not np.isfinite(oracle.orient(tuple(pos), tuple(aim))):
+            continue
+        w, h = (int(x) for x in rng.integers(16, 49, 2))
+        v = capi.make_view(w, h, float(rng.uniform(5.0, 70.0)), float(rng.uniform(5.0, 70.0)), tuple(pos), tuple(aim),
+                           float(rng.uniform(10.0, 90.0)), int(rng.integers(200, 1201)), done % 2, capi.MATH_FAST)
+        ref = vols[capi.DTYPE_F32].render_f64(v)
+        for dt, tol in ((capi.DTYPE_F32, 1e-5), (capi.DTYPE_F16, 1e-4)):
+            img, _ = vols[dt].render(v)
+            if not ref.any():
+                assert not img.any(), (done, dt)
+            else:
+                assert rel_l2(img, ref) <= tol, (done, dt, pos, aim, w, h)
+        lit += bool(ref.any())
+        done += 1
+    assert lit >= 16
+
+
 def test_f16_scale_modes(device):
     """F16 storage keeps one power-of-two scale for the whole volume when every nonzero |v| then
     lies in F16's normal range, and a scale per 8^3 brick otherwise. A cube spanning 30 decades
