@@ -16,12 +16,17 @@ Also measured in the same run (keys under "forms"):
                         against the FP64 render
   serving               Poisson requests at 200/s over 8 resident cubes: p50/p99 latency, renders/s;
                         the same requests as one burst (saturated server) and through the plain path
-The CPU baseline is the oracle's FP32 restatement of the reference render (a port: the reference's
-own Camera.cpp cannot be compiled here) on the same 576x336 image, on the host's cores; its image
-also checks the headline render bit for bit ("error": "vs_oracle_fp32"), beside the error against
-the FP64 render of the same sample grid. "cpu_reference" times the reference's own DataCube
-sampler (compiled from its sources into oracle/_ref) in the render loop, single-threaded, on the
-whole image, and checks it against both images bit for bit.
+The roofline is the VALU issue rate: the cycles in which the kernel's SIMDs issued a VALU
+instruction (rocprofv3 PMC counts committed under profiles/) over the kernel time measured here with
+HIP events, against every SIMD issuing every cycle. The HBM bytes the kernel moves (FETCH_SIZE and
+WRITE_SIZE, copy-calibrated) and the algorithmic 8-corner gather bytes are reported beside it: the
+cube stays in L2/MALL, so HBM is not the binding resource.
+The CPU baseline leg is the only part that uses the oracle: its FP32 restatement of the reference
+render (a port: the reference's own Camera.cpp cannot be compiled here) times whole renders of the
+headline view on the host's cores and checks the GPU image bit for bit ("error": "vs_oracle_fp32"),
+and it checks the CMEM image against the oracle's FP32 render with the zone-edge rule of
+tests/parity.py. `--cpu-reference` times the reference's own DataCube sampler (oracle/_ref, which
+is built and run only in the development container) in the render loop, single-threaded.
 """
 from __future__ import annotations
 
@@ -71,20 +76,26 @@ def read_profile_value(name, key):
     return None
 
 
-PEAK_VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9  # non-packed FP32 issue: CUs x SIMDs x lanes x clock
+PEAK_VALU_ISSUE = 256 * 4 * 2.4e9  # SIMD cycles per second: 256 CUs x 4 SIMDs x 2.4 GHz
 
 
-def valu_roofline(kern_ms):
-    """VALU issue rate of the plain kernel: the committed rocprofv3 instruction count per launch
-    (SQ_INSTS_VALU, 64 lanes each) over this run's measured kernel time."""
-    name = "plain_exact_256cube_576x336_valu_pmc.txt"
-    insts = read_profile_value(name, "valu_wave_insts_per_launch")
-    if insts is None:
+def valu_roofline(profile, kern_ms):
+    """VALU issue roofline of one kernel: the cycles in which its SIMDs issued at least one VALU
+    instruction, from a committed rocprofv3 --pmc summary (SQ_ACTIVE_INST_VALU counts one quad-cycle
+    per wave64 VALU instruction, SQ_ACTIVE_INST_VALU2 the quad-cycles in which a SIMD issued two, so
+    their difference is the quad-cycles with any VALU issue), over this run's measured kernel time,
+    against every SIMD issuing every cycle. This fraction cannot exceed 1."""
+    busy = read_profile_value(profile, "valu_issue_quad_cycles_per_launch")
+    insts = read_profile_value(profile, "valu_wave_insts_per_launch")
+    if busy is None or insts is None:
         return None
-    achieved = insts * 64 / (kern_ms * 1e-3)
-    return {"bound": "valu", "achieved": round(achieved / 1e12, 2), "peak": round(PEAK_VALU_LANE_OPS / 1e12, 2),
-            "unit": "T lane-instructions/s", "frac": round(achieved / PEAK_VALU_LANE_OPS, 3),
-            "lane_ops_per_sample": read_profile_value(name, "valu_lane_ops_per_sample"), "source": "profiles/" + name}
+    achieved = busy * 4 / (kern_ms * 1e-3)
+    return {"bound": "valu", "achieved": round(achieved / 1e9, 1), "peak": round(PEAK_VALU_ISSUE / 1e9, 1),
+            "unit": "G SIMD-cycles/s with a VALU issue", "frac": round(achieved / PEAK_VALU_ISSUE, 3),
+            "lane_instructions_per_s": float(f"{insts * 64 / (kern_ms * 1e-3):.4g}"),
+            "dual_issue_frac": read_profile_value(profile, "valu_dual_issue_fraction"),
+            "resident_waves_per_simd": read_profile_value(profile, "resident_waves_per_simd"),
+            "source": "profiles/" + profile}
 
 
 def read_traffic():
@@ -131,9 +142,16 @@ def bench_plain(capi, W, dev, steps, warmup):
                 sync_rps_pinned=sync_rps_pinned, image=host)
 
 
-def cpu_baseline(O, W, capi, vals, org, spc, v, n_in):
+def cpu_baseline_leg(W, plain, cmem_img):
+    """The CPU baseline and the oracle checks of this run's images (the only part of bench.py that
+    uses the oracle): the oracle's FP32 restatement of Camera::Render times whole renders of the
+    headline view on the host's cores and checks the GPU image bit for bit; the CMEM image is checked
+    against the oracle's FP32 render, whole-image and with the pixels near the model's zone edges
+    split off and held to their zone-flip bound (tests/parity.py)."""
+    from oracle import oracle as O
     nthreads = min(16, os.cpu_count() or 1)
-    oc = O.OracleCube(vals, org, spc)
+    v, n_in = plain["view"], plain["n_in"]
+    oc = O.OracleCube(plain["vals"], plain["org"], plain["spc"])
     ov = O.make_view(v.width, v.height, v.fov_w, v.fov_h, list(v.position), list(v.aim), v.ray_length, v.samples)
     # whole renders, repeated until about a second of wall time (10-20 CPU-seconds over the threads)
     reps, el = 0, 0.0
@@ -143,34 +161,61 @@ def cpu_baseline(O, W, capi, vals, org, spc, v, n_in):
         assert cnt == n_in
         reps += 1
         el = time.perf_counter() - t
-    return {"value": round(reps * cnt / el / 1e9, 6), "unit": "Gsamples/s", "cores": nthreads, "kind": "port",
+    base = {"value": round(reps * cnt / el / 1e9, 6), "unit": "Gsamples/s", "cores": nthreads, "kind": "port",
             "sample": f"{reps} full 576x336 renders (n=800, {cnt} in-cube samples each) of the 256^3 cube by the oracle's "
-                      f"FP32 restatement of Camera::Render, {nthreads} threads, {el:.2f} s ({el * nthreads:.0f} CPU-s)"}, ref
+                      f"FP32 restatement of Camera::Render, {nthreads} threads, {el:.2f} s ({el * nthreads:.0f} CPU-s)"}
+    same = np.array_equal(ref.view(np.uint32), plain["image"].view(np.uint32))
+    checks = {"vs_oracle_fp32": "bitwise" if same else float(f"{rel_l2(plain['image'], ref):.4e}")}
+    if cmem_img is not None:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from parity import CMEM_EDGE_BAND
+        m = O.OracleCmem(None)
+        cv = W.view_576x336()
+        ocv = O.make_view(cv.width, cv.height, cv.fov_w, cv.fov_h, list(cv.position), list(cv.aim), cv.ray_length, cv.samples)
+        ref32, _ = O.render_f32(m, ocv, O.SEM_EXTENDED, nthreads=nthreads)
+        mask, bound = m.edge_bound(ocv, CMEM_EDGE_BAND, nthreads=nthreads)
+        img = cmem_img.astype(np.float64)
+        d = np.abs(img - ref32)
+        allow = bound * 1.001 + 1e-5 * np.abs(ref32) + 1e-30
+        checks["cmem"] = {"vs_oracle_fp32_rel_l2": float(f"{rel_l2(img, ref32):.4e}"),
+                          "edge_pixel_fraction": round(float(mask.mean()), 5),
+                          "vs_oracle_fp32_rel_l2_off_edges": float(f"{rel_l2(img[~mask], ref32[~mask]):.4e}"),
+                          "edge_pixels_within_flip_bound": bool(np.isfinite(img).all() and (d[mask] <= allow[mask]).all()),
+                          "pixels_differing": int((d > 1e-5 * np.abs(ref32)).sum())}
+    return base, checks
 
 
-def cpu_reference(O, vals, org, spc, v, ref32, gpu_img, rows=(0, 336)):
-    """The reference's own DataCube::GetSample (oracle/_ref, compiled from its sources) inside the
-    Camera::Render loop, single-threaded as the reference runs, on the whole 576x336 image of the
-    256^3 cube; its image is checked against the oracle's and the GPU's bit for bit."""
+def cpu_reference_container():
+    """The reference's own DataCube::GetSample (oracle/_ref, compiled from its sources in this
+    container only; it never travels) inside the oracle's Camera::Render loop, single-threaded as
+    the reference runs, on the whole 576x336 image of the 256^3 cube; the image is checked against
+    the oracle's FP32 render bit for bit. Prints one JSON line."""
+    from birp_cpp_amd import workloads as W
+    from oracle import oracle as O
     if not os.path.exists(O.REF_PATH):
-        return None
+        print("oracle/_ref/libbirp_ref.so is not built here (it needs the reference sources)", file=sys.stderr)
+        sys.exit(2)
+    vals, org, spc, _ = W.cube_256()
+    v = W.view_576x336()
     ov = O.make_view(v.width, v.height, v.fov_w, v.fov_h, list(v.position), list(v.aim), v.ray_length, v.samples)
     dirs = O.ray_dirs(ov)
     rc = O.RefCube(vals, org, spc)
     try:
         t = time.perf_counter()
-        img, cnt = rc.render_rows(ov, dirs, rows[0], rows[1], O.SEM_EXTENDED)
+        img, cnt = rc.render_rows(ov, dirs, 0, v.height, O.SEM_EXTENDED)
         el = time.perf_counter() - t
     finally:
         rc.free()
-    part = slice(rows[0], rows[1])
-    same = bool(np.array_equal(img[part].view(np.uint32), ref32[part].view(np.uint32)) and
-                np.array_equal(img[part].view(np.uint32), gpu_img[part].view(np.uint32)))
-    return {"value": round(cnt / el / 1e9, 6), "unit": "Gsamples/s", "cores": 1, "kind": "reference",
-            "ns_per_sample": round(el / cnt * 1e9, 1), "rows_bitwise_equal_to_gpu_and_oracle": same,
-            "sample": f"rows {rows[0]}..{rows[1] - 1} of the 576x336 render of the 256^3 cube ({cnt} in-cube samples) with the "
-                      f"reference's DataCube::GetSample compiled from its sources, in the Camera::Render loop, "
-                      f"1 thread, {el:.2f} s"}
+    oc = O.OracleCube(vals, org, spc)
+    t = time.perf_counter()
+    ref, cnt1 = O.render_f32(oc, ov, O.SEM_EXTENDED, nthreads=1)
+    el1 = time.perf_counter() - t
+    print(json.dumps({"reference_sampler": {"gsamples_per_s": round(cnt / el / 1e9, 6), "ns_per_sample": round(el / cnt * 1e9, 1),
+                                            "seconds": round(el, 2), "cores": 1},
+                      "oracle_port_1_thread": {"gsamples_per_s": round(cnt1 / el1 / 1e9, 6), "ns_per_sample": round(el1 / cnt1 * 1e9, 1),
+                                               "seconds": round(el1, 2), "cores": 1},
+                      "in_cube_samples": int(cnt), "bitwise_equal": bool(np.array_equal(img.view(np.uint32), ref.view(np.uint32))),
+                      "host": os.uname().nodename, "nproc": os.cpu_count()}))
 
 
 def bench_batched(capi, W, dev, vals, org, spc, fp32_vol):
@@ -229,7 +274,8 @@ def bench_batched(capi, W, dev, vals, org, spc, fp32_vol):
             "plain_renders_per_s_same_views": round(256 / plain_el, 1), "speedup_vs_plain": round(plain_el / el, 2),
             "max_rel_l2_vs_fp64": float(f"{max(errs):.3e}"), "in_cube_samples": int(in_cube),
             "in_cube_gsamples_per_s": round(in_cube / el / 1e9, 2),
-            "algorithmic_gb_per_s_device": round(alg / (last_ms * 1e-3) / 1e9, 1)}
+            "algorithmic_gb_per_s_device": round(alg / (last_ms * 1e-3) / 1e9, 1),
+            "roofline": valu_roofline("batched_f16_256cube_orbit256_valu_pmc.txt", last_ms)}
 
 
 def bench_cmem(capi, W, dev, steps=20):
@@ -255,7 +301,8 @@ def bench_cmem(capi, W, dev, steps=20):
     nsamp = v.width * v.height * v.samples
     return {"image": f"{v.width}x{v.height}", "samples_per_ray": v.samples, "samples": nsamp,
             "kernel_ms": round(dev_ms, 4), "gsamples_per_s": round(nsamp / wall / 1e9, 2),
-            "rel_l2_vs_fp64": float(f"{rel_l2(host, ref64):.4e}"), "bound": "valu"}
+            "rel_l2_vs_fp64": float(f"{rel_l2(host, ref64):.4e}"),
+            "roofline": valu_roofline("cmem_exact_576x336_valu_pmc.txt", dev_ms)}, host
 
 
 def bench_precision(capi, W, dev, n=512, count=64):
@@ -383,10 +430,15 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--forms", default="batched,cmem,precision,serving", help="extra forms to measure ('' for none)")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle checks")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (torch) to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE")
+    ap.add_argument("--cpu-reference", action="store_true",
+                    help="time the reference's own sampler (oracle/_ref, built in the development container only)")
     args = ap.parse_args()
+    if args.cpu_reference:
+        cpu_reference_container()
+        return
     if args.copy_calibration:
         import torch
         n = 1 << 28  # 1 GiB of float32
@@ -412,8 +464,15 @@ def main():
     ms_step = p["wall"] * 1e3 / args.steps
     kern_ms = p["dev_ms"] / args.steps
     alg_bytes = 32.0 * n_in + 4.0 * v.width * v.height
-    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
     traffic = read_traffic()
+    roof = valu_roofline("plain_exact_256cube_576x336_valu_pmc.txt", kern_ms) or {"bound": "valu", "frac": None}
+    roof["traffic"] = traffic
+    roof["kernel_ms"] = round(kern_ms, 5)
+    roof["hbm"] = {"achieved": round(traffic / (kern_ms * 1e-3) / 1e9, 1) if traffic else None, "peak": PEAK_HBM_GBS,
+                   "unit": "GB/s", "frac": round(traffic / (kern_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4) if traffic else None,
+                   "source": "profiles/plain_exact_256cube_576x336_pmc.txt"}
+    roof["gathers"] = {"algorithmic_bytes": int(alg_bytes), "achieved": round(alg_bytes / (kern_ms * 1e-3) / 1e9, 1),
+                       "unit": "GB/s", "note": "8 FP32 corners per in-cube sample + 4 B per pixel, served mostly by L1/L2"}
     line = {
         "metric": METRIC, "value": round(n_in * args.steps / p["wall"] / 1e9, 3), "unit": "Gsamples/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
@@ -421,10 +480,7 @@ def main():
         "config": {"workload": "plain exact FP32 render, 256^3 shell cube (seed 2), 576x336 px, 800 samples/ray "
                                "(step 0.1 RE, L 80 RE), camera (0,-20,10) aim (10,0,0), extended semantics",
                    "in_cube_samples": n_in, "nominal_samples": v.width * v.height * v.samples},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                     "kernel_ms": round(kern_ms, 5), "algorithmic_bytes": int(alg_bytes),
-                     "valu": valu_roofline(kern_ms)},
+        "roofline": roof,
         "forms": {"plain_renders_per_s": round(p["sync_rps"], 1),
                   "plain_renders_per_s_pinned_image": round(p["sync_rps_pinned"], 1),
                   "plain_renders_per_s_async": round(args.steps / p["wall"], 1)},
@@ -433,25 +489,15 @@ def main():
     ref64 = p["vol"].render_f64(v)
     line["error"] = {"vs_fp64_rel_l2": float(f"{rel_l2(p['image'], ref64):.4e}"),
                      "vs_fp64_max_rel": float(f"{max_rel(p['image'], ref64):.4e}")}
-    if not args.no_cpu:
-        log("cpu baseline: oracle render")
-        from oracle import oracle as O
-        line["cpu_baseline"], ref32 = cpu_baseline(O, W, capi, p["vals"], p["org"], p["spc"], v, n_in)
-        line["error"]["vs_oracle_fp32"] = "bitwise" if np.array_equal(ref32.view(np.uint32), p["image"].view(np.uint32)) \
-            else float(f"{rel_l2(p['image'], ref32):.4e}")
-        log("cpu reference: the reference's DataCube sampler over the whole image")
-        line["cpu_reference"] = cpu_reference(O, p["vals"], p["org"], p["spc"], v, ref32, p["image"])
-    else:
-        line["cpu_baseline"] = None
     forms = [f for f in args.forms.split(",") if f]
     if "batched" in forms:
         log("batched: 256 orbit poses, F16")
         line["forms"]["batched"] = bench_batched(capi, W, dev, p["vals"], p["org"], p["spc"], p["vol"])
     p["vol"].free()
-    del p["vals"]
+    cmem_img = None
     if "cmem" in forms:
         log("cmem: analytic model, 576x336 x 800 samples")
-        line["forms"]["cmem"] = bench_cmem(capi, W, dev)
+        line["forms"]["cmem"], cmem_img = bench_cmem(capi, W, dev)
     if "precision" in forms:
         log("precision: 512^3 FP32/BF16/F16")
         line["forms"]["precision"] = bench_precision(capi, W, dev)
@@ -459,6 +505,14 @@ def main():
         log("serving: Poisson 200 req/s")
         line["forms"]["serving"] = bench_serving(capi, W, dev)
     dev.close()
+    if not args.no_cpu:
+        log("cpu baseline: oracle render, oracle checks of the plain and CMEM images")
+        line["cpu_baseline"], checks = cpu_baseline_leg(W, p, cmem_img)
+        line["error"]["vs_oracle_fp32"] = checks["vs_oracle_fp32"]
+        if "cmem" in checks:
+            line["forms"]["cmem"]["vs_oracle"] = checks["cmem"]
+    else:
+        line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
 
 

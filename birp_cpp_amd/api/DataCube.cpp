@@ -1,7 +1,10 @@
 // DataCube.cpp -- the reference's DataCube API (reference src/DataCube.cpp) over the GPU.
 #include "DataCube.h"
 
+#include <algorithm>
+#include <charconv>
 #include <cmath>
+#include <exception>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -9,6 +12,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <thread>
 
 #include "birp.h"
 #include "device.h"
@@ -57,42 +61,108 @@ void DataCube::Init() {}
 
 void DataCube::Init(std::string& filename, bool debug) { Load(filename, debug); }
 
+namespace {
+[[noreturn]] void malformed() { throw std::invalid_argument("Datacube is invalid, inaccessible, or malformed."); }
+
+bool is_blank(char c) { return c == ' ' || c == '\t' || c == '\r' || c == '\n'; }
+
+// The blank-separated numbers of text [a, b) appended to out. std::from_chars reads a decimal or
+// "inf"/"nan" token correctly rounded to float, as strtof does; a token it does not take (a leading
+// '+', hexadecimal, a value that under- or overflows float) goes to strtof itself. A token that
+// is not a number throws.
+void parse_floats(const char* a, const char* b, std::vector<float>& out) {
+    const char* p = a;
+    while (true) {
+        while (p < b && is_blank(*p)) ++p;
+        if (p >= b) return;
+        float f = 0.f;
+        std::from_chars_result r = std::from_chars(p, b, f);
+        if (r.ec != std::errc() || (r.ptr < b && !is_blank(*r.ptr))) {
+            const char* e = p;
+            while (e < b && !is_blank(*e)) ++e;
+            const std::string tok(p, e);
+            char* end = nullptr;
+            f = std::strtof(tok.c_str(), &end);
+            if (end != tok.c_str() + tok.size()) malformed();
+            r.ptr = e;
+        }
+        out.push_back(f);
+        p = r.ptr;
+    }
+}
+}  // namespace
+
 // The .dat export (DataCube.cpp:49-125): line 1 is metadata, line 10 holds "nx ny nz"; every
 // number after line 10 is, in order, the nx x-coordinates, the ny y-coordinates, the nz
 // z-coordinates, then the nx*ny*nz values with x fastest, then y, then z. The reference walks the
 // same numbers line by line and so also needs each axis' coordinates and each x-row of values to
-// end a line; files in that layout read the same here.
+// end a line; files in that layout read the same here. The file is read whole and its numbers are
+// parsed on up to 16 threads, each taking a run of whole lines (a 512^3 cube is ~1.6 GB of text).
 void DataCube::Load(std::string filename, bool debug) {
-    std::ifstream in(filename);
-    if (!in) throw std::invalid_argument("Datacube is invalid, inaccessible, or malformed.");
-    std::string line;
-    int line_num = 0;
-    std::vector<int> dims;
-    std::vector<float> nums;
-    while (std::getline(in, line)) {
-        ++line_num;
-        if (line_num < 10) continue;
-        if (line_num == 10) {
-            dims = Helper_dims(line);
-            if (dims.size() < 3 || dims[0] < 1 || dims[1] < 1 || dims[2] < 1)
-                throw std::invalid_argument("Datacube is invalid, inaccessible, or malformed.");
-            nums.reserve((size_t)dims[0] + dims[1] + dims[2] + (size_t)dims[0] * dims[1] * dims[2]);
-            continue;
-        }
-        const char* p = line.c_str();
-        char* end = nullptr;
-        while (true) {
-            while (*p == ' ' || *p == '\t' || *p == '\r') ++p;
-            if (!*p) break;
-            float f = std::strtof(p, &end);
-            if (end == p) throw std::invalid_argument("Datacube is invalid, inaccessible, or malformed.");
-            nums.push_back(f);
-            p = end;
-        }
+    std::ifstream in(filename, std::ios::binary);
+    if (!in) malformed();
+    in.seekg(0, std::ios::end);
+    const std::streamoff len = in.tellg();
+    if (len <= 0) malformed();
+    std::string text((size_t)len, '\0');
+    in.seekg(0);
+    if (!in.read(&text[0], len)) malformed();
+    const char* const beg = text.data();
+    const char* const end = beg + text.size();
+    // lines 1-9 are skipped; line 10 holds the dimensions
+    const char* p = beg;
+    for (int line = 1; line < 10; line++) {
+        p = static_cast<const char*>(std::memchr(p, '\n', (size_t)(end - p)));
+        if (!p) malformed();
+        ++p;
     }
-    if (dims.size() < 3) throw std::invalid_argument("Datacube is invalid, inaccessible, or malformed.");
+    const char* eol = static_cast<const char*>(std::memchr(p, '\n', (size_t)(end - p)));
+    if (!eol) eol = end;
+    const std::vector<int> dims = Helper_dims(std::string(p, eol));
+    if (dims.size() < 3 || dims[0] < 1 || dims[1] < 1 || dims[2] < 1) malformed();
     const size_t nx = dims[0], ny = dims[1], nz = dims[2];
-    if (nums.size() != nx + ny + nz + nx * ny * nz) throw std::invalid_argument("Datacube is invalid, inaccessible, or malformed.");
+    const size_t want = nx + ny + nz + nx * ny * nz;
+    // split the rest into runs of whole lines, one per thread
+    const char* body = eol < end ? eol + 1 : end;
+    const size_t bytes = (size_t)(end - body);
+    unsigned hw = std::thread::hardware_concurrency();
+    size_t nt = std::min<size_t>(std::max(1u, std::min(hw, 16u)), std::max<size_t>(1, bytes >> 20));
+    std::vector<const char*> cut(1, body);
+    for (size_t t = 1; t < nt; t++) {
+        const char* q = body + bytes * t / nt;
+        if (q < cut.back()) q = cut.back();
+        const char* nl = static_cast<const char*>(std::memchr(q, '\n', (size_t)(end - q)));
+        cut.push_back(nl ? nl + 1 : end);
+    }
+    cut.push_back(end);
+    nt = cut.size() - 1;
+    std::vector<std::vector<float>> part(nt);
+    std::vector<std::exception_ptr> err(nt);
+    {
+        std::vector<std::thread> pool;
+        for (size_t t = 0; t < nt; t++)
+            pool.emplace_back([&, t] {
+                try {
+                    part[t].reserve(want / nt + 1024);
+                    parse_floats(cut[t], cut[t + 1], part[t]);
+                } catch (...) {
+                    err[t] = std::current_exception();
+                }
+            });
+        for (auto& th : pool) th.join();
+    }
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
+    size_t got = 0;
+    for (auto& v : part) got += v.size();
+    if (got != want) malformed();
+    // numbers in file order: the three axes, then the values
+    std::vector<float> nums;
+    nums.reserve(want);
+    for (auto& v : part) {
+        nums.insert(nums.end(), v.begin(), v.end());
+        std::vector<float>().swap(v);
+    }
     coords_x.assign(nums.begin(), nums.begin() + nx);
     coords_y.assign(nums.begin() + nx, nums.begin() + nx + ny);
     coords_z.assign(nums.begin() + nx + ny, nums.begin() + nx + ny + nz);

@@ -101,6 +101,8 @@ SIGNATURES = [
     ("birp_volume_bytes", _i, [_P, C.POINTER(C.c_size_t)]),
     ("birp_volume_sample", _i, [_P, _FP, C.c_longlong, _FP]),
     ("birp_render", _i, [_P, C.POINTER(View), _FP, C.POINTER(RenderStats)]),
+    ("birp_space_points", _i, [_P, C.POINTER(View), _i, _i, _FP]),
+    ("birp_space_integrate", _i, [_P, C.POINTER(View), _i, _i, _FP, _FP]),
     ("birp_image_create", _i, [_P, _i, _i, _PP]),
     ("birp_image_free", _i, [_P]),
     ("birp_image_download", _i, [_P, _FP]),
@@ -118,6 +120,8 @@ SIGNATURES = [
     ("birp_batch_download", _i, [_P, _i, _FP]),
     ("birp_batch_stats", _i, [_P, C.POINTER(C.c_longlong), _DP]),
     ("birp_batch_set_output", _i, [_P, _FP]),
+    ("birp_batch_create_exact", _i, [C.POINTER(_P), _i, C.POINTER(View), C.POINTER(_i), _i, _PP]),
+    ("birp_batch_samples", _i, [_P, C.POINTER(C.c_longlong)]),
     ("birp_autotune", _i, [_P, C.POINTER(View), _i, C.c_char_p]),
     ("birp_autotune_load", _i, [C.c_char_p]),
     ("birp_autotune_lookup", _i, [_i, _i, _i, _i, C.POINTER(Tuning)]),
@@ -311,6 +315,33 @@ class Batch:
         self._arr = arr
         self.h = C.c_void_p()
         check(lib().birp_batch_create(vol.h, arr, len(self.views), C.byref(self.h)), "birp_batch_create")
+
+    @classmethod
+    def exact(cls, volumes, views, volume_of=None):
+        """An exact batch (birp_batch_create_exact): view k of volumes[volume_of[k]] (or of the one
+        volume), each image bit-identical to Volume.render of that view with MATH_EXACT."""
+        self = cls.__new__(cls)
+        volumes = list(volumes) if isinstance(volumes, (list, tuple)) else [volumes]
+        self.vol = volumes[0]
+        self._vols = volumes
+        self.views = list(views)
+        self._arr = (View * len(self.views))(*self.views)
+        vh = (_P * len(volumes))(*[v.h.value for v in volumes])
+        self._vh = vh
+        vo = None
+        if volume_of is not None:
+            vo = (_i * len(self.views))(*[int(x) for x in volume_of])
+        self._vo = vo
+        self.h = C.c_void_p()
+        check(lib().birp_batch_create_exact(vh, len(volumes), self._arr, vo, len(self.views), C.byref(self.h)),
+              "birp_batch_create_exact")
+        return self
+
+    def samples(self):
+        """Per-view in-cube sample counts of the last finished launch (exact batches)."""
+        out = (C.c_longlong * len(self.views))()
+        check(lib().birp_batch_samples(self.h, out), "birp_batch_samples")
+        return [int(x) for x in out]
 
     def set_tuning(self, tuning: Tuning | None):
         check(lib().birp_batch_set_tuning(self.h, C.byref(tuning) if tuning is not None else None), "birp_batch_set_tuning")

@@ -79,6 +79,7 @@ struct ExactView {
     float w;  // ray_width = t[1] - t[0]
     int W, H, n;
     int semantics;    // BIRP_SEM_*
+    int vol;          // multi-view launches: index of the view's volume in the volume table
     int accumulate;   // start from out's values
     float margin;     // FP64 slab expansion (RE) that covers FP32 position error
     int fast_guards;  // positive cube whose scaled samples stay normal floats: only the
@@ -121,8 +122,27 @@ namespace birp {
 int launch_exact(const DevVolume& v, const ExactView& ev, int lanes_per_ray, int pipe, void* stream);
 // Slots of ExactView::wave_counts for a W x H image at `lanes_per_ray` (4 waves per block).
 int exact_wave_slots(int W, int H, int lanes_per_ray);
+// Sample modes of the exact kernels. SM_QUAD_POS: positive finite cube, |origin| >= 2^-30 on every
+// axis and scaled samples that stay normal floats, so the only guard left is the scale's midpoint
+// test; SM_QUAD_GEN: any finite cube (std::lerp's branches); SM_LIN_TRI / SM_LIN_NEAREST: the
+// [z][y][x] copy (any data, nearest sampling); SM_CMEM: the analytic model.
+enum SampleMode { SM_QUAD_POS = 0, SM_QUAD_GEN = 1, SM_LIN_TRI = 2, SM_LIN_NEAREST = 3, SM_CMEM = 4 };
+int exact_mode_of(const DevVolume& v, const ExactView& ev);
+// Lanes per ray of multi-view exact launches: 4 for CMEM, 2 for cubes (the measured best of each).
+inline int exact_multi_lanes(int mode) { return mode == SM_CMEM ? 4 : 2; }
+// Dynamic LDS bytes of an exact block: the t table, plus the brick offset tables in quad modes.
+size_t exact_lds_bytes(const DevVolume& v, int n, int mode);
+// 2x2-wave tiles of a W x H image at `lanes_per_ray`.
+int exact_tiles(int W, int H, int lanes_per_ray);
+// Views [0, nview) of one sample mode, tiles view-major from prefix (nview + 1 entries, device).
+int launch_exact_multi(const DevVolume* vols_dev, const ExactView* views_dev, const int* prefix_dev, int nview, int total,
+                       int mode, size_t lds, void* stream);
 int launch_f64(const DevVolume& v, const F64View& fv, void* stream);
 int launch_sample_points(const DevVolume& v, const float* xyz, long long n, float* out, void* stream);
+// Any other Space: the sample points of rows [row0, row0 + rows) of a view (3 floats per (pixel, k),
+// k fastest), and the integration of the caller's values (one float per (pixel, k)) into the rows.
+int launch_space_points(const ExactView& ev, int row0, int rows, float* xyz, void* stream);
+int launch_space_integrate(const ExactView& ev, int row0, int rows, const float* vals, float* img, void* stream);
 int launch_fast_batch(const DevVolume* vols_dev, int dtype, const FastPose* poses_dev, const int* tile_prefix_dev,
                       int npose, int total_tiles, unsigned* work_counter, int tile_w, int tile_h,
                       int lanes_per_ray, int unroll, int grid, void* stream);
@@ -133,11 +153,16 @@ int launch_build_quad(const DevVolume& v, void* quad, void* stream);
 int launch_build_oct(const DevVolume& v, void* oct, float* bscale, void* stream);
 
 // Fast-path kernel variants: the storage dtype (BIRP_DTYPE_*), or F16 whose records share one
-// scale for the whole volume, which the kernel then applies once per ray instead of per sample.
+// scale for the whole volume, which the kernel then applies once per ray instead of per sample;
+// plus BIRP_KDT_SIGNED for volumes with values <= 0, whose kernels stop each ray at its first
+// negative sample (Camera.cpp:207-210) and take one sample per lane per trip. Positive volumes
+// use the kernels without the stop test.
 #define BIRP_KDT_F16_UNIFORM 3
-#define BIRP_KDT_COUNT 4
+#define BIRP_KDT_SIGNED 4
+#define BIRP_KDT_COUNT 8
 inline int fast_kernel_dtype(const DevVolume& d) {
-    return (d.dtype == BIRP_DTYPE_F16 && d.f16_uniform) ? BIRP_KDT_F16_UNIFORM : d.dtype;
+    const int base = (d.dtype == BIRP_DTYPE_F16 && d.f16_uniform) ? BIRP_KDT_F16_UNIFORM : d.dtype;
+    return base | (d.vclass != VC_POSITIVE ? BIRP_KDT_SIGNED : 0);
 }
 int fast_grid_size(int dtype, int lanes_per_ray, int unroll);
 // The grid of one persistent launch over `tiles` tiles: the occupancy-limited grid, or about one

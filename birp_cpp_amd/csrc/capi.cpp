@@ -99,6 +99,7 @@ int exact_view(const birp_volume* vol, const birp_view* v, float* out_dev, Exact
     const double kf = (1.0 * BIRP_SCALE_K_HOST / BIRP_SCALE_C_HOST) / 10000;
     ev.fast_guards = (d.vclass == VC_POSITIVE && (double)d.vmin * ev.w * kf >= 0x1p-124 && (double)d.vmax * ev.w * kf <= 0x1p126) ? 1 : 0;
     ev.out = out_dev;
+    ev.vol = 0;
     ev.counter = nullptr;
     ev.wave_counts = nullptr;
     return BIRP_OK;
@@ -133,7 +134,7 @@ int fast_pose(const birp_volume* vol, const birp_view* v, int vol_index, float* 
     fp.vol = vol_index;
     fp.tiles_x = (v->width + tile_w - 1) / tile_w;
     fp.w = (float)((double)v->ray_length / v->samples);
-    if (fast_kernel_dtype(vol->dv) == BIRP_KDT_F16_UNIFORM) fp.w *= vol->dv.f16_scale;  // power of two: exact
+    if ((fast_kernel_dtype(vol->dv) & ~BIRP_KDT_SIGNED) == BIRP_KDT_F16_UNIFORM) fp.w *= vol->dv.f16_scale;  // power of two: exact
     fp.pad = 0.f;
     fp.out = out_dev;
     (void)tile_h;
@@ -572,6 +573,59 @@ extern "C" int birp_volume_sample(birp_volume* v, const float* xyz, long long n,
     return e == hipSuccess ? BIRP_OK : hip_fail(e, "sample");
 }
 
+// ============================================================================ any other Space
+namespace birp {
+// The exact-path view constants of a view without a volume (the geometry does not depend on it).
+static int space_view(const birp_view* view, int row0, int row1, ExactView& ev) {
+    birp_view v = *view;
+    v.math = BIRP_MATH_EXACT;
+    int rc = validate_view(&v, false);
+    if (rc) return rc;
+    if (row0 < 0 || row1 > view->height || row0 >= row1) return fail(BIRP_ERR_INVALID_ARGUMENT, "rows out of range");
+    if ((long long)(row1 - row0) * view->width * view->samples > (1ll << 28))
+        return fail(BIRP_ERR_INVALID_ARGUMENT, "at most 2^28 samples per call: pass fewer rows");
+    birp_volume none;
+    none.dv.kind = SK_CMEM;  // no cube: margin and guards are unused here
+    none.dv.vclass = VC_FINITE;
+    return exact_view(&none, &v, nullptr, ev);
+}
+}  // namespace birp
+
+extern "C" int birp_space_points(birp_device* dev, const birp_view* view, int row0, int row1, float* xyz) {
+    if (!dev || !view || !xyz) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
+    ExactView ev;
+    int rc = space_view(view, row0, row1, ev);
+    if (rc) return rc;
+    (void)hipSetDevice(dev->ordinal);
+    const size_t bytes = (size_t)(row1 - row0) * view->width * view->samples * 12;
+    void* buf = nullptr;
+    if ((rc = device_scratch(dev, bytes, &buf))) return rc;
+    if ((rc = launch_space_points(ev, row0, row1 - row0, (float*)buf, dev->stream))) return fail(rc, "points launch failed");
+    hipError_t e = hipMemcpyAsync(xyz, buf, bytes, hipMemcpyDeviceToHost, dev->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
+    return e == hipSuccess ? BIRP_OK : hip_fail(e, "space points");
+}
+
+extern "C" int birp_space_integrate(birp_device* dev, const birp_view* view, int row0, int row1, const float* values, float* image) {
+    if (!dev || !view || !values || !image) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
+    ExactView ev;
+    int rc = space_view(view, row0, row1, ev);
+    if (rc) return rc;
+    (void)hipSetDevice(dev->ordinal);
+    const size_t nval = (size_t)(row1 - row0) * view->width * view->samples, npix = (size_t)(row1 - row0) * view->width;
+    void* buf = nullptr;
+    if ((rc = device_scratch(dev, (nval + npix) * 4, &buf))) return rc;
+    float* dvals = (float*)buf;
+    float* dimg = dvals + nval;
+    hipError_t e = hipMemcpyAsync(dvals, values, nval * 4, hipMemcpyHostToDevice, dev->stream);
+    if (e == hipSuccess && ev.accumulate) e = hipMemcpyAsync(dimg, image, npix * 4, hipMemcpyHostToDevice, dev->stream);
+    if (e != hipSuccess) return hip_fail(e, "space values upload");
+    if ((rc = launch_space_integrate(ev, row0, row1 - row0, dvals, dimg, dev->stream))) return fail(rc, "integrate launch failed");
+    e = hipMemcpyAsync(image, dimg, npix * 4, hipMemcpyDeviceToHost, dev->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
+    return e == hipSuccess ? BIRP_OK : hip_fail(e, "space integrate");
+}
+
 // ============================================================================ single renders
 extern "C" int birp_view_basis(const birp_view* view, float* angle_sun, float rot[9], float rinv[9]) {
     if (!view || !finite3(view->position) || !finite3(view->aim)) return fail(BIRP_ERR_INVALID_ARGUMENT, "bad view");
@@ -855,6 +909,7 @@ extern "C" int birp_render_f64(birp_volume* vol, const birp_view* view, double* 
 namespace birp {
 
 int batch_setup(birp_batch* b);
+int exact_view(const birp_volume* vol, const birp_view* v, float* out_dev, ExactView& ev);
 
 int choose_tuning(const birp_volume* vol, const std::vector<birp_view>& views, birp_tuning& t) {
     // the tuning of the batch's dominant (largest) image class
@@ -901,12 +956,102 @@ void batch_release(birp_batch* b) {
     if (b->d_poses) (void)hipFree(b->d_poses);
     if (b->d_prefix) (void)hipFree(b->d_prefix);
     if (b->d_vols) (void)hipFree(b->d_vols);
+    if (b->d_evs) (void)hipFree(b->d_evs);
     b->d_poses = nullptr;
     b->d_prefix = nullptr;
     b->d_vols = nullptr;
+    b->d_evs = nullptr;
+}
+
+// The kernels of an exact batch on stream st: one multi-view launch per sample-mode group.
+int exact_batch_launch(birp_batch* b, hipStream_t st) {
+    int base = 0;
+    for (size_t g = 0; g < b->groups.size(); g++) {
+        const birp_batch::ExactGroup& G = b->groups[g];
+        int rc = launch_exact_multi(b->d_vols, b->d_evs + G.first, b->d_prefix + base, G.count, G.total, G.mode, G.lds, st);
+        if (rc) return rc;
+        base += G.count + 1;
+    }
+    return BIRP_OK;
+}
+
+// Exact batches: per-view ExactView descriptors (the arithmetic of birp_render's exact path), grouped
+// by sample mode, the tile prefix sums of each group, the volume table; then the graph.
+int exact_batch_setup(birp_batch* b) {
+    birp_device* dev = b->vol->dev;
+    batch_release(b);
+    const int n = (int)b->views.size();
+    std::vector<ExactView> evs(n);
+    std::vector<int> mode(n);
+    for (int k = 0; k < n; k++) {
+        const birp_volume* vol = b->evols[b->vol_of[k]];
+        float* out = (b->host_out_dev ? b->host_out_dev : b->d_images) + b->offsets[k];
+        int rc = exact_view(vol, &b->views[k], out, evs[k]);
+        if (rc) return rc;
+        evs[k].vol = b->vol_of[k];
+        mode[k] = exact_mode_of(vol->dv, evs[k]);
+    }
+    // group by mode; each view's sample-count slots (4 per tile at the group's lanes per ray)
+    b->groups.clear();
+    b->order.clear();
+    b->count_off.assign(n, 0);
+    b->count_slots = 0;
+    std::vector<int> prefix;
+    std::vector<ExactView> packed;
+    for (int m = SM_QUAD_POS; m <= SM_CMEM; m++) {
+        birp_batch::ExactGroup G;
+        G.mode = m;
+        G.first = (int)b->order.size();
+        prefix.push_back(0);
+        for (int k = 0; k < n; k++) {
+            if (mode[k] != m) continue;
+            const int lanes = exact_multi_lanes(m);
+            const int tiles = exact_tiles(b->views[k].width, b->views[k].height, lanes);
+            const long long next = (long long)prefix.back() + tiles;
+            if (next > INT32_MAX) return fail(BIRP_ERR_UNSUPPORTED, "exact batch of more than 2^31 - 1 tiles: split it");
+            prefix.push_back((int)next);
+            b->count_off[k] = b->count_slots;
+            b->count_slots += 4 * (size_t)tiles;
+            G.lds = std::max(G.lds, exact_lds_bytes(b->evols[b->vol_of[k]]->dv, b->views[k].samples, m));
+            b->order.push_back(k);
+            packed.push_back(evs[k]);
+            G.count++;
+        }
+        if (!G.count) {
+            prefix.pop_back();
+            continue;
+        }
+        G.total = prefix.back();
+        if (G.lds > 64 * 1024) return fail(BIRP_ERR_UNSUPPORTED, "exact batch view needs more than 64 KiB of LDS");
+        b->groups.push_back(G);
+    }
+    hipError_t e;
+    if (b->d_counts) (void)hipFree(b->d_counts);
+    b->d_counts = nullptr;
+    if ((e = hipMalloc(&b->d_counts, std::max<size_t>(b->count_slots, 1) * 4)) != hipSuccess) return hip_fail(e, "hipMalloc(counts)");
+    for (size_t q = 0; q < packed.size(); q++) packed[q].wave_counts = b->d_counts + b->count_off[b->order[q]];
+    std::vector<DevVolume> vt(b->evols.size());
+    for (size_t k = 0; k < vt.size(); k++) vt[k] = b->evols[k]->dv;
+    if ((e = hipMalloc(&b->d_evs, sizeof(ExactView) * n)) != hipSuccess) return hip_fail(e, "hipMalloc(views)");
+    if ((e = hipMalloc(&b->d_prefix, sizeof(int) * prefix.size())) != hipSuccess) return hip_fail(e, "hipMalloc(prefix)");
+    if ((e = hipMalloc(&b->d_vols, sizeof(DevVolume) * vt.size())) != hipSuccess) return hip_fail(e, "hipMalloc(vols)");
+    e = hipMemcpyAsync(b->d_evs, packed.data(), sizeof(ExactView) * n, hipMemcpyHostToDevice, dev->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(b->d_prefix, prefix.data(), sizeof(int) * prefix.size(), hipMemcpyHostToDevice, dev->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(b->d_vols, vt.data(), sizeof(DevVolume) * vt.size(), hipMemcpyHostToDevice, dev->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
+    if (e != hipSuccess) return hip_fail(e, "exact batch upload");
+    hipStream_t cs = b->capture_stream;
+    if ((e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal)) != hipSuccess) return hip_fail(e, "begin capture");
+    int rc = exact_batch_launch(b, cs);
+    e = hipStreamEndCapture(cs, &b->graph);
+    if (rc != BIRP_OK) return fail(rc, "exact batch capture launch failed");
+    if (e != hipSuccess) return hip_fail(e, "end capture");
+    if ((e = hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0)) != hipSuccess) return hip_fail(e, "graph instantiate");
+    return BIRP_OK;
 }
 
 int batch_setup(birp_batch* b) {
+    if (b->exact) return exact_batch_setup(b);
     birp_device* dev = b->vol->dev;
     batch_release(b);
     const birp_tuning& t = b->tuning;
@@ -984,11 +1129,79 @@ bad:
     return rc;
 }
 
+extern "C" int birp_batch_create_exact(birp_volume* const* volumes, int nvolumes, const birp_view* views, const int* volume_of,
+                                       int count, birp_batch** out) {
+    if (!volumes || nvolumes <= 0 || !views || count <= 0 || !out || (nvolumes > 1 && !volume_of))
+        return fail(BIRP_ERR_INVALID_ARGUMENT, "bad argument");
+    if (count > (1 << 20) || nvolumes > (1 << 16)) return fail(BIRP_ERR_INVALID_ARGUMENT, "too many views or volumes");
+    *out = nullptr;
+    for (int k = 0; k < nvolumes; k++) {
+        if (!volumes[k]) return fail(BIRP_ERR_INVALID_ARGUMENT, "null volume");
+        if (volumes[k]->dev != volumes[0]->dev) return fail(BIRP_ERR_INVALID_ARGUMENT, "volumes on different devices");
+        if (volumes[k]->dv.kind == SK_CUBE && !volumes[k]->dv.linear) return fail(BIRP_ERR_UNSUPPORTED, "exact renders need an FP32 volume");
+    }
+    for (int k = 0; k < count; k++) {
+        birp_view v = views[k];
+        v.math = BIRP_MATH_EXACT;
+        int rc = validate_view(&v, true);
+        if (rc) return rc;
+        const int vi = volume_of ? volume_of[k] : 0;
+        if (vi < 0 || vi >= nvolumes) return fail(BIRP_ERR_INVALID_ARGUMENT, "volume index out of range");
+    }
+    birp_device* dev = volumes[0]->dev;
+    (void)hipSetDevice(dev->ordinal);
+    birp_batch* b = new birp_batch();
+    b->exact = true;
+    b->vol = volumes[0];
+    b->evols.assign(volumes, volumes + nvolumes);
+    b->views.assign(views, views + count);
+    size_t off = 0;
+    for (int k = 0; k < count; k++) {
+        b->views[k].math = BIRP_MATH_EXACT;
+        b->vol_of.push_back(volume_of ? volume_of[k] : 0);
+        b->offsets.push_back(off);
+        off += (size_t)views[k].width * views[k].height;
+        b->nominal += (long long)views[k].width * views[k].height * views[k].samples;
+    }
+    b->total_pixels = off;
+    hipError_t e;
+    int rc;
+    if ((e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking)) != hipSuccess) { rc = hip_fail(e, "stream"); goto bad; }
+    if ((e = hipMalloc(&b->d_images, off * sizeof(float))) != hipSuccess) { rc = hip_fail(e, "hipMalloc(images)"); goto bad; }
+    if ((rc = launch_zero_f32(b->d_images, off, dev->stream)) != BIRP_OK) goto bad;  // accumulating views start from 0
+    if ((e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess) { rc = hip_fail(e, "event"); goto bad; }
+    if ((rc = batch_setup(b)) != BIRP_OK) goto bad;
+    *out = b;
+    return BIRP_OK;
+bad:
+    birp_batch_free(b);
+    return rc;
+}
+
+extern "C" int birp_batch_samples(birp_batch* b, long long* in_cube) {
+    if (!b || !in_cube) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
+    if (!b->exact) return fail(BIRP_ERR_UNSUPPORTED, "sample counts are kept by exact batches");
+    std::vector<unsigned> c(b->count_slots);
+    hipError_t e = hipStreamSynchronize(b->vol->dev->stream);
+    if (e == hipSuccess && !c.empty()) e = hipMemcpy(c.data(), b->d_counts, c.size() * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "batch sample counts");
+    for (size_t q = 0; q < b->order.size(); q++) {
+        const int k = b->order[q];
+        const size_t first = b->count_off[k];
+        const size_t last = (q + 1 < b->order.size()) ? b->count_off[b->order[q + 1]] : b->count_slots;
+        long long s = 0;
+        for (size_t i = first; i < last; i++) s += c[i];
+        in_cube[k] = s;
+    }
+    return BIRP_OK;
+}
+
 extern "C" int birp_batch_free(birp_batch* b) {
     if (!b) return BIRP_OK;
     if (b->capture_stream) (void)hipStreamSynchronize(b->capture_stream);
     (void)hipStreamSynchronize(b->vol->dev->stream);
     batch_release(b);
+    if (b->d_counts) (void)hipFree(b->d_counts);
     if (b->d_images) (void)hipFree(b->d_images);
     if (b->d_work) (void)hipFree(b->d_work);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -1000,6 +1213,7 @@ extern "C" int birp_batch_free(birp_batch* b) {
 
 extern "C" int birp_batch_set_tuning(birp_batch* b, const birp_tuning* t) {
     if (!b) return fail(BIRP_ERR_INVALID_ARGUMENT, "null batch");
+    if (b->exact) return fail(BIRP_ERR_UNSUPPORTED, "exact batches have no tuning");
     birp_tuning nt;
     if (t) {
         nt = *t;
@@ -1028,11 +1242,18 @@ extern "C" int birp_batch_launch(birp_batch* b) {
 }
 
 extern "C" int birp_batch_launch_eager(birp_batch* b) {
-    if (!b || !b->d_poses) return fail(BIRP_ERR_INVALID_ARGUMENT, "batch not ready");
+    if (!b || (!b->d_poses && !b->d_evs)) return fail(BIRP_ERR_INVALID_ARGUMENT, "batch not ready");
     birp_device* dev = b->vol->dev;
     if (take_injected_failure(dev)) return fail(BIRP_ERR_INJECTED, "injected failure");
     const birp_tuning& t = b->tuning;
     (void)hipEventRecord(b->ev0, dev->stream);
+    if (b->exact) {
+        int rc = exact_batch_launch(b, dev->stream);
+        if (rc) return fail(rc, "exact batch launch failed");
+        (void)hipEventRecord(b->ev1, dev->stream);
+        b->launched = true;
+        return BIRP_OK;
+    }
     int rc = launch_reset_counter(b->d_work, dev->stream);
     if (rc == BIRP_OK)
         rc = launch_fast_batch(b->d_vols, fast_kernel_dtype(b->vol->dv), b->d_poses, b->d_prefix, (int)b->views.size(),

@@ -80,6 +80,21 @@ struct birp_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool launched = false;
     double last_ms = 0.0;
+    // exact batches (birp_batch_create_exact): views over several volumes, grouped by sample mode,
+    // one multi-view launch per group
+    bool exact = false;
+    std::vector<birp_volume*> evols;  // the batch's volumes (evols[vol_of[k]] renders view k)
+    std::vector<int> vol_of;
+    struct ExactGroup {
+        int mode = 0, first = 0, count = 0, total = 0;  // views order[first, first + count)
+        size_t lds = 0;
+    };
+    std::vector<ExactGroup> groups;
+    std::vector<int> order;           // view indices, group by group
+    std::vector<size_t> count_off;    // each view's first sample-count slot
+    size_t count_slots = 0;
+    birp::ExactView* d_evs = nullptr;
+    unsigned* d_counts = nullptr;     // one slot per wave: its in-cube samples of the last launch
 };
 
 namespace birp {

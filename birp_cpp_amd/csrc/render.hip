@@ -393,15 +393,13 @@ template <int S> struct ExactTile {
     static constexpr int TH = (64 / S) / TW;
 };
 
-// SM_QUAD_POS: positive finite cube, |origin| >= 2^-30 on every axis and scaled samples that
-// stay normal floats, so the only guard left is the scale's midpoint test.
-enum SampleMode { SM_QUAD_POS = 0, SM_QUAD_GEN = 1, SM_LIN_TRI = 2, SM_LIN_NEAREST = 3, SM_CMEM = 4 };
 
 // ------------------------------------------------------------------------------------------
-// the exact render kernel
+// the exact render: one block renders one 2x2-wave tile `lb` of a view (wg_x tiles per row of
+// tiles); every thread of the block calls it
 // ------------------------------------------------------------------------------------------
-template <int S, int MODE, int P = 1>
-__global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact(DevVolume v, ExactView ev, int wg_x, int nblocks) {
+template <int S, int MODE, int P>
+__device__ __forceinline__ void exact_block(const DevVolume& v, const ExactView& ev, int lb, int wg_x) {
     constexpr bool QUAD = (MODE == SM_QUAD_POS || MODE == SM_QUAD_GEN);
     constexpr bool POS = (MODE == SM_QUAD_POS);
     extern __shared__ unsigned s_mem[];
@@ -423,12 +421,6 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact(DevVolume v, ExactV
         qc.hz = v.fnz - 2.0f;
     }
     __syncthreads();
-
-    // XCD-aware block order: consecutive dispatch slots go round-robin over the 8 XCDs, so give
-    // each XCD a contiguous band of tiles (neighbouring rays share cube bricks in its L2).
-    const int per_xcd = (nblocks + 7) / 8;
-    const int lb = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    if (lb >= nblocks) return;
 
     constexpr int TW = ExactTile<S>::TW, TH = ExactTile<S>::TH;
     const int wave = threadIdx.x >> 6, lane = lane_id();
@@ -554,6 +546,41 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact(DevVolume v, ExactV
     }
 }
 
+// One view: block b renders tile lb(b). Consecutive dispatch slots go round-robin over the 8
+// XCDs, so each XCD is given a contiguous band of tiles (neighbouring rays share cube bricks in
+// its L2).
+template <int S, int MODE, int P = 1>
+__global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact(DevVolume v, ExactView ev, int wg_x, int nblocks) {
+    const int per_xcd = (nblocks + 7) / 8;
+    const int lb = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (lb >= nblocks) return;
+    exact_block<S, MODE, P>(v, ev, lb, wg_x);
+}
+
+// Many views (each with its own volume, image and sample-count slots) in one launch: items are
+// the views' tiles, view-major (prefix[k] = first item of view k), and each XCD takes a contiguous
+// band of items as above. Every block renders one tile with exactly the arithmetic of
+// k_render_exact, so each image is bit-identical to its single-view render; with more tiles in
+// flight than one view has, the SIMDs stay fed while the longest rays of a view finish.
+template <int S, int MODE, int P = 1>
+__global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact_multi(const DevVolume* __restrict__ vols,
+                                                                   const ExactView* __restrict__ views,
+                                                                   const int* __restrict__ prefix, int nview, int total) {
+    const int per_xcd = (total + 7) / 8;
+    const int g = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (g >= total) return;
+    int lo = 0, hi = nview - 1;  // the view of item g: the last k with prefix[k] <= g
+    while (lo < hi) {
+        const int mid = __builtin_amdgcn_readfirstlane((lo + hi + 1) >> 1);
+        if (prefix[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    const ExactView& ev = views[lo];
+    constexpr int TW = ExactTile<S>::TW;
+    exact_block<S, MODE, P>(vols[ev.vol], ev, g - prefix[lo], (ev.W + 2 * TW - 1) / (2 * TW));
+}
+
 // ------------------------------------------------------------------------------------------
 // FP64 render (every operation in double) for error reporting on large configurations
 // ------------------------------------------------------------------------------------------
@@ -639,6 +666,43 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_f64(DevVolume v, F64View 
         acc += s;
     }
     fv.out[pix] = acc;
+}
+
+// ------------------------------------------------------------------------------------------
+// Any other Space (its GetSample is the caller's host code): the rays' sample points, then the
+// integration of the caller's sample values (Camera.cpp:140-229 around the call at :191)
+// ------------------------------------------------------------------------------------------
+// One thread per (pixel, k) of rows [row0, row0 + rows): X = d t_k + P in the reference's FP32
+// order (Camera.cpp:183-189), stored at xyz[((pixel - row0 W) n + k) * 3].
+__global__ __launch_bounds__(BIRP_BLOCK) void k_space_points(ExactView ev, int row0, int rows, float* xyz) {
+    const long long q = (long long)blockIdx.x * BIRP_BLOCK + threadIdx.x;
+    const long long total = (long long)rows * ev.W * ev.n;
+    if (q >= total) return;
+    const int k = (int)(q % ev.n);
+    const long long pix = q / ev.n;
+    const int i = (int)(pix % ev.W), j = row0 + (int)(pix / ev.W);
+    float dx, dy, dz;
+    exact_ray_dir(ev, i, j, dx, dy, dz);
+    const float t = ev.L * (((float)k + 0.5f) / (float)ev.n);  // Camera.cpp:73
+    xyz[3 * q] = dx * t + ev.px;
+    xyz[3 * q + 1] = dy * t + ev.py;
+    xyz[3 * q + 2] = dz * t + ev.pz;
+}
+
+// One thread per pixel: the reference's scale chain (Camera.cpp:201), the stop at the first
+// negative sample (:207-210) and the FP32 sum in k order (:223) over vals[(pixel - row0 W) n + k].
+__global__ __launch_bounds__(BIRP_BLOCK) void k_space_integrate(ExactView ev, int row0, int rows, const float* vals, float* img) {
+    const long long pix = (long long)blockIdx.x * BIRP_BLOCK + threadIdx.x;
+    if (pix >= (long long)rows * ev.W) return;
+    float acc = ev.accumulate ? img[pix] : 0.f;
+    const float* v = vals + pix * ev.n;
+    for (int k = 0; k < ev.n; ++k) {
+        const float s = birp_scale_sample(v[k] * ev.w);
+        if (s < 0) break;
+        acc = acc + s;
+    }
+    img[pix] = acc;
+    (void)row0;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -885,10 +949,15 @@ __device__ __forceinline__ void kahan_add(float x, float& sum, float& nc) {
 // per XCD queue. Block b starts on queue b % 8 (dispatch places consecutive blocks on successive
 // XCDs, so a queue's tiles -- neighbouring rays of one image -- meet in one L2) and moves on to
 // the other queues when its own is drained. Every wave leaves after finding all 8 queues empty.
-template <int DT, int S, int U>
+// KV is a kernel variant (fast_kernel_dtype): the storage kernel in its low bits, plus
+// BIRP_KDT_SIGNED for volumes with values <= 0, whose rays stop at their first negative sample.
+template <int KV, int S, int U>
 __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __restrict__ vols, const FastPose* __restrict__ poses,
                                                             const int* __restrict__ prefix, int npose, int total,
                                                             unsigned* counters, int tw_log2) {
+    constexpr int DT = KV & 3;
+    constexpr bool SIGNED = (KV & BIRP_KDT_SIGNED) != 0;
+    static_assert(!SIGNED || U == 1, "signed kernels take one sample per lane per trip");
     __shared__ unsigned s_tab[BIRP_BLOCK / 64][BIRP_FAST_TAB];
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
@@ -990,8 +1059,31 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
         // of the wave fills completely need no clamp and no mask; the rest of the trips run below
         // with both (the same samples in the same order and grouping either way).
         float sum = 0.f, nc = 0.f;
-        const int tfull = wave_min_i(my) / U * U;
-        int q = 0;
+        if constexpr (SIGNED) {
+            // Camera.cpp:207-210: a ray ends at its first negative sample. The S lanes of a ray take
+            // samples q*S + sub in trip q, so the ray's first negative sample of a trip is the lowest
+            // set bit of its S-bit group in the wave's ballot: a lane adds its sample unless its
+            // group has a negative sample at or before it, and the ray dies with any in the group.
+            const int gshift = ray * S;
+            bool alive = true;  // the same in all S lanes of a ray
+            for (int q = 0; q < trips; ++q) {
+                const int js = q * S + sub;
+                const bool act = alive && js < cnt;
+                const float fj = (float)min(js, cnt - 1);
+                const float xd = __builtin_fmaf(fj, dix, ix0);
+                const float yd = __builtin_fmaf(fj, diy, iy0);
+                const float zd = __builtin_fmaf(fj, diz, iz0);
+                const float x = fast_sample<DT>(V, rs, tab, xd, yd, zd);
+                const unsigned long long neg = __ballot(act && x < 0.f);
+                const unsigned grp = (unsigned)(neg >> gshift) & ((1u << S) - 1u);
+                const bool cut = (grp & ((2u << sub) - 1u)) != 0u;
+                kahan_add((act && !cut) ? x : 0.f, sum, nc);
+                alive = alive && grp == 0u;
+                if (__ballot(alive && (q + 1) * S < cnt) == 0ull) break;  // every ray of the wave is done
+            }
+        }
+        const int tfull = SIGNED ? 0 : wave_min_i(my) / U * U;
+        int q = SIGNED ? trips : 0;
         for (; q < tfull; q += U) {
             float part = 0.f;
 #pragma unroll
@@ -1043,6 +1135,20 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
 // ------------------------------------------------------------------------------------------
 static int hip_status(hipError_t e) { return e == hipSuccess ? BIRP_OK : BIRP_ERR_DEVICE; }
 
+int exact_mode_of(const DevVolume& v, const ExactView& ev) {
+    if (v.kind == SK_CMEM) return SM_CMEM;
+    if (v.interp == BIRP_INTERP_NEAREST) return SM_LIN_NEAREST;
+    if (v.vclass == VC_NONFINITE || !v.quad || !v.qtab) return SM_LIN_TRI;
+    if (v.vclass == VC_POSITIVE && v.origin_far && ev.fast_guards) return SM_QUAD_POS;
+    return SM_QUAD_GEN;
+}
+
+size_t exact_lds_bytes(const DevVolume& v, int n, int mode) {
+    size_t lds = (size_t)((n + 3) & ~3) * sizeof(float);
+    if (mode == SM_QUAD_POS || mode == SM_QUAD_GEN) lds += (size_t)(v.nx + v.ny + v.nz) * sizeof(unsigned);
+    return lds;
+}
+
 template <int S>
 static int launch_exact_s(const DevVolume& v, const ExactView& ev, int pipe, hipStream_t st) {
     constexpr int TW = ExactTile<S>::TW, TH = ExactTile<S>::TH;
@@ -1050,15 +1156,8 @@ static int launch_exact_s(const DevVolume& v, const ExactView& ev, int pipe, hip
     int wg_y = (ev.H + 2 * TH - 1) / (2 * TH);
     int nblocks = wg_x * wg_y;
     int grid = ((nblocks + 7) / 8) * 8;
-    int mode;
-    if (v.kind == SK_CMEM) mode = SM_CMEM;
-    else if (v.interp == BIRP_INTERP_NEAREST) mode = SM_LIN_NEAREST;
-    else if (v.vclass == VC_NONFINITE || !v.quad || !v.qtab) mode = SM_LIN_TRI;
-    else if (v.vclass == VC_POSITIVE && v.origin_far && ev.fast_guards) mode = SM_QUAD_POS;
-    else mode = SM_QUAD_GEN;
-    size_t lds = (size_t)((ev.n + 3) & ~3) * sizeof(float);
-    if (mode == SM_QUAD_POS || mode == SM_QUAD_GEN) lds += (size_t)(v.nx + v.ny + v.nz) * sizeof(unsigned);
-
+    const int mode = exact_mode_of(v, ev);
+    const size_t lds = exact_lds_bytes(v, ev.n, mode);
     if (lds > 64 * 1024) return BIRP_ERR_UNSUPPORTED;
     switch (mode) {
         case SM_QUAD_POS:
@@ -1081,9 +1180,27 @@ static int exact_blocks(int W, int H) {
     return ((W + 2 * TW - 1) / (2 * TW)) * ((H + 2 * TH - 1) / (2 * TH));
 }
 
-int exact_wave_slots(int W, int H, int lanes_per_ray) {
-    const int b = lanes_per_ray == 4 ? exact_blocks<4>(W, H) : (lanes_per_ray == 1 ? exact_blocks<1>(W, H) : exact_blocks<2>(W, H));
-    return 4 * b;
+int exact_tiles(int W, int H, int lanes_per_ray) {
+    return lanes_per_ray == 4 ? exact_blocks<4>(W, H) : (lanes_per_ray == 1 ? exact_blocks<1>(W, H) : exact_blocks<2>(W, H));
+}
+
+int exact_wave_slots(int W, int H, int lanes_per_ray) { return 4 * exact_tiles(W, H, lanes_per_ray); }
+
+int launch_exact_multi(const DevVolume* vols, const ExactView* views, const int* prefix, int nview, int total, int mode,
+                       size_t lds, void* stream) {
+    if (nview <= 0 || total <= 0) return BIRP_OK;
+    if (lds > 64 * 1024) return BIRP_ERR_UNSUPPORTED;
+    hipStream_t st = (hipStream_t)stream;
+    const int grid = ((total + 7) / 8) * 8;
+    switch (mode) {
+        case SM_QUAD_POS: hipLaunchKernelGGL((k_render_exact_multi<2, SM_QUAD_POS, 2>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total); break;
+        case SM_QUAD_GEN: hipLaunchKernelGGL((k_render_exact_multi<2, SM_QUAD_GEN>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total); break;
+        case SM_LIN_TRI: hipLaunchKernelGGL((k_render_exact_multi<2, SM_LIN_TRI>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total); break;
+        case SM_LIN_NEAREST: hipLaunchKernelGGL((k_render_exact_multi<2, SM_LIN_NEAREST>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total); break;
+        case SM_CMEM: hipLaunchKernelGGL((k_render_exact_multi<4, SM_CMEM>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total); break;
+        default: return BIRP_ERR_INVALID_ARGUMENT;
+    }
+    return hip_status(hipGetLastError());
 }
 
 int launch_exact(const DevVolume& v, const ExactView& ev, int lanes_per_ray, int pipe, void* stream) {
@@ -1097,6 +1214,22 @@ int launch_exact(const DevVolume& v, const ExactView& ev, int lanes_per_ray, int
 int launch_f64(const DevVolume& v, const F64View& fv, void* stream) {
     int npix = fv.W * fv.H;
     hipLaunchKernelGGL(k_render_f64, dim3((npix + BIRP_BLOCK - 1) / BIRP_BLOCK), dim3(BIRP_BLOCK), 0, (hipStream_t)stream, v, fv);
+    return hip_status(hipGetLastError());
+}
+
+int launch_space_points(const ExactView& ev, int row0, int rows, float* xyz, void* stream) {
+    const long long total = (long long)rows * ev.W * ev.n;
+    if (total <= 0) return BIRP_OK;
+    hipLaunchKernelGGL(k_space_points, dim3((unsigned)((total + BIRP_BLOCK - 1) / BIRP_BLOCK)), dim3(BIRP_BLOCK), 0, (hipStream_t)stream,
+                       ev, row0, rows, xyz);
+    return hip_status(hipGetLastError());
+}
+
+int launch_space_integrate(const ExactView& ev, int row0, int rows, const float* vals, float* img, void* stream) {
+    const long long total = (long long)rows * ev.W;
+    if (total <= 0) return BIRP_OK;
+    hipLaunchKernelGGL(k_space_integrate, dim3((unsigned)((total + BIRP_BLOCK - 1) / BIRP_BLOCK)), dim3(BIRP_BLOCK), 0,
+                       (hipStream_t)stream, ev, row0, rows, vals, img);
     return hip_status(hipGetLastError());
 }
 
@@ -1159,10 +1292,14 @@ static void* fast_kernel_ptr() { return (void*)k_render_fast<DT, S, U>; }
 template <int DT, int S>
 static int launch_fast_su(int unroll, int grid, hipStream_t st, const DevVolume* vols, const FastPose* poses, const int* prefix,
                           int npose, int total, unsigned* counter, int twl) {
-    switch (unroll) {
-        case 4: hipLaunchKernelGGL((k_render_fast<DT, S, 4>), dim3(grid), dim3(BIRP_BLOCK), 0, st, vols, poses, prefix, npose, total, counter, twl); break;
-        case 2: hipLaunchKernelGGL((k_render_fast<DT, S, 2>), dim3(grid), dim3(BIRP_BLOCK), 0, st, vols, poses, prefix, npose, total, counter, twl); break;
-        default: hipLaunchKernelGGL((k_render_fast<DT, S, 1>), dim3(grid), dim3(BIRP_BLOCK), 0, st, vols, poses, prefix, npose, total, counter, twl); break;
+    if constexpr ((DT & BIRP_KDT_SIGNED) != 0) {
+        hipLaunchKernelGGL((k_render_fast<DT, S, 1>), dim3(grid), dim3(BIRP_BLOCK), 0, st, vols, poses, prefix, npose, total, counter, twl);
+    } else {
+        switch (unroll) {
+            case 4: hipLaunchKernelGGL((k_render_fast<DT, S, 4>), dim3(grid), dim3(BIRP_BLOCK), 0, st, vols, poses, prefix, npose, total, counter, twl); break;
+            case 2: hipLaunchKernelGGL((k_render_fast<DT, S, 2>), dim3(grid), dim3(BIRP_BLOCK), 0, st, vols, poses, prefix, npose, total, counter, twl); break;
+            default: hipLaunchKernelGGL((k_render_fast<DT, S, 1>), dim3(grid), dim3(BIRP_BLOCK), 0, st, vols, poses, prefix, npose, total, counter, twl); break;
+        }
     }
     return hip_status(hipGetLastError());
 }
@@ -1182,15 +1319,17 @@ int launch_fast_batch(const DevVolume* vols_dev, int dtype, const FastPose* pose
     int twl = 0;
     while ((1 << twl) < tile_w) twl++;
     if ((1 << twl) != tile_w || tile_w * tile_h * lanes_per_ray != 64) return BIRP_ERR_INVALID_ARGUMENT;
+    if (dtype < 0 || dtype >= BIRP_KDT_COUNT) return BIRP_ERR_INVALID_ARGUMENT;
     grid = ((grid + 7) / 8) * 8;
     hipStream_t st = (hipStream_t)stream;
-    if (dtype == BIRP_KDT_F16_UNIFORM)
-        return launch_fast_d<BIRP_KDT_F16_UNIFORM>(lanes_per_ray, unroll, grid, st, vols_dev, poses_dev, tile_prefix_dev, npose, total_tiles, work_counter, twl);
-    if (dtype == BIRP_DTYPE_F16)
-        return launch_fast_d<BIRP_DTYPE_F16>(lanes_per_ray, unroll, grid, st, vols_dev, poses_dev, tile_prefix_dev, npose, total_tiles, work_counter, twl);
-    if (dtype == BIRP_DTYPE_BF16)
-        return launch_fast_d<BIRP_DTYPE_BF16>(lanes_per_ray, unroll, grid, st, vols_dev, poses_dev, tile_prefix_dev, npose, total_tiles, work_counter, twl);
-    return launch_fast_d<BIRP_DTYPE_F32>(lanes_per_ray, unroll, grid, st, vols_dev, poses_dev, tile_prefix_dev, npose, total_tiles, work_counter, twl);
+    if (dtype & BIRP_KDT_SIGNED) unroll = 1;  // the signed kernels take one sample per lane per trip
+#define BIRP_LFD(KV) \
+    case KV: return launch_fast_d<KV>(lanes_per_ray, unroll, grid, st, vols_dev, poses_dev, tile_prefix_dev, npose, total_tiles, work_counter, twl);
+    switch (dtype) {
+        BIRP_LFD(0) BIRP_LFD(1) BIRP_LFD(2) BIRP_LFD(3) BIRP_LFD(4) BIRP_LFD(5) BIRP_LFD(6) BIRP_LFD(7)
+        default: return BIRP_ERR_INVALID_ARGUMENT;
+    }
+#undef BIRP_LFD
 }
 
 int fast_grid_for(int dtype, int lanes_per_ray, int unroll, int tiles) {
@@ -1204,6 +1343,7 @@ int fast_grid_size(int dtype, int lanes_per_ray, int unroll) {
     int dev = 0;
     (void)hipGetDevice(&dev);
     // cached per device and kernel variant: the occupancy query costs microseconds per launch
+    if (dtype & BIRP_KDT_SIGNED) unroll = 1;
     static std::atomic<int> cache[16][BIRP_KDT_COUNT][5][5];
     const bool cacheable = dev >= 0 && dev < 16 && dtype >= 0 && dtype < BIRP_KDT_COUNT && lanes_per_ray <= 4 && unroll <= 4;
     if (cacheable) {
@@ -1219,10 +1359,19 @@ int fast_grid_size(int dtype, int lanes_per_ray, int unroll) {
     if (lanes_per_ray == 4) fn = unroll == 4 ? fast_kernel_ptr<DT, 4, 4>() : unroll == 2 ? fast_kernel_ptr<DT, 4, 2>() : fast_kernel_ptr<DT, 4, 1>(); \
     else if (lanes_per_ray == 2) fn = unroll == 4 ? fast_kernel_ptr<DT, 2, 4>() : unroll == 2 ? fast_kernel_ptr<DT, 2, 2>() : fast_kernel_ptr<DT, 2, 1>(); \
     else fn = unroll == 4 ? fast_kernel_ptr<DT, 1, 4>() : unroll == 2 ? fast_kernel_ptr<DT, 1, 2>() : fast_kernel_ptr<DT, 1, 1>();
-    if (dtype == BIRP_KDT_F16_UNIFORM) { BIRP_FK(BIRP_KDT_F16_UNIFORM) }
-    else if (dtype == BIRP_DTYPE_F16) { BIRP_FK(BIRP_DTYPE_F16) }
-    else if (dtype == BIRP_DTYPE_BF16) { BIRP_FK(BIRP_DTYPE_BF16) }
-    else { BIRP_FK(BIRP_DTYPE_F32) }
+#define BIRP_FKS(DT) \
+    fn = lanes_per_ray == 4 ? fast_kernel_ptr<DT, 4, 1>() : lanes_per_ray == 2 ? fast_kernel_ptr<DT, 2, 1>() : fast_kernel_ptr<DT, 1, 1>();
+    switch (dtype) {
+        case 3: BIRP_FK(3) break;
+        case 2: BIRP_FK(2) break;
+        case 1: BIRP_FK(1) break;
+        case 4: BIRP_FKS(4) break;
+        case 5: BIRP_FKS(5) break;
+        case 6: BIRP_FKS(6) break;
+        case 7: BIRP_FKS(7) break;
+        default: BIRP_FK(0) break;
+    }
+#undef BIRP_FKS
 #undef BIRP_FK
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BIRP_BLOCK, 0) != hipSuccess || per_cu <= 0) per_cu = 4;
     if (cacheable) cache[dev][dtype][lanes_per_ray][unroll].store(cus * per_cu, std::memory_order_relaxed);

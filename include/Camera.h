@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "DataCube.h"
+#include "birp.h"
 
 class Camera {
 public:
@@ -18,9 +19,10 @@ public:
 
     void SetPosition(float&, float&, float&);
     void SetAim(float, float, float);
-    // Adds one render into `image` (repeated calls accumulate, as Camera.cpp:223 does).
-    // Throws std::invalid_argument for a Space other than DataCube or CMEM, and
-    // std::runtime_error for a device failure.
+    // Adds one render into `image` (repeated calls accumulate, as Camera.cpp:223 does). DataCube and
+    // CMEM spaces render on the GPU; any other Space subclass is sampled through its own virtual
+    // GetSample on the host at the sample points the GPU forms, and the GPU integrates the values.
+    // Throws std::invalid_argument for a malformed image and std::runtime_error for a device failure.
     void Render();
     int ToDat(std::string filename);   // CSV, one row per line, trailing commas (Camera.cpp:235-250)
     int ToFITS(std::string filename);  // FITS, BITPIX -32, row 0 first (Camera.cpp:252-307)
@@ -34,10 +36,17 @@ public:
     void SetMath(int math);                  // birp.h BIRP_MATH_*, default BIRP_MATH_EXACT
     int Width() const { return image_dimension_x; }
     int Height() const { return image_dimension_y; }
-    long long LastSamplesInCube() const { return last_in_cube_; }
+    long long LastSamplesInCube() const { return last_in_cube_; }  // -1 for other Space types
     double LastKernelMs() const { return last_ms_; }
+    // Render() of every camera, the exact DataCube / CMEM ones together in one batched launch
+    // (the same images as rendering them one after another).
+    static void RenderAll(const std::vector<Camera*>& cameras);
 
 private:
+    birp_view View() const;
+    void CheckImage() const;
+    birp_volume* DeviceSpace() const;
+    void RenderOtherSpace();
     int image_dimension_x;
     int image_dimension_y;
     float fov_x;

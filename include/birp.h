@@ -156,6 +156,22 @@ int birp_volume_bytes(const birp_volume* vol, size_t* bytes);
  * give -1 as in the reference. */
 int birp_volume_sample(birp_volume* vol, const float* xyz, long long n, float* out);
 
+/* Any other Space (a subclass whose virtual GetSample is the caller's host code, such as the
+ * reference tests' CubeFake, tests/Camera_test.cpp:25-29): replaces the per-pixel loop of
+ * Camera::Render around the virtual call (Camera.cpp:140-229, the call at :191) by two device
+ * steps around the caller's own sampling. birp_space_points writes the FP32 sample points
+ * X = d t_k + P of image rows [row0, row1), exactly as the exact renders form them, into xyz:
+ * 3 floats per (pixel, k), k fastest, pixels row-major from (0, row0). The caller evaluates its
+ * GetSample there (the reference stops calling for a ray once a value v has v * w < 0, with w the
+ * view's ray width) and birp_space_integrate adds the values (one float per (pixel, k), same order;
+ * values after a ray's stop are not read) into image rows [row0, row1) of `image` ((row1 - row0) *
+ * W floats; read first when the view has BIRP_RENDER_ACCUMULATE): the scale chain of
+ * Camera.cpp:201, the stop at the first negative sample (:207-210) and the FP32 sum in k order
+ * (:223), bit-identical to the reference. The view's semantics has no effect (a generic Space has
+ * no bounds); at most 2^28 samples per call. Synchronous. */
+int birp_space_points(birp_device* dev, const birp_view* view, int row0, int row1, float* xyz);
+int birp_space_integrate(birp_device* dev, const birp_view* view, int row0, int row1, const float* values, float* image);
+
 /* ------------------------------------------------------------------ render (Camera) */
 /* Replaces Camera::Render (Camera.cpp:63-233): ray generation, march, integration. Host image
  * (H*W floats): read first when BIRP_RENDER_ACCUMULATE is set, always written. Synchronous. An
@@ -209,6 +225,22 @@ int birp_batch_download(birp_batch* b, int index, float* host);
  * device buffer. Re-captures the launch. */
 int birp_batch_set_output(birp_batch* b, float* host);
 int birp_batch_stats(birp_batch* b, long long* samples_nominal, double* last_ms);
+
+/* Exact batches: replaces the reference's serial loop of Camera::Render over the cubes of a batch
+ * directory (main.cpp:242-264; Camera.cpp:63-233 per cube) with one launch per sample mode. View
+ * k renders volume volumes[volume_of[k]] (volume_of may be NULL when nvolumes == 1) with the
+ * BIRP_MATH_EXACT arithmetic of birp_render, so every image is bit-identical to birp_render of
+ * that view and volume; volumes may be FP32 cubes (any interpolation and sign) and CMEM models, on
+ * one device, and views may differ in size, pose, ray and semantics. A view with
+ * BIRP_RENDER_ACCUMULATE adds into its image's current contents (zero at creation; the previous
+ * launch's result after that). The tiles of all views of one sample mode share one launch, so SIMDs
+ * stay fed while a view's longest rays finish. Launch, wait, download, set_output, stats and free
+ * work as for fast batches; birp_batch_set_tuning returns BIRP_ERR_UNSUPPORTED. */
+int birp_batch_create_exact(birp_volume* const* volumes, int nvolumes, const birp_view* views, const int* volume_of,
+                            int count, birp_batch** out);
+/* Per-view in-cube sample counts of an exact batch's last finished launch (count entries, as
+ * birp_render_stats.samples_in_cube); BIRP_ERR_UNSUPPORTED for fast batches. */
+int birp_batch_samples(birp_batch* b, long long* in_cube);
 
 /* Autotuner: times every candidate birp_tuning on the given views (few launches each), keeps
  * the fastest per (dtype, image-size class, samples class) in an in-process table and, when

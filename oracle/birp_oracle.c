@@ -300,14 +300,35 @@ static cmem_zones64 cmem_zones_f64(const or_cmem* m, double x, double y, double 
     return o;
 }
 
+/* The zone chain of Cmem.cpp:132-176 in FP64, with the zone chosen by r_zone and the value
+ * formed at the point's own r (they differ only when asking what a neighbouring zone gives). */
+static double cmem_zone_value(const cmem_zones64* o, double r_zone) {
+    const double h = 0.025, r = o->r;
+    if (r_zone < o->rmp - h) return 0.0;
+    if (r_zone <= o->rmp + h) return or_lerp(0.0, o->inner, (r - (o->rmp - h)) / (o->rmp + h));
+    if (r_zone < o->rbs - h) return o->inner;
+    if (r_zone <= o->rbs + h) return or_lerp(o->inner, o->outer, (r - (o->rbs - h)) / (o->rbs + h));
+    return o->outer;
+}
+
 double or_cmem_sample_f64(const or_cmem* m, double x, double y, double z) {
     cmem_zones64 o = cmem_zones_f64(m, x, y, z);
-    const double h = 0.025, r = o.r;
-    if (r < o.rmp - h) return 0.0;
-    if (r <= o.rmp + h) return or_lerp(0.0, o.inner, (r - (o.rmp - h)) / (o.rmp + h));
-    if (r < o.rbs - h) return o.inner;
-    if (r <= o.rbs + h) return or_lerp(o.inner, o.outer, (r - (o.rbs - h)) / (o.rbs + h));
-    return o.outer;
+    return cmem_zone_value(&o, o.r);
+}
+
+double or_cmem_edge_jump_f64(const or_cmem* m, double x, double y, double z, double rel_band) {
+    cmem_zones64 o = cmem_zones_f64(m, x, y, z);
+    const double h = 0.025;
+    const double e[4] = {o.rmp - h, o.rmp + h, o.rbs - h, o.rbs + h};
+    double jump = 0.0;
+    for (int k = 0; k < 4; k++) {
+        if (!(fabs(o.r - e[k]) < rel_band * o.r)) continue;
+        /* the two zones meeting at e[k]: just below it and just above it (the chain compares
+         * with < and <=, so both sides are reached within a relative 1e-12) */
+        const double d = fabs(e[k]) * 1e-12 + 1e-300;
+        jump += fabs(cmem_zone_value(&o, e[k] - d) - cmem_zone_value(&o, e[k] + d));
+    }
+    return jump;
 }
 
 double or_cmem_edge_distance_f64(const or_cmem* m, double x, double y, double z) {
@@ -816,6 +837,73 @@ int or_render_f64(const or_space* s, const or_view* v, int semantics, int row0, 
     if (n_incube) *n_incube = jb.count;
     free(t);
     return 0;
+}
+
+typedef struct {
+    const or_cmem* m;
+    const or_view* v;
+    const float* rinv;
+    const float* t;
+    double rel_band, wk;
+    int row0, row1, count;
+    unsigned char* mask;
+    double* bound;
+} edge_job;
+
+static void* edge_rows(void* p) {
+    edge_job* jb = (edge_job*)p;
+    const or_view* v = jb->v;
+    for (int j = jb->row0; j < jb->row1; ++j)
+        for (int i = 0; i < v->W; ++i) {
+            float d[3];
+            or_ray_dir(v, jb->rinv, i, j, d);
+            double b = 0.0;
+            unsigned char hit = 0;
+            for (int k = 0; k < v->n; ++k) {
+                float x = d[0] * jb->t[k] + v->pos[0], y = d[1] * jb->t[k] + v->pos[1], z = d[2] * jb->t[k] + v->pos[2];
+                if (or_cmem_edge_distance_f64(jb->m, x, y, z) < jb->rel_band) {
+                    hit = 1;
+                    b += or_cmem_edge_jump_f64(jb->m, x, y, z, jb->rel_band) * jb->wk;
+                }
+            }
+            jb->mask[(size_t)(j - jb->row0) * v->W + i] = hit;
+            jb->bound[(size_t)(j - jb->row0) * v->W + i] = b;
+            jb->count += hit;
+        }
+    return NULL;
+}
+
+int or_cmem_edge_bound(const or_cmem* m, const or_view* v, double rel_band, int row0, int row1, int nthreads,
+                       unsigned char* mask, double* bound) {
+    float rot[9], rinv[9], w;
+    if (or_view_basis(v, rot, rinv) != 0) return -1;
+    float* t = (float*)malloc(sizeof(float) * (size_t)(v->n > 0 ? v->n : 1));
+    or_ray_dist(v->L, v->n, t, &w);
+    const int rows = row1 - row0;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > rows) nthreads = rows > 0 ? rows : 1;
+    edge_job* jobs = (edge_job*)calloc((size_t)nthreads, sizeof(edge_job));
+    pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
+    /* a pixel's sample adds v * w * 637810000 / (4/pi) / 10000 to the image (Camera.cpp:201) */
+    const double wk = (double)w * (1000 * 100 * 6378.1) / (4 / M_PI) / 10000;
+    for (int k = 0; k < nthreads; k++) {
+        edge_job* jb = &jobs[k];
+        jb->m = m; jb->v = v; jb->rinv = rinv; jb->t = t; jb->rel_band = rel_band; jb->wk = wk;
+        jb->row0 = row0 + (int)((long long)rows * k / nthreads);
+        jb->row1 = row0 + (int)((long long)rows * (k + 1) / nthreads);
+        jb->mask = mask + (size_t)(jb->row0 - row0) * v->W;
+        jb->bound = bound + (size_t)(jb->row0 - row0) * v->W;
+        pthread_create(&th[k], NULL, edge_rows, jb);
+    }
+    int count = 0;
+    for (int k = 0; k < nthreads; k++) {
+        pthread_join(th[k], NULL);
+        count += jobs[k].count;
+    }
+    free(jobs);
+    free(th);
+    free(t);
+    return count;
 }
 
 int or_cmem_edge_pixels(const or_cmem* m, const or_view* v, double rel_band, unsigned char* mask) {

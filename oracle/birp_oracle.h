@@ -106,6 +106,17 @@ double or_cmem_edge_distance_f64(const or_cmem* m, double x, double y, double z)
 /* mask[j*W+i] = 1 when pixel (i,j)'s FP32 ray (traced as or_render_f32 traces it) has a sample
  * within rel_band of a zone edge; returns the number of such pixels, -1 for a singular basis. */
 int or_cmem_edge_pixels(const or_cmem* m, const or_view* v, double rel_band, unsigned char* mask);
+/* The change in the model's value at (x,y,z) when its r lands on the other side of each zone edge
+ * e within rel_band (|r - e| < rel_band * r): the sum over those edges of |zone below e - zone above
+ * e|, both formed at the point's own r, in FP64 (0 when no edge is that close). A sample whose
+ * FP32 evaluations (under two libms) fall on opposite sides of an edge differ by about this. */
+double or_cmem_edge_jump_f64(const or_cmem* m, double x, double y, double z, double rel_band);
+/* Rows [row0, row1) of a view: mask as or_cmem_edge_pixels, and bound = the sum over the pixel's
+ * near-edge samples of or_cmem_edge_jump_f64 * w * 637810000 / (4/pi) / 10000 (the largest image
+ * change the zone flips of those samples can make, Camera.cpp:201). Both arrays hold
+ * (row1 - row0) * W entries; threaded over rows. Returns the count of masked pixels. */
+int or_cmem_edge_bound(const or_cmem* m, const or_view* v, double rel_band, int row0, int row1, int nthreads,
+                       unsigned char* mask, double* bound);
 
 /* Camera::Camera image size (Camera.cpp:38-39). */
 void or_image_dims(float pixel_deg, float fov_h, float fov_w, int* W, int* H);

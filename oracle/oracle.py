@@ -82,6 +82,10 @@ def lib():
         L.or_cmem_edge_distance_f64.argtypes = [C.POINTER(Cmem), _d, _d, _d]
         L.or_cmem_edge_pixels.restype = _i
         L.or_cmem_edge_pixels.argtypes = [C.POINTER(Cmem), C.POINTER(View), _d, C.POINTER(C.c_ubyte)]
+        L.or_cmem_edge_jump_f64.restype = _d
+        L.or_cmem_edge_jump_f64.argtypes = [C.POINTER(Cmem), _d, _d, _d, _d]
+        L.or_cmem_edge_bound.restype = _i
+        L.or_cmem_edge_bound.argtypes = [C.POINTER(Cmem), C.POINTER(View), _d, _i, _i, _i, C.POINTER(C.c_ubyte), _dp]
         L.or_image_dims.argtypes = [_f, _f, _f, _ip, _ip]
         L.or_ray_dist.argtypes = [_f, _i, _fp, _fp]
         L.or_orient.restype = _f
@@ -169,6 +173,23 @@ class OracleCmem:
         if rc < 0:
             raise ValueError("singular view basis")
         return mask.astype(bool)
+
+    def edge_jump(self, x, y, z, rel_band: float) -> float:
+        """FP64 change of the model's value at a point whose r lies within rel_band of zone edges
+        (sum over those edges of |zone below - zone above| at the point's r; 0 if none is near)."""
+        return lib().or_cmem_edge_jump_f64(C.byref(self.m), x, y, z, rel_band)
+
+    def edge_bound(self, view: "View", rel_band: float, rows=None, nthreads=1):
+        """(mask, bound): HxW (or rows) near-edge pixel mask and, per pixel, the largest image change
+        that zone flips of its near-edge samples can make (FP64, in image units)."""
+        r0, r1 = (0, view.H) if rows is None else rows
+        mask = np.zeros((r1 - r0, view.W), np.uint8)
+        bound = np.zeros((r1 - r0, view.W), np.float64)
+        rc = lib().or_cmem_edge_bound(C.byref(self.m), C.byref(view), rel_band, r0, r1, nthreads,
+                                      mask.ctypes.data_as(C.POINTER(C.c_ubyte)), _dptr(bound))
+        if rc < 0:
+            raise ValueError("singular view basis")
+        return mask.astype(bool), bound
 
     def state(self) -> dict:
         return {name: (list(getattr(self.m, name)) if isinstance(getattr(self.m, name), C.Array)

@@ -3,7 +3,7 @@
 // prints results (floats as IEEE bit patterns) for the Python side to compare with the golden
 // vectors and the oracle. Host-only subcommands: inverse, pointing, cmem_state, load, write_outputs,
 // save_load. GPU subcommands: sample, render, cmem_sample, cmem_render, render_twice,
-// unsupported_space.
+// space_render, render_all.
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -36,6 +36,18 @@ class CubeFake : public Space {
 public:
     void Init() override {}
     float GetSample(float, float, float) override { return 1.f; }
+};
+
+// A user Space with negative values: 1 for x > 0.5, -1 elsewhere (rays stop where they cross
+// into x <= 0.5); counts its GetSample calls.
+class HalfSpace : public Space {
+public:
+    long long calls = 0;
+    void Init() override {}
+    float GetSample(float x, float, float) override {
+        calls++;
+        return x > 0.5f ? 1.f : -1.f;
+    }
 };
 
 static void print_image(const Camera& cam) {
@@ -273,16 +285,70 @@ int main(int argc, char** argv) {
             print_image(cam);
             return 0;
         }
-        if (cmd == "unsupported_space") {
+        if (cmd == "space_render") {
+            // space_render <fake|halfspace> <outdir or -> <pixel> <fov_h> <fov_w> <px> <py> <pz> <ax> <L> <n> <times>:
+            // the reference test's CubeFake scenario (Camera_test.cpp:31-73) and a user Space with
+            // negative values (1 for x > x0, -1 elsewhere), rendered <times> times (Render accumulates),
+            // then ToDat / ToFITS into <outdir>
+            const std::string kind = argv[2], outdir = argv[3];
             CubeFake fake;
-            Camera cam(fake, 0.25f, 36.f, 36.f);
-            try {
-                cam.Render();
-            } catch (const std::invalid_argument& e) {
-                std::printf("invalid_argument %s\n", e.what());
-                return 0;
+            HalfSpace half;
+            Space* space = kind == "fake" ? static_cast<Space*>(&fake) : static_cast<Space*>(&half);
+            Camera cam(*space, (float)std::atof(argv[4]), (float)std::atof(argv[5]), (float)std::atof(argv[6]));
+            float px = std::atof(argv[7]), py = std::atof(argv[8]), pz = std::atof(argv[9]);
+            cam.SetPosition(px, py, pz);
+            cam.SetAim(std::atof(argv[10]), 0.f, 0.f);
+            cam.SetRay(std::atof(argv[11]), std::atoi(argv[12]));
+            const int times = std::atoi(argv[13]);
+            for (int t = 0; t < times; t++) cam.Render();
+            if (outdir != "-") {
+                cam.ToDat(outdir + "/testout");
+                cam.ToFITS(outdir + "/testout");
             }
-            return 1;
+            print_image(cam);
+            std::fprintf(stderr, "calls %lld\n", half.calls);
+            return 0;
+        }
+        if (cmd == "render_all") {
+            // render_all <L> <n> <semantics> <dat>...: one camera per cube (trilinear, sizes and poses
+            // varying), then a CMEM camera; Camera::RenderAll twice, then the same cameras again with
+            // Render() twice each; prints every image of both, then the in-cube counts
+            std::vector<DataCube> cubes(argc - 5);
+            CMEM m;
+            m.Init();
+            for (size_t k = 0; k < cubes.size(); k++) {
+                std::string path = argv[5 + k];
+                cubes[k].Init(path, false);
+                cubes[k].SetTrilinear(true);
+            }
+            auto make = [&](std::vector<Camera>& cams) {
+                cams.reserve(cubes.size() + 1);
+                for (size_t k = 0; k < cubes.size(); k++) cams.emplace_back(cubes[k], 48 + 8 * (int)k, 32, 20.f, 30.f);
+                cams.emplace_back(m, 40, 24, 20.f, 30.f);
+                for (size_t k = 0; k < cams.size(); k++) {
+                    float px = 0.f + k, py = -20.f + 3.f * k, pz = 10.f;
+                    cams[k].SetPosition(px, py, pz);
+                    cams[k].SetAim(10.f, 0.f, 0.f);
+                    cams[k].SetRay(std::atof(argv[2]), std::atoi(argv[3]));
+                    cams[k].SetSemantics(std::atoi(argv[4]));
+                }
+            };
+            std::vector<Camera> batched, single;
+            make(batched);
+            make(single);
+            std::vector<Camera*> ptrs;
+            for (auto& c : batched) ptrs.push_back(&c);
+            Camera::RenderAll(ptrs);
+            Camera::RenderAll(ptrs);
+            for (auto& c : single) {
+                c.Render();
+                c.Render();
+            }
+            for (auto* set : {&batched, &single})
+                for (auto& c : *set) print_image(c);
+            for (auto* set : {&batched, &single})
+                for (auto& c : *set) std::printf("%lld\n", c.LastSamplesInCube());
+            return 0;
         }
     } catch (const std::exception& e) {
         std::fprintf(stderr, "exception: %s\n", e.what());

@@ -148,6 +148,42 @@ def test_load_matches_reference_loader(exe, golden_dat):
     assert vals == np.asarray(g["values"]).ravel().tolist()
 
 
+def test_parallel_load_of_a_large_varied_file(exe, tmp_path):
+    """DataCube::Load splits the numbers after line 10 over threads by runs of whole lines: a
+    70 x 60 x 50 cube (3.6 MB of text) written with mixed number forms ('%.9g', '%.8e', a leading
+    '+', subnormals, zeros, negatives), blanks (runs of spaces, tabs) and CRLF line ends loads
+    with every value bitwise equal to the float32 it was printed from."""
+    rng = np.random.default_rng(8)
+    nz, ny, nx = 50, 60, 70
+    vals = (rng.standard_normal((nz, ny, nx)) * np.exp(rng.uniform(-30, 30, (nz, ny, nx)))).astype(np.float32)
+    vals[rng.random(vals.shape) < 0.05] = 0.0
+    vals.ravel()[::997] = np.float32(1e-40)  # a subnormal
+    fmts = ["%.9g", "%.8e", "+%.9g"]
+    seps = [" ", "  ", "\t", " \t "]
+    p = tmp_path / "varied.dat"
+    with open(p, "w", newline="") as f:
+        f.write("cube\r\n" + "#\r\n" * 8 + f"{nx} {ny} {nz}\r\n")
+        for n in (nx, ny, nz):
+            f.write(" ".join("%.9g" % x for x in np.arange(n, dtype=np.float32) * np.float32(0.5)) + "\r\n")
+        k = 0
+        for z in range(nz):
+            for y in range(ny):
+                toks = []
+                for x in vals[z, y]:
+                    fmt = fmts[k % 3]
+                    toks.append((fmt if x >= 0 or fmt[0] != "+" else fmt[1:]) % x)
+                    toks.append(seps[k % 4])
+                    k += 1
+                f.write("".join(toks[:-1]) + ("\r\n" if (z + y) % 2 else "\n"))
+    assert os.path.getsize(p) > 3_000_000
+    rc, out, err = run(exe, "load", p)
+    assert rc == 0, err
+    lines = out.split("\n")
+    assert [int(x) for x in lines[0].split()] == [nx, ny, nz]
+    got = np.array([int(x) for x in lines[3:] if x], np.uint32)
+    assert np.array_equal(got, vals.ravel().view(np.uint32))
+
+
 def test_save_then_load_roundtrip(exe, golden_dat, tmp_path):
     rc, out, err = run(exe, "save_load", golden_dat[0], tmp_path / "copy.dat")
     assert rc == 0 and out.strip() == "same", err
@@ -277,24 +313,116 @@ def test_cmem_api_known_answers_and_render(exe, device):
     oc = O.OracleCmem()
     ref, _ = O.render_f32(oc, v, O.SEM_REFERENCE, nthreads=16)
     ref64, _ = O.render_f64(oc, v, O.SEM_REFERENCE, nthreads=16)
-    cmem_render_parity(img, ref, ref64, oc.edge_pixels(v, CMEM_EDGE_BAND))
+    cmem_render_parity(img, ref, ref64, *oc.edge_bound(v, CMEM_EDGE_BAND, nthreads=16))
+
+
+def _fp32_prefix_sums(s1, n):
+    """P[m] = the FP32 sum, in order, of m copies of s1 (P[0] = 0)."""
+    P = np.zeros(n + 1, np.float32)
+    acc = np.float32(0.0)
+    for m in range(1, n + 1):
+        acc = np.float32(acc + s1)
+        P[m] = acc
+    return P
 
 
 @pytest.mark.gpu
-def test_render_of_other_space_types_is_refused(exe, device):
-    rc, out, _ = run(exe, "unsupported_space")
-    assert rc == 0 and out.startswith("invalid_argument")
+def test_cubefake_renders_its_constant_field(exe, device, tmp_path):
+    """The reference's own Camera tests (tests/Camera_test.cpp:25-73): a CubeFake (GetSample == 1)
+    at (1,1,1) aiming at x = 1, 0.25 deg pixels over 36 x 36 deg, rendered and written with ToDat
+    and ToFITS. Every pixel is the FP32 sum of 200 copies of the scaled sample s1 = f32(f64(w) *
+    637810000 / (4/pi) / 1e4) (closed form; w = 44 * 1.5/200 - 44 * 0.5/200 in FP32), and a second
+    Render adds 200 more; the files hold the image."""
+    rc, out, err = run(exe, "space_render", "fake", tmp_path, 0.25, 36.0, 36.0, 1.0, 1.0, 1.0, 1.0, 44.0, 200, 1)
+    assert rc == 0, err
+    img = _read_image(out)
+    t, w = O.ray_dist(44.0, 200)
+    s1 = np.float32(O.scale_sample(1.0, w))
+    P = _fp32_prefix_sums(s1, 400)
+    assert img.shape == (144, 144) and np.array_equal(img.view(np.uint32), np.full_like(img, P[200]).view(np.uint32))
+    rows = open(str(tmp_path / "testout.dat")).read().split("\n")
+    assert len(rows) == 145 and rows[0] == ("%f," % P[200]) * 144
+    cards, data = _read_fits(str(tmp_path / "testout.fits"))
+    assert np.array_equal(data, img) and float(cards["POS_X"]) == 1.0
+    rc, out, err = run(exe, "space_render", "fake", "-", 0.25, 36.0, 36.0, 1.0, 1.0, 1.0, 1.0, 44.0, 200, 2)
+    assert rc == 0, err
+    twice = _read_image(out)
+    acc = P[200]
+    for _ in range(200):
+        acc = np.float32(acc + s1)
+    assert np.array_equal(twice.view(np.uint32), np.full_like(twice, acc).view(np.uint32))
 
 
-# ---------------------------------------------------------------------------------- the reference's CLI
+@pytest.mark.gpu
+def test_user_space_with_negative_values_stops_rays(exe, device):
+    """A user Space that is 1 for x > 0.5 and -1 elsewhere, seen from (3, -10, 0) along +y (the FOV
+    spans rays that cross x = 0.5 within the ray length and rays that do not): each ray adds s1 for
+    its samples up to its first point with x <= 0.5 (FP32 points d t_k + P from the oracle's rays)
+    and stops there (Camera.cpp:207-210). The host calls GetSample up to and including that sample
+    and no further, as the reference's loop does."""
+    pos, aim, L, n = (3.0, -10.0, 0.0), 3.0, 20.0, 300
+    rc, out, err = run(exe, "space_render", "halfspace", "-", 0.5, 30.0, 40.0, *pos, aim, L, n, 1)
+    assert rc == 0, err
+    img = _read_image(out)
+    H, W = img.shape
+    v = O.make_view(W, H, 40.0, 30.0, list(pos), [aim, 0, 0], L, n)
+    d = O.ray_dirs(v).reshape(H, W, 3)
+    t, w = O.ray_dist(L, n)
+    x = (d[:, :, 0:1] * t[None, None, :].astype(np.float32)).astype(np.float32) + np.float32(pos[0])
+    neg = x <= np.float32(0.5)
+    m = np.where(neg.any(axis=2), neg.argmax(axis=2), n)  # samples added before the stop
+    P = _fp32_prefix_sums(np.float32(O.scale_sample(1.0, w)), n)
+    assert 0 < (m < n).mean() < 1 and m.min() > 0
+    assert np.array_equal(img.view(np.uint32), P[m].view(np.uint32))
+    calls = int(err.split("calls")[1].split()[0])
+    assert calls == int(np.minimum(m + 1, n).sum())
+
+
+@pytest.mark.gpu
+def test_render_all_equals_renders_one_by_one(exe, device, tmp_path):
+    """Camera::RenderAll (one exact batch over three .dat cubes and a CMEM model, twice) equals
+    Render() twice on each camera bit for bit, cube images equal the oracle's FP32 renders
+    (accumulated twice), and the in-cube counts agree."""
+    paths, ocs = [], []
+    for k, (n, seed) in enumerate(((48, 31), (40, 32), (56, 33))):
+        p = tmp_path / f"c{k}.dat"
+        ocs.append(_shell_dat(p, n, seed))
+        paths.append(str(p))
+    for sem in (0, 1):
+        rc, out, err = run(exe, "render_all", 30.0, 300, sem, *paths)
+        assert rc == 0, err
+        vals = out.split()
+        imgs, at = [], 0
+        for _ in range(2 * (len(paths) + 1)):
+            W, H = int(vals[at]), int(vals[at + 1])
+            imgs.append(f32([int(x) for x in vals[at + 2:at + 2 + W * H]]).reshape(H, W))
+            at += 2 + W * H
+        counts = [int(x) for x in vals[at:]]
+        half = len(paths) + 1
+        for k in range(half):
+            assert np.array_equal(imgs[k].view(np.uint32), imgs[half + k].view(np.uint32)), (sem, k)
+        for k, oc in enumerate(ocs):
+            H, W = imgs[k].shape
+            v = O.make_view(W, H, 30.0, 20.0, [0.0 + k, -20.0 + 3.0 * k, 10.0], [10.0, 0, 0], 30.0, 300)
+            once, c1 = O.render_f32(oc, v, sem, nthreads=16)
+            twice, _ = O.render_f32(oc, v, sem, nthreads=16, init=once)
+            assert np.array_equal(imgs[k].view(np.uint32), twice.view(np.uint32)), (sem, k)
+            assert counts[k] == counts[half + k] == c1, (sem, k)
+        assert imgs[half - 1].any()  # the CMEM camera rendered
+
+
+# ---------------------------------------------------------------------------------- the CLIs
 # oracle/_ref/birp_main is the reference's own src/main.cpp, compiled unchanged against include/ and
-# linked to libbirp_api.so (oracle/Makefile; built only where the reference sources are present).
+# linked to libbirp_api.so (oracle/Makefile; built only in the development container, where the
+# reference sources are; it never reaches the GPU). birp_cpp_amd/bin/birp is this repository's own
+# CLI with the same flags (birp_cpp_amd/cli/birp_cli.cpp).
 CLI = os.path.join(ROOT, "oracle", "_ref", "birp_main")
 needs_cli = pytest.mark.skipif(not os.path.exists(CLI), reason="oracle/_ref/birp_main not built (no reference sources)")
+BIRP = os.path.join(ROOT, "birp_cpp_amd", "bin", "birp")
 
 
-def run_cli(*args, cwd=None):
-    r = subprocess.run([CLI, *map(str, args)], capture_output=True, text=True, cwd=cwd, timeout=120)
+def run_birp(*args, cwd=None):
+    r = subprocess.run([BIRP, *map(str, args)], capture_output=True, text=True, cwd=cwd, timeout=120)
     return r.returncode, r.stdout, r.stderr
 
 
@@ -306,18 +434,30 @@ def test_reference_cli_links_this_library():
     lib = [l for l in out.split("\n") if "libbirp_api.so" in l]
     assert lib and os.path.realpath(LIB) in os.path.realpath(lib[0].split("=>")[1].split("(")[0].strip())
     if not os.path.exists("/dev/kfd"):
-        rc, _, err = run_cli("-c", "-o", "/nonexistent/")
+        r = subprocess.run([CLI, "-c", "-o", "/nonexistent/"], capture_output=True, text=True, timeout=120)
+        assert r.returncode != 0 and "no GPU" in r.stderr
+
+
+def test_birp_cli_arguments_and_no_gpu():
+    """The CLI refuses a missing or unusable input like main.cpp (exit 3), and without a GPU a render
+    fails loudly (no CPU fallback)."""
+    assert os.path.exists(BIRP), "birp_cpp_amd/bin/birp is missing: run `make -C birp_cpp_amd`"
+    rc, so, err = run_birp()
+    assert rc == 3 and "Please provide valid .dat" in so + err
+    rc, so, err = run_birp("-i", "cube.txt")
+    assert rc == 3
+    if not os.path.exists("/dev/kfd"):
+        rc, so, err = run_birp("-c", "-o", "/nonexistent/")
         assert rc != 0 and "no GPU" in err
 
 
 @pytest.mark.gpu
-@needs_cli
-def test_reference_cli_datacube_render(device, tmp_path):
+def test_birp_cli_datacube_render(device, tmp_path):
     """main.cpp:266-305 single-cube mode: -i cube.dat -t, pose flags, default 0.25 deg pixels and
-    36 x 36 deg FOV; its FITS output equals the oracle's FP32 render bit for bit."""
+    36 x 36 deg FOV; the FITS output equals the oracle's FP32 render bit for bit."""
     oc = _shell_dat(tmp_path / "c.dat", 64, 12)
     out = str(tmp_path) + "/"
-    rc, so, err = run_cli("-i", tmp_path / "c.dat", "-t", "-x", 12, "-y", 4, "-z", 6, "-a", 1, "-o", out)
+    rc, so, err = run_birp("-i", tmp_path / "c.dat", "-t", "-x", 12, "-y", 4, "-z", 6, "-a", 1, "-o", out)
     assert rc == 0, so + err
     cards, img = _read_fits(out + "c.dat.fits")
     v = O.make_view(144, 144, 36.0, 36.0, [12.0, 4.0, 6.0], [1.0, 0, 0], 44, 200)
@@ -327,18 +467,20 @@ def test_reference_cli_datacube_render(device, tmp_path):
 
 
 @pytest.mark.gpu
-@needs_cli
-def test_reference_cli_batch_directory(device, tmp_path):
-    """main.cpp:242-264 batch mode: every .dat in the input directory is rendered (nearest
-    sampling without -t) and written as <name>.fits; each equals the oracle bit for bit."""
+@pytest.mark.parametrize("trilinear", [False, True])
+def test_birp_cli_batch_directory(device, tmp_path, trilinear):
+    """main.cpp:242-264 batch mode: every file of the input directory is rendered (nearest
+    sampling without -t) and written as <name>.fits, here all in one batched launch; each equals
+    the oracle bit for bit."""
     d = tmp_path / "cubes"
     d.mkdir()
-    ocs = {name: _shell_dat(d / name, n, seed) for name, n, seed in (("a.dat", 48, 21), ("b.dat", 40, 22))}
+    ocs = {name: _shell_dat(d / name, n, seed) for name, n, seed in (("a.dat", 48, 21), ("b.dat", 40, 22), ("c.dat", 64, 23))}
     for oc in ocs.values():
-        oc.c.trilinear = 0
+        oc.c.trilinear = int(trilinear)
     out = tmp_path / "out"
     out.mkdir()
-    rc, so, err = run_cli("-i", str(d) + "/", "-x", 9, "-y", 2, "-z", 3, "-a", 0.5, "-s", 0.5, "-o", str(out) + "/")
+    args = ["-i", str(d) + "/", "-x", 9, "-y", 2, "-z", 3, "-a", 0.5, "-s", 0.5, "-o", str(out) + "/"] + (["-t"] if trilinear else [])
+    rc, so, err = run_birp(*args)
     assert rc == 0, so + err
     v = O.make_view(72, 72, 36.0, 36.0, [9.0, 2.0, 3.0], [0.5, 0, 0], 44, 200)
     for name, oc in ocs.items():
@@ -348,18 +490,20 @@ def test_reference_cli_batch_directory(device, tmp_path):
 
 
 @pytest.mark.gpu
-@needs_cli
-def test_reference_cli_cmem_default_render(device, tmp_path):
-    """main.cpp:276-305 with -c and no other flag: CMEM through the CLI's parameter Init
-    (p = {1,1,3,4}, the NULL floats 0), the default pose (5.91813, 6.74116, 17.7067) aiming at
-    7.81768, 0.25 deg pixels over 36 x 36 deg. Checked by the CMEM zone-edge rule."""
+@pytest.mark.parametrize("flags,params", [
+    ([], dict(dipole=0.0, p0=1.0, p1=1, p2=3, p3=4, B=0.0, alpha=0.0, beta=0.0, bs=0.0, A1=0.0, A2=0.0)),
+    (["-p", "0.9,2,3,4", "-q", "2", "-r", "2.5", "-f", "-1.6"],
+     dict(dipole=0.0, p0=0.9, p1=2, p2=3, p3=4, B=2.0, alpha=2.5, beta=-1.6, bs=0.0, A1=0.0, A2=0.0))])
+def test_birp_cli_cmem_render(device, tmp_path, flags, params):
+    """main.cpp:276-305 with -c: CMEM through the CLI's parameter Init (with no flags p = {1,1,3,4}
+    and the NULL floats 0), the default pose (5.91813, 6.74116, 17.7067) aiming at 7.81768, 0.25 deg
+    pixels over 36 x 36 deg. Checked by the CMEM zone-edge rule."""
     out = str(tmp_path) + "/"
-    rc, so, err = run_cli("-c", "-o", out)
+    rc, so, err = run_birp("-c", *flags, "-o", out)
     assert rc == 0, so + err
     _, img = _read_fits(out + "cmembirp.fits")
-    oc = O.OracleCmem(dict(dipole=0.0, p0=1.0, p1=1, p2=3, p3=4, B=0.0, alpha=0.0, beta=0.0, bs=0.0, A1=0.0, A2=0.0))
+    oc = O.OracleCmem(params)
     v = O.make_view(144, 144, 36.0, 36.0, [5.91813, 6.74116, 17.7067], [7.81768, 0, 0], 44, 200)
     ref, _ = O.render_f32(oc, v, O.SEM_REFERENCE, nthreads=16)
     ref64, _ = O.render_f64(oc, v, O.SEM_REFERENCE, nthreads=16)
-    cmem_render_parity(img, ref, ref64, oc.edge_pixels(v, CMEM_EDGE_BAND))
-
+    cmem_render_parity(img, ref, ref64, *oc.edge_bound(v, CMEM_EDGE_BAND, nthreads=16))

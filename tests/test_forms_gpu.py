@@ -61,13 +61,38 @@ def test_cmem_points_and_render(device, oracle, params):
         g = vol.sample(P)
         r = np.array([oc.sample(*map(float, p)) for p in P], np.float32)
         close = np.isclose(g, r, rtol=2e-5, atol=1e-12)
-        assert close.mean() > 0.999, f"{(~close).sum()} of {len(P)} CMEM points differ"
+        # a point may differ by more only where a zone edge lies within the band, and then by at
+        # most the model's jump there
+        for k in np.nonzero(~close)[0]:
+            p = tuple(map(float, P[k]))
+            assert oc.edge_distance(*p) < CMEM_EDGE_BAND, (p, g[k], r[k])
+            assert np.isfinite(g[k]) and abs(float(g[k]) - float(r[k])) <= oc.edge_jump(*p, CMEM_EDGE_BAND) * 1.001 + 2e-5 * abs(float(r[k])), (p, g[k], r[k])
         v = W.view(144, 84, 0.25, pos=(1.0, 1.0, 1.0), aim=(1.0, 0.0, 0.0), length=44.0)
         img, st = vol.render(v)
         ov = oview(oracle, v)
         ref, _ = oracle.render_f32(oc, ov, oracle.SEM_EXTENDED, nthreads=NT)
         ref64, _ = oracle.render_f64(oc, ov, oracle.SEM_EXTENDED, nthreads=NT)
-        cmem_render_parity(img, ref, ref64, oc.edge_pixels(ov, CMEM_EDGE_BAND))
+        cmem_render_parity(img, ref, ref64, *oc.edge_bound(ov, CMEM_EDGE_BAND, nthreads=NT))
+        assert st.samples_in_cube == v.width * v.height * v.samples
+    finally:
+        vol.free()
+
+
+def test_cmem_render_576x336_800_samples(device, oracle):
+    """The CMEM render bench.py times (the default model, 576x336 pixels, 800 samples per ray over
+    80 RE from the workload camera): non-edge pixels within 1e-5 of the FP32 oracle, every edge
+    pixel within its zone-flip bound."""
+    oc = oracle.OracleCmem(None)
+    vol = capi.Volume.cmem(device, capi.cmem_state_from(oc.state()))
+    try:
+        v = W.view_576x336()
+        img, st = vol.render(v)
+        ov = oview(oracle, v)
+        ref, _ = oracle.render_f32(oc, ov, oracle.SEM_EXTENDED, nthreads=NT)
+        ref64, _ = oracle.render_f64(oc, ov, oracle.SEM_EXTENDED, nthreads=NT)
+        e32, e64, whole, frac, worst = cmem_render_parity(img, ref, ref64, *oc.edge_bound(ov, CMEM_EDGE_BAND, nthreads=NT))
+        print(f"CMEM 576x336x800: non-edge {e32:.3e} vs FP32, {e64:.3e} vs FP64; whole image {whole:.3e}; "
+              f"edge pixels {frac:.4f}, worst edge pixel at {worst:.3f} of its bound")
         assert st.samples_in_cube == v.width * v.height * v.samples
     finally:
         vol.free()

@@ -78,24 +78,72 @@ def test_cmem_f64_model_and_zone_edges(lib):
     assert m.edge_distance(8.0, 10.0, 10.0) > 1e-3 and m.edge_distance(3.0, 1.5, 1.2) > 1e-3
 
 
+def test_cmem_edge_jump_matches_the_model_across_each_edge(lib):
+    """or_cmem_edge_jump_f64 at a zone edge equals the FP64 model's own step there: along radial
+    lines (the zone radii depend on direction only), each discontinuity of or_cmem_sample_f64 is
+    bracketed by bisection to 1e-13 RE and the model's change across the bracket is compared with
+    the jump the function gives at the bracket's midpoint. Far from every edge the jump is 0."""
+    m = O.OracleCmem()
+    rng = np.random.default_rng(7)
+    checked = 0
+    for _ in range(12):
+        u = rng.normal(size=3)
+        u[0] = abs(u[0]) + 0.3  # sunward half, where the magnetopause and bow shock are closed
+        u /= np.linalg.norm(u)
+        f = lambda s: m.sample_f64(*(s * u))
+        grid = np.linspace(3.0, 40.0, 20001)
+        vals = np.array([f(s) for s in grid])
+        step = np.abs(np.diff(vals))
+        smooth = np.median(step) * 50 + 1e-30
+        for k in np.nonzero(step > np.maximum(smooth, 0.05 * np.abs(vals[1:])))[0][:4]:
+            lo, hi = grid[k], grid[k + 1]
+            for _ in range(80):
+                mid = 0.5 * (lo + hi)
+                if abs(f(mid) - f(lo)) < abs(f(hi) - f(mid)):
+                    lo = mid
+                else:
+                    hi = mid
+                if hi - lo < 1e-13:
+                    break
+            measured = abs(f(hi) - f(lo))
+            got = m.edge_jump(*(0.5 * (lo + hi) * u), 1e-9)
+            assert got == pytest.approx(measured, rel=1e-6, abs=1e-20), (u, lo, measured, got)
+            checked += measured > 0
+    assert checked >= 10
+    assert m.edge_jump(8.0, 10.0, 10.0, 1e-5) == 0.0 and m.edge_jump(3.0, 1.5, 1.2, 1e-5) == 0.0
+
+
 def test_cmem_parity_rule_is_not_vacuous(lib):
     """The CMEM render rule of the GPU tests accepts the FP32 reference against itself and
-    rejects an image with one pixel 1% off or an error hidden in the edge exclusion."""
+    rejects an image with one pixel 1% off, an edge pixel moved by more than its zone-flip bound,
+    or a NaN at an edge pixel; an edge pixel moved by one flip's worth passes."""
     from parity import CMEM_EDGE_BAND, cmem_render_parity
     m = O.OracleCmem()
     v = O.make_view(72, 72, 36.0, 36.0, [1.0, 1.0, 1.0], [1.0, 0, 0], 44, 200)
     a, _ = O.render_f32(m, v, O.SEM_REFERENCE, nthreads=8)
     b, _ = O.render_f64(m, v, O.SEM_REFERENCE, nthreads=8)
-    mask = m.edge_pixels(v, CMEM_EDGE_BAND)
-    assert 0 < mask.sum() < 0.02 * mask.size
-    cmem_render_parity(a, a, b, mask)
+    mask, bound = m.edge_bound(v, CMEM_EDGE_BAND, nthreads=8)
+    assert np.array_equal(mask, m.edge_pixels(v, CMEM_EDGE_BAND))
+    assert 0 < mask.sum() < 0.02 * mask.size and (bound[mask] > 0).any() and not bound[~mask].any()
+    cmem_render_parity(a, a, b, mask, bound)
     j, i = np.unravel_index(np.argmax(np.where(mask, 0, a)), a.shape)
     bad = a.copy()
     bad[j, i] *= 1.01
     with pytest.raises(AssertionError):
-        cmem_render_parity(bad, a, b, mask)
+        cmem_render_parity(bad, a, b, mask, bound)
+    je, ie = np.unravel_index(np.argmax(np.where(mask, bound, 0)), a.shape)
+    flip = a.copy()
+    flip[je, ie] += bound[je, ie]  # one allowed flip's worth
+    cmem_render_parity(flip, a, b, mask, bound)
+    flip[je, ie] += 0.1 * bound[je, ie] + 1e-3 * abs(a[je, ie])
     with pytest.raises(AssertionError):
-        cmem_render_parity(a, a, b, np.ones_like(mask))
+        cmem_render_parity(flip, a, b, mask, bound)
+    nan = a.copy()
+    nan[je, ie] = np.nan
+    with pytest.raises(AssertionError):
+        cmem_render_parity(nan, a, b, mask, bound)
+    with pytest.raises(AssertionError):
+        cmem_render_parity(a, a, b, np.ones_like(mask), bound)
 
 
 # ------------------------------------------------------------------------- golden vectors (reference code)
