@@ -108,6 +108,7 @@ SIGNATURES = [
     ("birp_image_download", _i, [_P, _FP]),
     ("birp_image_upload", _i, [_P, _FP]),
     ("birp_render_async", _i, [_P, C.POINTER(View), _P]),
+    ("birp_render_views_noalloc", _i, [_P, C.POINTER(View), _i, C.POINTER(C.c_void_p)]),
     ("birp_count_samples", _i, [_P, C.POINTER(View), C.POINTER(C.c_longlong)]),
     ("birp_render_f64", _i, [_P, C.POINTER(View), _DP]),
     ("birp_view_basis", _i, [C.POINTER(View), _FP, _FP, _FP]),

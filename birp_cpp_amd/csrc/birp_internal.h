@@ -165,6 +165,22 @@ inline int fast_kernel_dtype(const DevVolume& d) {
     return base | (d.vclass != VC_POSITIVE ? BIRP_KDT_SIGNED : 0);
 }
 int fast_grid_size(int dtype, int lanes_per_ray, int unroll);
+
+// birp_render_views_noalloc: its fast launches read their descriptors from a fixed device array
+// (render.hip) laid out as [DevVolume | FastPose x BIRP_NOALLOC_MAX_VIEWS | tile prefix sums of the
+// three lanes-per-ray groups | work-queue counters of the three groups, 256 each].
+#define BIRP_NOALLOC_MAX_VIEWS 64
+#define BIRP_NOALLOC_POSE_OFF 1024
+#define BIRP_NOALLOC_PREFIX_OFF (BIRP_NOALLOC_POSE_OFF + 256 * BIRP_NOALLOC_MAX_VIEWS)
+#define BIRP_NOALLOC_CTR_OFF (BIRP_NOALLOC_PREFIX_OFF + 1024)
+#define BIRP_NOALLOC_SCRATCH_BYTES (BIRP_NOALLOC_CTR_OFF + 3 * 256 * 4)
+// The device address of this GPU's array (the current device's), from hipGetSymbolAddress.
+int noalloc_scratch_address(void** out);
+// Stage the volume, the prefix sums (3 groups x (BIRP_NOALLOC_MAX_VIEWS + 1) ints) and cleared
+// counters, then poses [first, first + count) (count <= 4 per call), into the array at `scratch`
+// by kernels on `stream` (their arguments carry the data: no copy from host memory).
+int noalloc_stage_head(const DevVolume& v, const int* prefix, void* scratch, void* stream);
+int noalloc_stage_poses(const FastPose* poses, int first, int count, void* scratch, void* stream);
 // The grid of one persistent launch over `tiles` tiles: the occupancy-limited grid, or about one
 // wave per tile when that is smaller (idle waves cost serialized work-queue claims on exit).
 int fast_grid_for(int dtype, int lanes_per_ray, int unroll, int tiles);

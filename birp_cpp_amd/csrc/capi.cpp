@@ -255,6 +255,10 @@ extern "C" int birp_device_open(int ordinal, birp_device** out) {
              (e = hipEventCreate(&d->ev_a)) != hipSuccess || (e = hipEventCreate(&d->ev_b)) != hipSuccess)
         what = "hipEventCreate";
     else if ((e = hipMalloc(&d->d_counter, 64)) != hipSuccess) what = "hipMalloc(counter)";
+    else if (noalloc_scratch_address(&d->noalloc_scratch) != BIRP_OK) {
+        e = hipErrorInvalidSymbol;
+        what = "hipGetSymbolAddress(scratch)";
+    }
     if (what) {
         int rc = hip_fail(e, what);
         birp_device_close(d);
@@ -1308,6 +1312,72 @@ extern "C" int birp_batch_stats(birp_batch* b, long long* nominal, double* last_
     if (!b) return fail(BIRP_ERR_INVALID_ARGUMENT, "null batch");
     if (nominal) *nominal = b->nominal;
     if (last_ms) *last_ms = b->last_ms;
+    return BIRP_OK;
+}
+
+// birp_render_views_noalloc: birp_render_async of each view without allocating. Exact views launch
+// the exact kernel (it needs no scratch); fast views are grouped by the lanes per ray of their
+// latency tuning (the tuning birp_render_async uses, so the images are the same bit for bit) and
+// each group is one persistent launch whose volume, poses, prefix sums and work counters are
+// staged by kernels into the device's fixed scratch array.
+extern "C" int birp_render_views_noalloc(birp_volume* vol, const birp_view* views, int count, float* const* images_dev) {
+    if (!vol || !views || !images_dev || count <= 0) return fail(BIRP_ERR_INVALID_ARGUMENT, "bad argument");
+    if (count > BIRP_NOALLOC_MAX_VIEWS)
+        return fail(BIRP_ERR_UNSUPPORTED, "birp_render_views_noalloc takes at most 64 views per call (its fixed scratch holds 64 poses)");
+    birp_device* dev = vol->dev;
+    if (!dev->noalloc_scratch) return fail(BIRP_ERR_DEVICE, "the device's scratch array was not set up");
+    for (int k = 0; k < count; k++) {
+        int rc = validate_view(&views[k], views[k].math == BIRP_MATH_EXACT);
+        if (rc) return rc;
+        if (!images_dev[k]) return fail(BIRP_ERR_INVALID_ARGUMENT, "null image");
+        if (views[k].math == BIRP_MATH_FAST && (views[k].flags & BIRP_RENDER_ACCUMULATE))
+            return fail(BIRP_ERR_UNSUPPORTED, "fast renders do not accumulate");
+    }
+    (void)hipSetDevice(dev->ordinal);
+    FastPose poses[BIRP_NOALLOC_MAX_VIEWS];
+    int prefix[3 * (BIRP_NOALLOC_MAX_VIEWS + 1)] = {};
+    int nfast[3] = {0, 0, 0};
+    birp_tuning gt[3] = {};
+    int nf = 0;
+    for (int g = 0; g < 3; g++) {  // lanes per ray 1, 2, 4: poses of group g are contiguous
+        for (int k = 0; k < count; k++) {
+            if (views[k].math != BIRP_MATH_FAST) continue;
+            const birp_tuning t = latency_tuning(views[k]);
+            if (t.lanes_per_ray != (1 << g)) continue;
+            int rc = fast_pose(vol, &views[k], 0, images_dev[k], t.tile_w, t.tile_h, poses[nf]);
+            if (rc) return rc;
+            int* pre = prefix + g * (BIRP_NOALLOC_MAX_VIEWS + 1);
+            const long long next = (long long)pre[nfast[g]] + tiles_of(views[k], t.tile_w, t.tile_h);
+            if (next > INT32_MAX) return fail(BIRP_ERR_UNSUPPORTED, "more than 2^31 - 1 tiles");
+            pre[nfast[g] + 1] = (int)next;
+            nfast[g]++;
+            gt[g] = t;
+            nf++;
+        }
+    }
+    if (take_injected_failure(dev)) return fail(BIRP_ERR_INJECTED, "injected failure");
+    int rc = BIRP_OK;
+    for (int k = 0; k < count && rc == BIRP_OK; k++)
+        if (views[k].math == BIRP_MATH_EXACT) rc = render_exact_into(vol, &views[k], images_dev[k], nullptr);
+    if (rc || nf == 0) return rc;
+    char* s = static_cast<char*>(dev->noalloc_scratch);
+    if ((rc = noalloc_stage_head(vol->dv, prefix, s, dev->stream))) return fail(rc, "scratch staging failed");
+    for (int q = 0; q < nf; q += 4)
+        if ((rc = noalloc_stage_poses(poses + q, q, std::min(4, nf - q), s, dev->stream))) return fail(rc, "scratch staging failed");
+    const int kdt = fast_kernel_dtype(vol->dv);
+    int first = 0;
+    for (int g = 0; g < 3; g++) {
+        if (!nfast[g]) continue;
+        const int* pre_dev = reinterpret_cast<const int*>(s + BIRP_NOALLOC_PREFIX_OFF) + g * (BIRP_NOALLOC_MAX_VIEWS + 1);
+        const int total = prefix[g * (BIRP_NOALLOC_MAX_VIEWS + 1) + nfast[g]];
+        const birp_tuning& t = gt[g];
+        const int grid = fast_grid_for(kdt, t.lanes_per_ray, t.unroll, total);
+        rc = launch_fast_batch(reinterpret_cast<const DevVolume*>(s), kdt, reinterpret_cast<const FastPose*>(s + BIRP_NOALLOC_POSE_OFF) + first,
+                               pre_dev, nfast[g], total, reinterpret_cast<unsigned*>(s + BIRP_NOALLOC_CTR_OFF) + g * 256, t.tile_w, t.tile_h,
+                               t.lanes_per_ray, t.unroll, grid, dev->stream);
+        if (rc) return fail(rc, "fast launch failed");
+        first += nfast[g];
+    }
     return BIRP_OK;
 }
 

@@ -36,6 +36,7 @@ struct birp_device {
     unsigned long long count_base = 0; // the counter's value at its last read (it only grows)
     bool count_known = false;          // false: clear the counter before its next use
     std::atomic<int> inject_failures{0};
+    void* noalloc_scratch = nullptr;   // this GPU's fixed scratch array (birp_render_views_noalloc)
 };
 
 struct birp_volume {

@@ -1131,6 +1131,71 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
 }
 
 // ------------------------------------------------------------------------------------------
+// the fixed scratch of birp_render_views_noalloc
+// ------------------------------------------------------------------------------------------
+// Each GPU holds its own copy for the life of the process. In host code `scratch` names a host-side
+// stand-in, not a device address: host code uses only the address noalloc_scratch_address returns.
+static __device__ __align__(256) unsigned char scratch[BIRP_NOALLOC_SCRATCH_BYTES];
+
+static_assert(sizeof(DevVolume) <= BIRP_NOALLOC_POSE_OFF, "noalloc scratch: volume slot");
+static_assert(BIRP_NOALLOC_SCRATCH_BYTES <= BIRP_NOALLOC_SCRATCH_KIB * 1024, "noalloc scratch: stated size");
+static_assert(sizeof(FastPose) <= 256, "noalloc scratch: pose slot");
+static_assert(3 * (BIRP_NOALLOC_MAX_VIEWS + 1) * 4 <= BIRP_NOALLOC_CTR_OFF - BIRP_NOALLOC_PREFIX_OFF, "noalloc scratch: prefix");
+
+struct NoallocHead {
+    DevVolume v;
+    int prefix[3 * (BIRP_NOALLOC_MAX_VIEWS + 1)];
+};
+struct NoallocPoses {
+    FastPose p[4];
+    int first, count;
+};
+
+__global__ __launch_bounds__(BIRP_BLOCK) void k_noalloc_head(NoallocHead h, unsigned char* s) {
+    if (threadIdx.x == 0) *reinterpret_cast<DevVolume*>(s) = h.v;
+    int* pre = reinterpret_cast<int*>(s + BIRP_NOALLOC_PREFIX_OFF);
+    for (int i = threadIdx.x; i < 3 * (BIRP_NOALLOC_MAX_VIEWS + 1); i += BIRP_BLOCK) pre[i] = h.prefix[i];
+    unsigned* ctr = reinterpret_cast<unsigned*>(s + BIRP_NOALLOC_CTR_OFF);
+    for (int i = threadIdx.x; i < 3 * 256; i += BIRP_BLOCK) ctr[i] = 0u;
+}
+
+__global__ void k_noalloc_poses(NoallocPoses c, unsigned char* s) {
+    FastPose* dst = reinterpret_cast<FastPose*>(s + BIRP_NOALLOC_POSE_OFF) + c.first;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        if (threadIdx.x == q && q < c.count) dst[q] = c.p[q];
+}
+
+int noalloc_scratch_address(void** out) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return BIRP_ERR_DEVICE;
+    }
+    *out = p;
+    return BIRP_OK;
+}
+
+int noalloc_stage_head(const DevVolume& v, const int* prefix, void* s, void* stream) {
+    NoallocHead h;
+    h.v = v;
+    for (int i = 0; i < 3 * (BIRP_NOALLOC_MAX_VIEWS + 1); i++) h.prefix[i] = prefix[i];
+    hipLaunchKernelGGL(k_noalloc_head, dim3(1), dim3(BIRP_BLOCK), 0, (hipStream_t)stream, h, (unsigned char*)s);
+    return hipGetLastError() == hipSuccess ? BIRP_OK : BIRP_ERR_DEVICE;
+}
+
+int noalloc_stage_poses(const FastPose* poses, int first, int count, void* s, void* stream) {
+    if (count < 1 || count > 4 || first < 0 || first + count > BIRP_NOALLOC_MAX_VIEWS) return BIRP_ERR_INVALID_ARGUMENT;
+    NoallocPoses c;
+    for (int q = 0; q < count; q++) c.p[q] = poses[q];
+    c.first = first;
+    c.count = count;
+    hipLaunchKernelGGL(k_noalloc_poses, dim3(1), dim3(64), 0, (hipStream_t)stream, c, (unsigned char*)s);
+    return hipGetLastError() == hipSuccess ? BIRP_OK : BIRP_ERR_DEVICE;
+}
+
+// ------------------------------------------------------------------------------------------
 // host-side launch wrappers
 // ------------------------------------------------------------------------------------------
 static int hip_status(hipError_t e) { return e == hipSuccess ? BIRP_OK : BIRP_ERR_DEVICE; }

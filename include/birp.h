@@ -185,6 +185,21 @@ int birp_image_download(birp_image* img, float* host); /* waits for the device's
 int birp_image_upload(birp_image* img, const float* host);
 /* Enqueue one render into img on the device's stream; no host synchronisation. */
 int birp_render_async(birp_volume* vol, const birp_view* view, birp_image* img);
+/* No reference counterpart. For callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): birp_render_async of each of `count` views of
+ * one volume into images_dev[k] (device memory the caller owns, views[k].height * width floats),
+ * the same images bit for bit, with no allocation of any kind. Exact views run the exact kernel;
+ * fast views are grouped by lanes per ray and each group is one persistent launch whose volume,
+ * poses, tile prefix sums and work-queue counters live in a fixed device array of the library,
+ * BIRP_NOALLOC_SCRATCH_KIB per GPU, set up by birp_device_open and held until the process ends. At
+ * most BIRP_NOALLOC_MAX_VIEWS views per call (BIRP_ERR_UNSUPPORTED beyond: the array holds that
+ * many poses). Asynchronous on
+ * the device's stream, like birp_render_async. Every call on a GPU shares that GPU's array, which
+ * keeps the last call's descriptors (the counters are cleared by each call): calls on one device
+ * from different threads or streams that overlap race on it. */
+#define BIRP_NOALLOC_SCRATCH_KIB 21
+#define BIRP_NOALLOC_MAX_VIEWS 64
+int birp_render_views_noalloc(birp_volume* vol, const birp_view* views, int count, float* const* images_dev);
 /* The exact in-cube sample count of a view (for throughput accounting; runs a counting kernel). */
 int birp_count_samples(birp_volume* vol, const birp_view* view, long long* in_cube);
 
