@@ -9,6 +9,9 @@ Also measured in the same run (keys under "forms"):
   batched               256 orbit poses, mixed 144x84/288x168/576x336 images, steps 0.1/0.25 RE,
                         F16 storage, one persistent-kernel launch replayed from a HIP graph; plain
                         renders/s of the same 256 views for the ratio
+  exact_batched         the same 256 views, and 8 copies of the headline view, as exact batches
+                        (bit-identical images, one launch per sample mode) against back-to-back
+                        single exact launches of the same views
   cmem                  the analytic CMEM model (default Init) on the same 576x336 x 800 grid:
                         samples/s of the exact path (VALU-bound) and its error against FP64
   precision             512^3 cube stored FP32/BF16/F16 (one or per-brick power-of-two scale),
@@ -278,6 +281,47 @@ def bench_batched(capi, W, dev, vals, org, spc, fp32_vol):
             "roofline": valu_roofline("batched_f16_256cube_orbit256_valu_pmc.txt", last_ms)}
 
 
+def bench_exact_batched(capi, W, dev, fp32_vol, view0, reps=10):
+    """Exact renders of many views in one launch per sample mode (birp_batch_create_exact), each image
+    bit-identical to its single exact render: the 256 orbit views of the batched workload, and 8
+    copies of the headline view; against the same views as back-to-back single exact launches on
+    the device (birp_render_async into device images)."""
+    out = {}
+    for name, views in (("orbit256", W.orbit_views_mixed(256, math=capi.MATH_EXACT)), ("headline_x8", [view0] * 8)):
+        b = capi.Batch.exact([fp32_vol], views)
+        for _ in range(2):
+            b.launch()
+            b.wait()
+        t = time.perf_counter()
+        dms = []
+        for _ in range(reps):
+            b.launch()
+            b.wait()
+            dms.append(b.stats()[1])
+        el = (time.perf_counter() - t) / reps
+        in_cube = sum(b.samples())
+        b.free()
+        imgs = [capi.Image(dev, v.width, v.height) for v in views]
+        for v, img in zip(views, imgs):
+            capi.render_async(fp32_vol, v, img)
+        dev.synchronize()
+        dev.timer_begin()
+        for _ in range(reps):
+            for v, img in zip(views, imgs):
+                capi.render_async(fp32_vol, v, img)
+        single_ms = dev.timer_end() / reps
+        for img in imgs:
+            img.free()
+        dev_ms = float(np.median(dms))
+        out[name] = {"views": len(views), "device_ms_per_batch": round(dev_ms, 4), "renders_per_s": round(len(views) / el, 1),
+                     "single_launches_device_ms": round(single_ms, 4), "speedup_vs_single_launches": round(single_ms / dev_ms, 3),
+                     "in_cube_samples": int(in_cube), "in_cube_gsamples_per_s_device": round(in_cube / (dev_ms * 1e-3) / 1e9, 1)}
+    prof = valu_roofline("exact_batched_256cube_orbit256_valu_pmc.txt", out["orbit256"]["device_ms_per_batch"])
+    if prof:
+        out["orbit256"]["roofline"] = prof
+    return out
+
+
 def bench_cmem(capi, W, dev, steps=20):
     """The analytic CMEM model (default Init) rendered by the exact path over the 576x336 image
     and ray grid: every sample is evaluated (the model has no bounds), about 25 transcendentals
@@ -429,7 +473,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--forms", default="batched,cmem,precision,serving", help="extra forms to measure ('' for none)")
+    ap.add_argument("--forms", default="batched,exact_batched,cmem,precision,serving", help="extra forms to measure ('' for none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle checks")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (torch) to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -493,6 +537,9 @@ def main():
     if "batched" in forms:
         log("batched: 256 orbit poses, F16")
         line["forms"]["batched"] = bench_batched(capi, W, dev, p["vals"], p["org"], p["spc"], p["vol"])
+    if "exact_batched" in forms:
+        log("exact batched: 256 orbit poses and 8 headline views, one launch each")
+        line["forms"]["exact_batched"] = bench_exact_batched(capi, W, dev, p["vol"], v)
     p["vol"].free()
     cmem_img = None
     if "cmem" in forms:

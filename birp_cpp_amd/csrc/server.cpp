@@ -14,9 +14,11 @@
 // pinned staging the same way, and when a slot's device work is done a completion thread copies
 // those into the callers' buffers (split over a pool of helper threads for large batches) and
 // releases the slot, so the dispatcher keeps launching meanwhile.
-// Every request is rendered with the tuning its own view selects (the one fast single renders
+// Every cube request is rendered with the tuning its own view selects (the one fast single renders
 // use) and the same per-tile arithmetic whatever it is batched with, so results do not depend on
-// the mix and equal birp_render's.
+// the mix and equal birp_render's. Requests on CMEM models render with the exact arithmetic (the
+// model's only form) through the multi-view exact kernel, in the same slot launch, and equal
+// birp_render's exact images bit for bit.
 // No reference counterpart: the reference renders one pose per process run (main.cpp:266-316).
 #include <hip/hip_runtime_api.h>
 
@@ -183,6 +185,10 @@ struct Slot {
     int* h_prefix = nullptr;      // pinned, max_batch + kGroups tile prefix sums
     int* d_prefix = nullptr;
     unsigned* d_work = nullptr;   // work-queue counters, 256 per kernel group
+    ExactView* h_evs = nullptr;   // pinned: the exact (CMEM) requests' view constants
+    ExactView* d_evs = nullptr;
+    int* h_eprefix = nullptr;     // pinned: their tile prefix sums (max_batch + 1)
+    int* d_eprefix = nullptr;
     float* d_images = nullptr;
     float* h_images = nullptr;    // pinned
     float* h_images_dev = nullptr;  // its device address: kernels write the staged images there
@@ -268,6 +274,9 @@ int request_group(const Request* r, birp_tuning* t) {
     *t = latency_tuning(r->req.view);
     return fast_kernel_dtype(r->vol->dv) * 3 + lane_index(t->lanes_per_ray);
 }
+// CMEM requests render with the exact arithmetic of birp_render (the model's only form) through the
+// multi-view exact kernel; cube requests take the fast kernels.
+bool exact_request(const Request* r) { return r->vol->dv.kind == SK_CMEM; }
 
 // Launch the admitted requests of slot s (dispatcher thread, lock not held). Group g's poses are
 // h_poses[base[g], base[g+1]) and its tile prefix sums h_prefix[base[g] + g, ... + count_g].
@@ -283,6 +292,7 @@ int launch_slot(birp_server* S, Slot& s) {
     int base[kGroups + 1] = {}, fill[kGroups];
     birp_tuning gt[kGroups];
     for (Request* r : s.reqs) {
+        if (exact_request(r)) continue;
         birp_tuning t;
         const int g = request_group(r, &t);
         base[g + 1]++;
@@ -292,13 +302,26 @@ int launch_slot(birp_server* S, Slot& s) {
         base[g + 1] += base[g];
         fill[g] = base[g];
     }
+    int nexact = 0;
+    size_t exact_lds = 0;
+    s.h_eprefix[0] = 0;
     for (size_t k = 0; k < s.reqs.size(); k++) {
         Request* r = s.reqs[k];
-        birp_tuning t;
-        const int g = request_group(r, &t);
         // a result in a birp_host_alloc buffer is written there by the kernel (zero-copy), the others
         // into the slot's pinned staging (or its device images when the staging is not mapped)
         float* out = r->dev_out ? r->dev_out : (s.h_images_dev ? s.h_images_dev : s.d_images) + s.offsets[k];
+        if (exact_request(r)) {
+            ExactView& ev = s.h_evs[nexact];
+            if ((rc = exact_view(r->vol, &r->req.view, out, ev))) return rc;
+            ev.vol = r->req.volume;
+            const int mode = exact_mode_of(r->vol->dv, ev);
+            exact_lds = std::max(exact_lds, exact_lds_bytes(r->vol->dv, r->req.view.samples, mode));
+            s.h_eprefix[nexact + 1] = s.h_eprefix[nexact] + exact_tiles(r->req.view.width, r->req.view.height, exact_multi_lanes(mode));
+            nexact++;
+            continue;
+        }
+        birp_tuning t;
+        const int g = request_group(r, &t);
         rc = fast_pose(r->vol, &r->req.view, r->req.volume, out, t.tile_w, t.tile_h, s.h_poses[fill[g]++]);
         if (rc) return rc;
     }
@@ -315,9 +338,15 @@ int launch_slot(birp_server* S, Slot& s) {
         }
     }
     const int n = base[kGroups];
-    hipError_t e = hipMemcpyAsync(s.d_poses, s.h_poses, sizeof(FastPose) * n, hipMemcpyHostToDevice, s.stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(s.d_prefix, s.h_prefix, sizeof(int) * (n + kGroups), hipMemcpyHostToDevice, s.stream);
+    hipError_t e = hipSuccess;
+    if (n) e = hipMemcpyAsync(s.d_poses, s.h_poses, sizeof(FastPose) * n, hipMemcpyHostToDevice, s.stream);
+    if (e == hipSuccess && n) e = hipMemcpyAsync(s.d_prefix, s.h_prefix, sizeof(int) * (n + kGroups), hipMemcpyHostToDevice, s.stream);
+    if (e == hipSuccess && nexact) e = hipMemcpyAsync(s.d_evs, s.h_evs, sizeof(ExactView) * nexact, hipMemcpyHostToDevice, s.stream);
+    if (e == hipSuccess && nexact)
+        e = hipMemcpyAsync(s.d_eprefix, s.h_eprefix, sizeof(int) * (nexact + 1), hipMemcpyHostToDevice, s.stream);
     if (e != hipSuccess) return hip_fail(e, "server descriptor upload");
+    if (nexact && (rc = launch_exact_multi(S->d_vols, s.d_evs, s.d_eprefix, nexact, s.h_eprefix[nexact], SM_CMEM, exact_lds, s.stream)))
+        return fail(rc, "exact launch failed");
     for (int g = 0; g < kGroups; g++) {
         const int count = base[g + 1] - base[g];
         if (!count) continue;
@@ -571,6 +600,10 @@ extern "C" int birp_server_create(birp_device* dev, const birp_server_config* cf
         if ((e = hipHostMalloc(&s.h_prefix, sizeof(int) * (nb + kGroups), hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
         if ((e = hipMalloc(&s.d_prefix, sizeof(int) * (nb + kGroups))) != hipSuccess) { rc = hip_fail(e, "prefix"); break; }
         if ((e = hipMalloc(&s.d_work, sizeof(unsigned) * kGroups * 256)) != hipSuccess) { rc = hip_fail(e, "counter"); break; }
+        if ((e = hipHostMalloc(&s.h_evs, sizeof(ExactView) * nb, hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
+        if ((e = hipMalloc(&s.d_evs, sizeof(ExactView) * nb)) != hipSuccess) { rc = hip_fail(e, "views"); break; }
+        if ((e = hipHostMalloc(&s.h_eprefix, sizeof(int) * (nb + 1), hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
+        if ((e = hipMalloc(&s.d_eprefix, sizeof(int) * (nb + 1))) != hipSuccess) { rc = hip_fail(e, "prefix"); break; }
         if (cfg->reserve_pixels) {
             if ((rc = slot_alloc_images(s, cfg->reserve_pixels))) break;
             s.held = cfg->reserve_pixels * 4;
@@ -617,6 +650,10 @@ extern "C" int birp_server_destroy(birp_server* S) {
         if (s.h_prefix) (void)hipHostFree(s.h_prefix);
         if (s.d_prefix) (void)hipFree(s.d_prefix);
         if (s.d_work) (void)hipFree(s.d_work);
+        if (s.h_evs) (void)hipHostFree(s.h_evs);
+        if (s.d_evs) (void)hipFree(s.d_evs);
+        if (s.h_eprefix) (void)hipHostFree(s.h_eprefix);
+        if (s.d_eprefix) (void)hipFree(s.d_eprefix);
         if (s.done) (void)hipEventDestroy(s.done);
         if (s.stream) (void)hipStreamDestroy(s.stream);
     }
@@ -629,7 +666,6 @@ extern "C" int birp_server_destroy(birp_server* S) {
 extern "C" int birp_server_add_volume(birp_server* S, birp_volume* vol, int* index) {
     if (!S || !vol || !index) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
     if (vol->dev != S->dev) return fail(BIRP_ERR_INVALID_ARGUMENT, "volume on another device");
-    if (vol->dv.kind != SK_CUBE) return fail(BIRP_ERR_UNSUPPORTED, "the server renders cube volumes");
     std::lock_guard<std::mutex> lk(S->mu);
     if ((int)S->vols.size() >= S->vol_cap) return fail(BIRP_ERR_OUT_OF_MEMORY, "volume table full");
     if (S->resident + S->held + vol->bytes > S->cfg.memory_budget)
@@ -644,10 +680,15 @@ extern "C" int birp_server_add_volume(birp_server* S, birp_volume* vol, int* ind
 
 extern "C" int birp_server_submit(birp_server* S, const birp_request* req, float* result, uint64_t* id) {
     if (!S || !req || !result || !id) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
+    bool cmem = false;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (req->volume >= 0 && req->volume < (int)S->vols.size()) cmem = S->vols[req->volume]->dv.kind == SK_CMEM;
+    }
     birp_view v = req->view;
-    v.math = BIRP_MATH_FAST;
+    v.math = cmem ? BIRP_MATH_EXACT : BIRP_MATH_FAST;
     v.flags = 0;
-    int rc = validate_view(&v, false);
+    int rc = validate_view(&v, cmem);
     if (rc) return rc;
     Request* r = new Request();
     r->req = *req;

@@ -283,7 +283,8 @@ typedef struct birp_server_config {
 
 typedef struct birp_request {
     int volume;     /* index returned by birp_server_add_volume */
-    birp_view view; /* math is forced to BIRP_MATH_FAST */
+    birp_view view; /* cube volumes render with BIRP_MATH_FAST; CMEM models with BIRP_MATH_EXACT
+                       (the model's only form, bit-identical to birp_render); flags are ignored */
 } birp_request;
 
 typedef struct birp_result_info {
@@ -296,7 +297,8 @@ typedef struct birp_result_info {
 
 int birp_server_create(birp_device* dev, const birp_server_config* cfg, birp_server** out);
 int birp_server_destroy(birp_server* s);
-/* Registers a resident volume (owned by the caller; must outlive the server). Its device bytes
+/* Registers a resident volume, a cube or a CMEM model (owned by the caller; must outlive the
+ * server). Its device bytes
  * count against the budget; fails with BIRP_ERR_ADMISSION when it does not fit beside the
  * volumes already registered and the image buffers the slots hold. A request whose image alone
  * exceeds the budget left after the volumes completes with BIRP_ERR_ADMISSION. */

@@ -283,6 +283,39 @@ def test_autotune_roundtrip(cube1, tmp_path):
 
 
 # ---------------------------------------------------------------------------------- server
+def test_server_serves_cmem_models_beside_cubes(device, cube1):
+    """CMEM models are resident volumes too: their requests render with the exact arithmetic in the
+    same slot launches as cube requests, each result bitwise equal to birp_render of its view (exact
+    for CMEM, fast for the cubes), in launches that mix both."""
+    vals, org, spc, vols, oc = cube1
+    cms = [capi.Volume.cmem(device, capi.cmem_init()),
+           capi.Volume.cmem(device, capi.cmem_init(capi.cmem_params(p0=1.0, p1=4, p2=6, p3=3, B=3.6, alpha=-1.0)))]
+    srv = capi.Server(device, memory_budget=4 << 30, max_batch=24, max_inflight=2, batch_window_ms=0.5)
+    try:
+        idx = [srv.add_volume(vols[capi.DTYPE_F16]), srv.add_volume(cms[0]), srv.add_volume(cms[1]), srv.add_volume(vols[capi.DTYPE_F32])]
+        spaces = [vols[capi.DTYPE_F16], cms[0], cms[1], vols[capi.DTYPE_F32]]
+        views = W.orbit_views_mixed(count=40, seed=13)
+        which = [k % 4 for k in range(len(views))]
+        ids = [srv.submit(idx[w], v) for v, w in zip(views, which)]
+        mixed = 0
+        for k, rid in enumerate(ids):
+            st, img, info = srv.wait(rid, 60000.0)
+            assert st == capi.OK
+            mixed += info.batch_size > 1
+            v = capi.make_view(views[k].width, views[k].height, views[k].fov_w, views[k].fov_h, tuple(views[k].position),
+                               tuple(views[k].aim), views[k].ray_length, views[k].samples, views[k].semantics,
+                               capi.MATH_EXACT if which[k] in (1, 2) else capi.MATH_FAST)
+            want, _ = spaces[which[k]].render(v)
+            assert np.array_equal(img.view(np.uint32), want.view(np.uint32)), (k, which[k])
+            if which[k] in (1, 2):
+                assert img.any()
+        assert mixed > 0
+    finally:
+        srv.destroy()
+        for c in cms:
+            c.free()
+
+
 def test_server_alone_equals_in_mix(device, cube1):
     vals, org, spc, vols, oc = cube1
     srv = capi.Server(device, memory_budget=4 << 30, max_batch=16, max_inflight=3)
