@@ -157,10 +157,14 @@ int launch_build_oct(const DevVolume& v, void* oct, float* bscale, void* stream)
 // plus BIRP_KDT_SIGNED for volumes with values <= 0, whose kernels stop each ray at its first
 // negative sample (Camera.cpp:207-210) and take one sample per lane per trip. Positive volumes
 // use the kernels without the stop test.
+// BIRP_KDT_CMEM is the analytic model in the fast form (FP32 with the hardware exp2/log2 and
+// polynomial arc cosines; it also stops rays at negative samples).
 #define BIRP_KDT_F16_UNIFORM 3
 #define BIRP_KDT_SIGNED 4
-#define BIRP_KDT_COUNT 8
+#define BIRP_KDT_CMEM 8
+#define BIRP_KDT_COUNT 9
 inline int fast_kernel_dtype(const DevVolume& d) {
+    if (d.kind == SK_CMEM) return BIRP_KDT_CMEM;
     const int base = (d.dtype == BIRP_DTYPE_F16 && d.f16_uniform) ? BIRP_KDT_F16_UNIFORM : d.dtype;
     return base | (d.vclass != VC_POSITIVE ? BIRP_KDT_SIGNED : 0);
 }

@@ -106,8 +106,8 @@ int exact_view(const birp_volume* vol, const birp_view* v, float* out_dev, Exact
 }
 
 int fast_pose(const birp_volume* vol, const birp_view* v, int vol_index, float* out_dev, int tile_w, int tile_h, FastPose& fp) {
-    if (vol->dv.kind != SK_CUBE || (vol->desc.dtype == BIRP_DTYPE_F32 ? !vol->dv.quad : !vol->dv.oct) ||
-        vol->dv.nx + vol->dv.ny + vol->dv.nz > 2048)
+    if (vol->dv.kind == SK_CUBE && ((vol->desc.dtype == BIRP_DTYPE_F32 ? !vol->dv.quad : !vol->dv.oct) ||
+                                    vol->dv.nx + vol->dv.ny + vol->dv.nz > 2048))
         return fail(BIRP_ERR_UNSUPPORTED, "fast renders need a bricked volume with nx+ny+nz <= 2048");
     Mat3d rot, rinv;
     if (!view_basis64(v->position, v->aim, rot, rinv)) return fail(BIRP_ERR_SINGULAR, "Determinant is 0");
@@ -1386,7 +1386,6 @@ extern "C" int birp_render_views_noalloc(birp_volume* vol, const birp_view* view
 // kFastRing renders earlier, not for the work queued before this render.
 static int render_fast_into(birp_volume* vol, const birp_view* view, float* dimg) {
     birp_device* dev = vol->dev;
-    if (vol->dv.kind != SK_CUBE) return fail(BIRP_ERR_UNSUPPORTED, "fast renders take cube volumes");
     if (view->flags & BIRP_RENDER_ACCUMULATE) return fail(BIRP_ERR_UNSUPPORTED, "fast renders do not accumulate");
     const birp_tuning t = latency_tuning(*view);
     FastPose fp;

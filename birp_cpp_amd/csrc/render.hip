@@ -915,6 +915,68 @@ __device__ __forceinline__ float fast_sample(const DevVolume& v, __amdgpu_buffer
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// CMEM::GetSample (Cmem.cpp:117-186) in the fast form: FP32 with the hardware exp2/log2, square
+// root and reciprocal, cosines and sines of the angles as coordinate ratios and half-angle
+// identities, and arc cosines from Abramowitz & Stegun 4.4.46 (|error| <= 2e-8). Zone logic as the
+// reference's. A cusp angle whose cosine rounds past 1 gives 0 here (the reference: NaN).
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ float acos_fast(float x) {
+    const float ax = fminf(fabsf(x), 1.0f);
+    float p = -0.0012624911f;
+    p = __builtin_fmaf(p, ax, 0.0066700901f);
+    p = __builtin_fmaf(p, ax, -0.0170881256f);
+    p = __builtin_fmaf(p, ax, 0.0308918810f);
+    p = __builtin_fmaf(p, ax, -0.0501743046f);
+    p = __builtin_fmaf(p, ax, 0.0889789874f);
+    p = __builtin_fmaf(p, ax, -0.2145988016f);
+    p = __builtin_fmaf(p, ax, 1.5707963050f);
+    const float r = __builtin_amdgcn_sqrtf(1.0f - ax) * p;
+    return x >= 0.f ? r : 3.14159265358979f - r;
+}
+__device__ __forceinline__ float pow_fast(float b, float e) { return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(b)); }
+__device__ __forceinline__ float exp_fast(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896f); }
+
+__device__ __forceinline__ float cmem_fast(const CmemParams& m, float x, float y, float z) {
+    const float ryz2 = y * y + z * z;
+    const float r = __builtin_amdgcn_sqrtf(__builtin_fmaf(x, x, ryz2)), ryz = __builtin_amdgcn_sqrtf(ryz2);
+    const float ir = __builtin_amdgcn_rcpf(r), iyz = __builtin_amdgcn_rcpf(ryz);
+    // theta = acos(x / r), 0 when x == 0; phi = acos(y / |(y, z)|), 0 when y == 0 (Cmem.cpp:224-246)
+    const bool xs = x != 0.f, ys = y != 0.f;
+    const float ct = xs ? x * ir : 1.f, st = xs ? ryz * ir : 0.f;
+    const float theta = xs ? acos_fast(ct) : 0.f;
+    const float cphi = ys ? y * iyz : 1.f, sphi = ys ? fabsf(z) * iyz : 0.f;
+    // LinScaled (Cmem.cpp:188-222): cos(phi - pi/2) = sin(phi), cos(phi - 3 pi/2) = -sin(phi)
+    const float phi_n = acos_fast(ct * m.cos_theta_n + st * m.sin_theta_n * sphi);
+    const float phi_s = acos_fast(ct * m.cos_theta_s - st * m.sin_theta_s * sphi);
+    const float cos_half = __builtin_amdgcn_sqrtf(0.5f * (1.f + ct));  // cos(theta / 2), theta in [0, pi]
+    const float sin_2t = 2.f * st * ct;
+    const float f = pow_fast(cos_half + m.a5 * sin_2t * (1.f - exp_fast(-theta)),
+                             m.p1 * (m.beta_c[0] + m.beta_c[1] * cphi + m.beta_c[3] * (sphi * sphi)));
+    const float q = m.p2 * m.charlie * (exp_fast(m.p3 * m.dn * pow_fast(phi_n, m.a21)) + exp_fast(m.p3 * m.ds * pow_fast(phi_s, m.a21)));
+    const float rmp = m.p0 * m.r0_lin * f + q;
+    const float h = 0.025f;
+    if (r < rmp - h) return 0.f;
+    // ShueModel (Cmem.cpp:280-288)
+    const float base = 2.f * __builtin_amdgcn_rcpf(1.f + ct);
+    const float ry = m.bs * pow_fast(base, m.ay_bs);
+    const float rz = (m.az_bs == m.ay_bs) ? ry : m.bs * pow_fast(base, m.az_bs);
+    const float rzc = rz * cphi, rys = ry * sphi;
+    const float rbs = ry * rz * __builtin_amdgcn_rsqf(rzc * rzc + rys * rys);
+    // the zones of Cmem.cpp:132-176 (r >= rmp - h here)
+    const bool z_mp = r <= rmp + h;
+    const bool z_sheath = !z_mp && r < rbs - h;
+    const bool z_bs = !z_mp && !z_sheath && r >= rbs - h && r <= rbs + h;
+    const float sd = 0.1f * r;
+    const float outer = m.A2 * __builtin_amdgcn_rcpf(sd * sd * sd);
+    if (!(z_mp || z_sheath || z_bs)) return outer;
+    const float ht = 0.5f * theta, ht2 = ht * ht;
+    const float inner = m.A1 * exp_fast(-m.B * (ht2 * ht2)) * pow_fast(sd, -m.alpha - m.beta * (st * st));
+    if (z_mp) return ((r - (rmp - h)) * __builtin_amdgcn_rcpf(rmp + h)) * inner;  // lerp(0, inner, t)
+    if (z_sheath) return inner;
+    return __builtin_fmaf((r - (rbs - h)) * __builtin_amdgcn_rcpf(rbs + h), outer - inner, inner);
+}
+
 // Keeps the FP32 march x(j) = RN(j d + i0), j = 0 .. last, inside [0, hi]. x(j) is one rounding of
 // a linear function of j, so it is monotone in j and stays inside when x(0) and x(last) do. When
 // either is outside (a ray end within FP32 rounding of a face), the start is clamped and the step
@@ -955,8 +1017,9 @@ template <int KV, int S, int U>
 __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __restrict__ vols, const FastPose* __restrict__ poses,
                                                             const int* __restrict__ prefix, int npose, int total,
                                                             unsigned* counters, int tw_log2) {
-    constexpr int DT = KV & 3;
-    constexpr bool SIGNED = (KV & BIRP_KDT_SIGNED) != 0;
+    constexpr bool CMEM = KV == BIRP_KDT_CMEM;
+    constexpr int DT = CMEM ? BIRP_DTYPE_F32 : (KV & 3);
+    constexpr bool SIGNED = CMEM || (KV & BIRP_KDT_SIGNED) != 0;
     static_assert(!SIGNED || U == 1, "signed kernels take one sample per lane per trip");
     __shared__ unsigned s_tab[BIRP_BLOCK / 64][BIRP_FAST_TAB];
     const int lane = lane_id();
@@ -986,7 +1049,7 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
         const int pidx = __builtin_amdgcn_readfirstlane(lo);
         const FastPose& P = poses[pidx];
         const DevVolume& V = vols[P.vol];
-        if (P.vol != cached_vol) {
+        if (!CMEM && P.vol != cached_vol) {
             const unsigned* src = (DT == BIRP_DTYPE_F32) ? V.qtab : V.otab;
             const int ntab = V.nx + V.ny + V.nz;
             for (int q = lane; q < ntab; q += 64) tab[q] = src[q];
@@ -1017,36 +1080,49 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
             double w2 = P.rinv[6] * u0 + P.rinv[7] * u1 + P.rinv[8] * u2;
             const double iwd = 1.0 / sqrt(w0 * w0 + w1 * w1 + w2 * w2);
             double dx = w0 * iwd, dy = w1 * iwd, dz = w2 * iwd;
-            // FP64 cube entry/exit: inside iff 0 <= (x - o)/s < n - 1 per axis
-            double tin = -1e300, tout = 1e300;
-            slab_axis(P.px, dx, (double)V.ox, (double)V.ox + (double)(V.nx - 1) * V.sx, tin, tout);
-            slab_axis(P.py, dy, (double)V.oy, (double)V.oy + (double)(V.ny - 1) * V.sy, tin, tout);
-            slab_axis(P.pz, dz, (double)V.oz, (double)V.oz + (double)(V.nz - 1) * V.sz, tin, tout);
-            if (tin <= tout) {
-                const double sc = P.sc;
-                double a = ceil(tin * sc - 0.5), b = ceil(tout * sc - 0.5) - 1.0;
-                a = fmax(a, 0.0);
-                b = fmin(b, (double)(P.n - 1));
-                if (a <= b) {
-                    kf = (int)a;
-                    kl = (int)b;
+            if constexpr (CMEM) {
+                // the model has no bounds: every sample, marched in world coordinates
+                kf = 0;
+                kl = P.n - 1;
+                const double dt = P.dt, t0 = 0.5 * dt;
+                ix0 = (float)(P.px + dx * t0);
+                iy0 = (float)(P.py + dy * t0);
+                iz0 = (float)(P.pz + dz * t0);
+                dix = (float)(dx * dt);
+                diy = (float)(dy * dt);
+                diz = (float)(dz * dt);
+            } else {
+                // FP64 cube entry/exit: inside iff 0 <= (x - o)/s < n - 1 per axis
+                double tin = -1e300, tout = 1e300;
+                slab_axis(P.px, dx, (double)V.ox, (double)V.ox + (double)(V.nx - 1) * V.sx, tin, tout);
+                slab_axis(P.py, dy, (double)V.oy, (double)V.oy + (double)(V.ny - 1) * V.sy, tin, tout);
+                slab_axis(P.pz, dz, (double)V.oz, (double)V.oz + (double)(V.nz - 1) * V.sz, tin, tout);
+                if (tin <= tout) {
+                    const double sc = P.sc;
+                    double a = ceil(tin * sc - 0.5), b = ceil(tout * sc - 0.5) - 1.0;
+                    a = fmax(a, 0.0);
+                    b = fmin(b, (double)(P.n - 1));
+                    if (a <= b) {
+                        kf = (int)a;
+                        kl = (int)b;
+                    }
                 }
-            }
-            if (P.semantics == BIRP_SEM_REFERENCE && kf != 0) kl = -1;
-            if (kf <= kl) {
-                const double dt = P.dt, t0 = (kf + 0.5) * dt;
-                ix0 = (float)((P.px + dx * t0 - V.ox) * V.rdx);
-                iy0 = (float)((P.py + dy * t0 - V.oy) * V.rdy);
-                iz0 = (float)((P.pz + dz * t0 - V.oz) * V.rdz);
-                dix = (float)(dx * dt * V.rdx);
-                diy = (float)(dy * dt * V.rdy);
-                diz = (float)(dz * dt * V.rdz);
+                if (P.semantics == BIRP_SEM_REFERENCE && kf != 0) kl = -1;
+                if (kf <= kl) {
+                    const double dt = P.dt, t0 = (kf + 0.5) * dt;
+                    ix0 = (float)((P.px + dx * t0 - V.ox) * V.rdx);
+                    iy0 = (float)((P.py + dy * t0 - V.oy) * V.rdy);
+                    iz0 = (float)((P.pz + dz * t0 - V.oz) * V.rdz);
+                    dix = (float)(dx * dt * V.rdx);
+                    diy = (float)(dy * dt * V.rdy);
+                    diz = (float)(dz * dt * V.rdz);
+                }
             }
         }
         const int cnt = (kl >= kf) ? (kl - kf + 1) : 0;
         const int my = (cnt > sub) ? (cnt - sub + S - 1) / S : 0;  // samples of this sub-lane
         const int trips = wave_max_i(my);
-        if (cnt > 0) {
+        if (!CMEM && cnt > 0) {
             // every sample inside [0, nextbelow(n - 1)] per axis, so that the cell index is the
             // truncation (0 .. n-2) with no clamp per sample; a sample rounded onto the far face
             // keeps weight ~1 on the last grid point
@@ -1073,7 +1149,9 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
                 const float xd = __builtin_fmaf(fj, dix, ix0);
                 const float yd = __builtin_fmaf(fj, diy, iy0);
                 const float zd = __builtin_fmaf(fj, diz, iz0);
-                const float x = fast_sample<DT>(V, rs, tab, xd, yd, zd);
+                float x;
+                if constexpr (CMEM) x = cmem_fast(V.cmem, xd, yd, zd);
+                else x = fast_sample<DT>(V, rs, tab, xd, yd, zd);
                 const unsigned long long neg = __ballot(act && x < 0.f);
                 const unsigned grp = (unsigned)(neg >> gshift) & ((1u << S) - 1u);
                 const bool cut = (grp & ((2u << sub) - 1u)) != 0u;
@@ -1357,7 +1435,7 @@ static void* fast_kernel_ptr() { return (void*)k_render_fast<DT, S, U>; }
 template <int DT, int S>
 static int launch_fast_su(int unroll, int grid, hipStream_t st, const DevVolume* vols, const FastPose* poses, const int* prefix,
                           int npose, int total, unsigned* counter, int twl) {
-    if constexpr ((DT & BIRP_KDT_SIGNED) != 0) {
+    if constexpr ((DT & (BIRP_KDT_SIGNED | BIRP_KDT_CMEM)) != 0) {
         hipLaunchKernelGGL((k_render_fast<DT, S, 1>), dim3(grid), dim3(BIRP_BLOCK), 0, st, vols, poses, prefix, npose, total, counter, twl);
     } else {
         switch (unroll) {
@@ -1387,11 +1465,11 @@ int launch_fast_batch(const DevVolume* vols_dev, int dtype, const FastPose* pose
     if (dtype < 0 || dtype >= BIRP_KDT_COUNT) return BIRP_ERR_INVALID_ARGUMENT;
     grid = ((grid + 7) / 8) * 8;
     hipStream_t st = (hipStream_t)stream;
-    if (dtype & BIRP_KDT_SIGNED) unroll = 1;  // the signed kernels take one sample per lane per trip
+    if (dtype & (BIRP_KDT_SIGNED | BIRP_KDT_CMEM)) unroll = 1;  // the signed and CMEM kernels take one sample per lane per trip
 #define BIRP_LFD(KV) \
     case KV: return launch_fast_d<KV>(lanes_per_ray, unroll, grid, st, vols_dev, poses_dev, tile_prefix_dev, npose, total_tiles, work_counter, twl);
     switch (dtype) {
-        BIRP_LFD(0) BIRP_LFD(1) BIRP_LFD(2) BIRP_LFD(3) BIRP_LFD(4) BIRP_LFD(5) BIRP_LFD(6) BIRP_LFD(7)
+        BIRP_LFD(0) BIRP_LFD(1) BIRP_LFD(2) BIRP_LFD(3) BIRP_LFD(4) BIRP_LFD(5) BIRP_LFD(6) BIRP_LFD(7) BIRP_LFD(8)
         default: return BIRP_ERR_INVALID_ARGUMENT;
     }
 #undef BIRP_LFD
@@ -1408,7 +1486,7 @@ int fast_grid_size(int dtype, int lanes_per_ray, int unroll) {
     int dev = 0;
     (void)hipGetDevice(&dev);
     // cached per device and kernel variant: the occupancy query costs microseconds per launch
-    if (dtype & BIRP_KDT_SIGNED) unroll = 1;
+    if (dtype & (BIRP_KDT_SIGNED | BIRP_KDT_CMEM)) unroll = 1;
     static std::atomic<int> cache[16][BIRP_KDT_COUNT][5][5];
     const bool cacheable = dev >= 0 && dev < 16 && dtype >= 0 && dtype < BIRP_KDT_COUNT && lanes_per_ray <= 4 && unroll <= 4;
     if (cacheable) {
@@ -1434,6 +1512,7 @@ int fast_grid_size(int dtype, int lanes_per_ray, int unroll) {
         case 5: BIRP_FKS(5) break;
         case 6: BIRP_FKS(6) break;
         case 7: BIRP_FKS(7) break;
+        case 8: BIRP_FKS(8) break;
         default: BIRP_FK(0) break;
     }
 #undef BIRP_FKS

@@ -56,7 +56,9 @@ extern "C" {
  *        division and square root, std::lerp's branches, the double scale chain); bit-identical
  *        to the reference CPU render for finite cubes.
  * FAST:  FP32 with FMA, FP64 camera basis / ray origin / cube entry and FP32 Kahan (compensated)
- *        accumulation; the only form for BF16/F16 storage. */
+ *        accumulation; the only form for BF16/F16 storage. For CMEM models: the model in FP32
+ *        with the hardware exp2/log2 and polynomial arc cosines (within the reference FP32
+ *        model's own error of the FP64 model). */
 #define BIRP_MATH_EXACT 0
 #define BIRP_MATH_FAST 1
 

@@ -98,6 +98,44 @@ def test_cmem_render_576x336_800_samples(device, oracle):
         vol.free()
 
 
+@pytest.mark.parametrize("params", [None, dict(v=[400, 0, 0], b=[0, 0, 4], dipole=0.0, p0=1, p1=4, p2=6, p3=3, B=3.6, alpha=-1.0)])
+def test_fast_cmem_as_accurate_as_the_fp32_model(device, oracle, params):
+    """The fast CMEM form (FP32 with the hardware exp2/log2, coordinate-ratio cosines and polynomial
+    arc cosines; Kahan sums) against the oracle's FP64 model: the relative L2 error of every image is
+    at most twice the reference FP32 model's own error against FP64 (plus 1e-6), over the whole image
+    and off the zone edges; a batch of the same views gives the single renders' images to 1e-6."""
+    oc = oracle.OracleCmem(params)
+    vol = capi.Volume.cmem(device, capi.cmem_state_from(oc.state()))
+    b = None
+    try:
+        views = [W.view(144, 84, 0.25, pos=(1.0, 1.0, 1.0), aim=(1.0, 0.0, 0.0), length=44.0, math=capi.MATH_FAST),
+                 W.view(288, 168, 0.1, math=capi.MATH_FAST),
+                 W.view(144, 84, 0.5, pos=(12.0, 9.0, 4.0), aim=(-2.0, 0.0, 0.0), length=60.0, math=capi.MATH_FAST)]
+        singles = []
+        for v in views:
+            fast, _ = vol.render(v)
+            singles.append(fast)
+            ov = oview(oracle, v)
+            ref32, _ = oracle.render_f32(oc, ov, oracle.SEM_EXTENDED, nthreads=NT)
+            ref64, _ = oracle.render_f64(oc, ov, oracle.SEM_EXTENDED, nthreads=NT)
+            mask, _ = oc.edge_bound(ov, CMEM_EDGE_BAND, nthreads=NT)
+            keep = ~mask
+            own, err = rel_l2(ref32, ref64), rel_l2(fast, ref64)
+            own_k, err_k = rel_l2(ref32[keep], ref64[keep]), rel_l2(fast[keep], ref64[keep])
+            print(f"fast CMEM {v.width}x{v.height}: vs FP64 {err:.3e} (FP32 model {own:.3e}); off edges {err_k:.3e} ({own_k:.3e})")
+            assert np.isfinite(fast).all() and fast.any()
+            assert err <= 2 * own + 1e-6 and err_k <= 2 * own_k + 1e-6
+        b = capi.Batch(vol, views)
+        b.launch()
+        b.wait()
+        for k in range(len(views)):
+            assert rel_l2(b.image(k), singles[k]) <= 1e-6, k
+    finally:
+        if b is not None:
+            b.free()
+        vol.free()
+
+
 # ---------------------------------------------------------------------------------- FP64
 def test_f64_render_matches_oracle_f64(cube1, oracle):
     vals, org, spc, vols, oc = cube1
