@@ -122,6 +122,19 @@ def bench_plain(capi, W, dev, steps, warmup):
         capi.render_async(vol, v, img)
     dev_ms = dev.timer_end()  # waits for the last render
     wall = time.perf_counter() - t0
+    # the same renders under the other tile orders: the XCD-band order without a plan, and a plan
+    # made on every render (what a stream of new poses pays); the timed loop above reuses its plan
+    orders = {}
+    for name, pol in (("band_ms", 0), ("plan_every_render_ms", 2)):
+        capi.exact_order(pol)
+        for _ in range(warmup):
+            capi.render_async(vol, v, img)
+        dev.timer_begin()
+        for _ in range(steps):
+            capi.render_async(vol, v, img)
+        orders[name] = round(dev.timer_end() / steps, 5)
+    capi.exact_order(-1)
+    orders["plan_reused_ms"] = round(dev_ms / steps, 5)
     # the drop-in contract: one launch + wait + host copy per image, into an image that persists
     # across renders as Camera::image does
     k2 = max(10, steps)
@@ -142,7 +155,7 @@ def bench_plain(capi, W, dev, steps, warmup):
     del pin
     img.free()
     return dict(vol=vol, vals=vals, org=org, spc=spc, view=v, n_in=n_in, wall=wall, dev_ms=dev_ms, sync_rps=sync_rps,
-                sync_rps_pinned=sync_rps_pinned, image=host)
+                sync_rps_pinned=sync_rps_pinned, image=host, orders=orders)
 
 
 def cpu_baseline_leg(W, plain, cmem_img):
@@ -485,8 +498,8 @@ def bench_serving(capi, W, dev, num=2000, rate=200.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--forms", default="batched,exact_batched,cmem,precision,serving", help="extra forms to measure ('' for none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle checks")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -541,7 +554,8 @@ def main():
         "roofline": roof,
         "forms": {"plain_renders_per_s": round(p["sync_rps"], 1),
                   "plain_renders_per_s_pinned_image": round(p["sync_rps_pinned"], 1),
-                  "plain_renders_per_s_async": round(args.steps / p["wall"], 1)},
+                  "plain_renders_per_s_async": round(args.steps / p["wall"], 1),
+                  "plain_tile_order": p["orders"]},
     }
     # every result carries its error against the FP64 render of the same sample grid
     ref64 = p["vol"].render_f64(v)

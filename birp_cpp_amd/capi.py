@@ -84,6 +84,7 @@ _P = C.c_void_p
 _PP = C.POINTER(C.c_void_p)
 _FP = C.POINTER(C.c_float)
 _DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int)
 SIGNATURES = [
     ("birp_strerror", C.c_char_p, [_i]),
     ("birp_last_error", C.c_char_p, []),
@@ -137,6 +138,8 @@ SIGNATURES = [
     ("birp_synth_shell", _i, [C.POINTER(ShellParams), _FP, _FP, _FP, _FP]),
     ("birp_synth_orbit", _i, [_i, _f, _f, _f, C.c_uint64, _FP]),
     ("birp_debug_inject_failure", _i, [_P, _i]),
+    ("birp_debug_exact_order", _i, [_i, _IP]),
+    ("birp_debug_exact_plan", _i, [_P, C.POINTER(View), _IP, _i, _IP]),
 ]
 
 _lib = None
@@ -298,6 +301,12 @@ class Image:
         check(lib().birp_image_download(self.h, fptr(out)), "birp_image_download")
         return out
 
+    def upload(self, host: np.ndarray) -> None:
+        a = np.ascontiguousarray(host, np.float32)
+        if a.shape != (self.H, self.W):
+            raise ValueError(f"image is {self.H}x{self.W}, got {a.shape}")
+        check(lib().birp_image_upload(self.h, fptr(a)), "birp_image_upload")
+
     def free(self):
         if self.h:
             lib().birp_image_free(self.h)
@@ -421,6 +430,24 @@ class Server:
         if self.h:
             lib().birp_server_destroy(self.h)
             self.h = C.c_void_p()
+
+
+def exact_order(policy: int) -> int:
+    """Set the tile order of single exact renders (0 XCD band, 1 planned and reused, 2 planned
+    every render, -1 the default); returns the policy in force before."""
+    prev = C.c_int()
+    check(lib().birp_debug_exact_order(int(policy), C.byref(prev)), "birp_debug_exact_order")
+    return prev.value
+
+
+def exact_plan(vol: "Volume", view: View) -> np.ndarray:
+    """The device's tile plan for `view` over `vol` (tile ids, heaviest estimate first)."""
+    nb = C.c_int()
+    cap = 8192
+    order = np.zeros(cap, np.int32)
+    check(lib().birp_debug_exact_plan(vol.h, C.byref(view), order.ctypes.data_as(_IP), cap, C.byref(nb)),
+          "birp_debug_exact_plan")
+    return order[:nb.value].copy()
 
 
 def pinned_empty(shape, dtype=np.float32) -> np.ndarray:

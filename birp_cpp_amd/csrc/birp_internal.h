@@ -119,7 +119,22 @@ struct F64View {
 
 // Launch wrappers implemented in render.hip (host side of the HIP translation unit).
 namespace birp {
-int launch_exact(const DevVolume& v, const ExactView& ev, int lanes_per_ray, int pipe, void* stream);
+// Tile order of a single exact render (see k_exact_plan): `order` is the device's plan buffer of
+// BIRP_PLAN_INTS ints (the order first, then the plan kernel's tile estimates and its ticket,
+// which starts at zero); `replan` launches the plan kernel into it first, otherwise it holds the
+// plan of the same view geometry, cube box and tile shape. ncu: the device's CU count.
+#define BIRP_PLAN_MAX_BLOCKS 8192
+#define BIRP_PLAN_BUCKETS 1024
+#define BIRP_PLAN_INTS (2 * BIRP_PLAN_MAX_BLOCKS + 64)
+struct ExactOrder {
+    int* order;
+    int ncu;
+    bool replan;
+};
+int launch_exact(const DevVolume& v, const ExactView& ev, int lanes_per_ray, int pipe, void* stream,
+                 const ExactOrder* ord = nullptr);
+// the plan alone, into order (test hook)
+int launch_exact_plan(const DevVolume& v, const ExactView& ev, int lanes_per_ray, int* order, void* stream);
 // Slots of ExactView::wave_counts for a W x H image at `lanes_per_ray` (4 waves per block).
 int exact_wave_slots(int W, int H, int lanes_per_ray);
 // Sample modes of the exact kernels. SM_QUAD_POS: positive finite cube, |origin| >= 2^-30 on every

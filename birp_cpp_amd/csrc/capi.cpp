@@ -204,6 +204,21 @@ int default_exact_pipe() {
     }();
     return pipe;
 }
+// BIRP_EXACT_ORDER: "band" renders single exact views in the XCD-band tile order, "replan"
+// plans the tile order on every render, and by default a plan is reused while the view's
+// geometry, the cube's box and the tile shape stay the same. 0 band, 1 cached, 2 replan.
+// birp_debug_exact_order overrides it (-1: no override).
+static std::atomic<int> g_exact_order_override{-1};
+int exact_order_policy() {
+    static int policy = [] {
+        const char* e = getenv("BIRP_EXACT_ORDER");
+        if (e && strcmp(e, "band") == 0) return 0;
+        if (e && strcmp(e, "replan") == 0) return 2;
+        return 1;
+    }();
+    const int o = g_exact_order_override.load();
+    return o >= 0 ? o : policy;
+}
 bool exact_force_linear() {
     static bool lin = [] {
         const char* e = getenv("BIRP_EXACT_LAYOUT");
@@ -255,6 +270,11 @@ extern "C" int birp_device_open(int ordinal, birp_device** out) {
              (e = hipEventCreate(&d->ev_a)) != hipSuccess || (e = hipEventCreate(&d->ev_b)) != hipSuccess)
         what = "hipEventCreate";
     else if ((e = hipMalloc(&d->d_counter, 64)) != hipSuccess) what = "hipMalloc(counter)";
+    else if ((e = hipMalloc(&d->plan_order, BIRP_PLAN_INTS * sizeof(int))) != hipSuccess) what = "hipMalloc(tile plan)";
+    else if ((e = hipMemsetAsync(d->plan_order, 0, BIRP_PLAN_INTS * sizeof(int), d->stream)) != hipSuccess)
+        what = "hipMemsetAsync(tile plan)";  // the plan kernel's ticket starts at zero
+    else if ((e = hipDeviceGetAttribute(&d->ncu, hipDeviceAttributeMultiprocessorCount, ordinal)) != hipSuccess)
+        what = "hipDeviceGetAttribute(CUs)";
     else if (noalloc_scratch_address(&d->noalloc_scratch) != BIRP_OK) {
         e = hipErrorInvalidSymbol;
         what = "hipGetSymbolAddress(scratch)";
@@ -280,6 +300,7 @@ extern "C" int birp_device_close(birp_device* dev) {
     if (dev->pinned) (void)hipHostFree(dev->pinned);
     if (dev->rbuf) (void)hipFree(dev->rbuf);
     if (dev->d_counter) (void)hipFree(dev->d_counter);
+    if (dev->plan_order) (void)hipFree(dev->plan_order);
     for (hipEvent_t ev : {dev->ev0, dev->ev1, dev->ev_a, dev->ev_b})
         if (ev) (void)hipEventDestroy(ev);
     if (dev->stream) (void)hipStreamDestroy(dev->stream);
@@ -314,6 +335,36 @@ extern "C" int birp_debug_inject_failure(birp_device* dev, int count) {
     if (!dev || count < 0) return fail(BIRP_ERR_INVALID_ARGUMENT, "bad argument");
     dev->inject_failures.store(count);
     return BIRP_OK;
+}
+
+extern "C" int birp_debug_exact_order(int policy, int* previous) {
+    if (policy < -1 || policy > 2) return fail(BIRP_ERR_INVALID_ARGUMENT, "policy must be -1, 0, 1 or 2");
+    const int was = exact_order_policy();
+    g_exact_order_override.store(policy);
+    if (previous) *previous = was;
+    return BIRP_OK;
+}
+
+extern "C" int birp_debug_exact_plan(birp_volume* vol, const birp_view* view, int* order, int capacity, int* nblocks) {
+    if (!vol || !view || !order || !nblocks) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = validate_view(view, true);
+    if (rc) return rc;
+    if (vol->dv.kind != SK_CUBE) return fail(BIRP_ERR_UNSUPPORTED, "tile plans are made for cubes");
+    birp_device* dev = vol->dev;
+    (void)hipSetDevice(dev->ordinal);
+    ExactView ev;
+    if ((rc = exact_view(vol, view, nullptr, ev))) return rc;
+    const int lanes = default_exact_lanes(false);
+    const int nb = exact_tiles(ev.W, ev.H, lanes);
+    *nblocks = nb;
+    if (nb > BIRP_PLAN_MAX_BLOCKS) return fail(BIRP_ERR_UNSUPPORTED, "view has more tiles than a plan holds");
+    if (capacity < nb) return fail(BIRP_ERR_INVALID_ARGUMENT, "order capacity below the view's tile count");
+    dev->plan_valid = false;  // the device's plan buffer is reused here
+    rc = launch_exact_plan(vol->dv, ev, lanes, dev->plan_order, dev->stream);
+    if (rc) return fail(rc, "plan launch failed");
+    hipError_t e = hipMemcpyAsync(order, dev->plan_order, (size_t)nb * sizeof(int), hipMemcpyDeviceToHost, dev->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
+    return e == hipSuccess ? BIRP_OK : hip_fail(e, "plan copy");
 }
 
 // ============================================================================ volumes
@@ -671,6 +722,31 @@ static int render_buffer(birp_device* dev, size_t npix, float** img, unsigned lo
     return BIRP_OK;
 }
 
+// The tile order of a single exact render (k_exact_plan): planned on the device when the view's
+// geometry, the cube's box or the tile shape differ from those of the device's last plan, reused
+// otherwise. Views of fewer tiles than CUs or of more than BIRP_PLAN_MAX_BLOCKS, and CMEM views
+// (every ray takes all n samples), keep the XCD-band order (null).
+static const ExactOrder* exact_order(birp_device* dev, const DevVolume& dv, const ExactView& ev, int lanes, ExactOrder& ord) {
+    const int policy = exact_order_policy();
+    if (policy == 0 || !dev->plan_order || dev->ncu <= 0 || dv.kind == SK_CMEM) return nullptr;
+    const int nblocks = exact_tiles(ev.W, ev.H, lanes);
+    if (nblocks < dev->ncu || nblocks > BIRP_PLAN_MAX_BLOCKS) return nullptr;
+    ExactPlanKey k;
+    memset(&k, 0, sizeof k);
+    for (int q = 0; q < 9; ++q) k.rinv[q] = ev.rinv[q];
+    k.px = ev.px, k.py = ev.py, k.pz = ev.pz;
+    k.tanx = ev.tanx, k.tany = ev.tany, k.aspect = ev.aspect, k.L = ev.L;
+    k.ox = dv.ox, k.oy = dv.oy, k.oz = dv.oz, k.sx = dv.sx, k.sy = dv.sy, k.sz = dv.sz;
+    k.nx = dv.nx, k.ny = dv.ny, k.nz = dv.nz;
+    k.W = ev.W, k.H = ev.H, k.n = ev.n, k.semantics = ev.semantics, k.lanes = lanes;
+    ord.order = dev->plan_order;
+    ord.ncu = dev->ncu;
+    ord.replan = policy == 2 || !dev->plan_valid || memcmp(&k, &dev->plan_key, sizeof k) != 0;
+    dev->plan_key = k;
+    dev->plan_valid = false;  // until the launch is enqueued
+    return &ord;
+}
+
 // `counter` (a device u64 that each wave adds its in-cube samples to) or `wave_counts` (one slot
 // per wave, exact_wave_slots of the image, each stored by its wave), or neither.
 static int render_exact_into(birp_volume* vol, const birp_view* view, float* dimg, unsigned long long* counter,
@@ -685,7 +761,11 @@ static int render_exact_into(birp_volume* vol, const birp_view* view, float* dim
     if (take_injected_failure(dev)) return fail(BIRP_ERR_INJECTED, "injected failure");
     DevVolume dv = vol->dv;
     if (exact_force_linear() && dv.vclass != VC_NONFINITE && dv.kind == SK_CUBE) dv.vclass = VC_NONFINITE;
-    rc = launch_exact(dv, ev, default_exact_lanes(dv.kind == SK_CMEM), default_exact_pipe(), dev->stream);
+    const int lanes = default_exact_lanes(dv.kind == SK_CMEM);
+    ExactOrder ord{};
+    const ExactOrder* use = exact_order(dev, dv, ev, lanes, ord);
+    rc = launch_exact(dv, ev, lanes, default_exact_pipe(), dev->stream, use);
+    if (use) dev->plan_valid = rc == BIRP_OK;
     return rc ? fail(rc, "exact render launch failed") : BIRP_OK;
 }
 

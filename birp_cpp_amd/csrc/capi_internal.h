@@ -11,6 +11,16 @@
 #include "../../include/birp.h"
 #include "birp_internal.h"
 
+// What a tile plan depends on: the rays' geometry, the cube's box and the tile shape. Filled
+// after a memset, so that two keys compare with memcmp.
+struct ExactPlanKey {
+    float rinv[9];
+    float px, py, pz, tanx, tany, aspect, L;
+    float ox, oy, oz, sx, sy, sz;
+    int nx, ny, nz;
+    int W, H, n, semantics, lanes;
+};
+
 struct birp_device {
     int ordinal = 0;
     hipStream_t stream = nullptr;      // render stream (default priority)
@@ -37,6 +47,12 @@ struct birp_device {
     bool count_known = false;          // false: clear the counter before its next use
     std::atomic<int> inject_failures{0};
     void* noalloc_scratch = nullptr;   // this GPU's fixed scratch array (birp_render_views_noalloc)
+    // single exact renders: the device's tile plan buffer (BIRP_PLAN_INTS ints, the order first)
+    // and what it was planned for
+    int* plan_order = nullptr;
+    int ncu = 0;
+    ExactPlanKey plan_key{};
+    bool plan_valid = false;
 };
 
 struct birp_volume {

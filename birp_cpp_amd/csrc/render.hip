@@ -546,15 +546,166 @@ __device__ __forceinline__ void exact_block(const DevVolume& v, const ExactView&
     }
 }
 
-// One view: block b renders tile lb(b). Consecutive dispatch slots go round-robin over the 8
-// XCDs, so each XCD is given a contiguous band of tiles (neighbouring rays share cube bricks in
-// its L2).
+// One view: block b renders tile lb(b).
+// Without a plan, consecutive dispatch slots go round-robin over the 8 XCDs, so each XCD is
+// given a contiguous band of tiles (neighbouring rays share cube bricks in its L2).
+// With a plan (`order`: the tiles sorted heaviest first by k_exact_plan), slot b takes the
+// sorted tile at its place in a snake over the CUs: round r of ncu slots runs forward when r is
+// even and backward when odd, so that every CU draws one tile from each round and the heavy
+// rounds are spread evenly (the longest-processing-time-first rule). The grid is then nblocks.
 template <int S, int MODE, int P = 1>
-__global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact(DevVolume v, ExactView ev, int wg_x, int nblocks) {
-    const int per_xcd = (nblocks + 7) / 8;
-    const int lb = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+__global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact(DevVolume v, ExactView ev, int wg_x, int nblocks,
+                                                             const int* __restrict__ order, int ncu) {
+    int lb;
+    if (order) {
+        const int b = blockIdx.x, r = b / ncu, base = r * ncu, c = b - base;
+        const int nr = min(ncu, nblocks - base);
+        lb = order[base + ((r & 1) ? nr - 1 - c : c)];
+    } else {
+        const int per_xcd = (nblocks + 7) / 8;
+        lb = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    }
     if (lb >= nblocks) return;
     exact_block<S, MODE, P>(v, ev, lb, wg_x);
+}
+
+// ------------------------------------------------------------------------------------------
+// Tile plan of a single exact view. A view of a few thousand tiles is resident all at once
+// (a 576x336 view is 1512 blocks, six per CU): each CU keeps the blocks the dispatcher handed it,
+// and a CU that drew several tiles across the middle of the cube's image finishes last while
+// others idle. The plan estimates every tile's work -- the longest in-cube run of samples among
+// five rays of each wave (its four corners and its centre), from a slab test against the cube's
+// box in approximate FP32 -- counting-sorts the tiles on it, heaviest first, and writes the
+// order. Orders change which CU renders a tile, never what it computes: the image is the same
+// bit for bit whatever the order.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void plan_slab(float p, float rd, float lo, float hi, float& tin, float& tout) {
+    const float t1 = (lo - p) * rd, t2 = (hi - p) * rd;  // rd = 1/d (inf for d = 0: NaN t drop out)
+    tin = fmaxf(tin, fminf(t1, t2));
+    tout = fminf(tout, fmaxf(t1, t2));
+}
+
+// samples of ray (i, j) inside the cube's box, approximately: an estimate of its trip count
+__device__ __forceinline__ int plan_ray_len(const DevVolume& v, const ExactView& ev, int i, int j, float sc, float rW,
+                                            float rH) {
+    const float sx = (2.f * ((float)i + 0.5f) * rW - 1.f) * ev.tanx * ev.aspect;
+    const float sy = (1.f - 2.f * ((float)j + 0.5f) * rH) * ev.tany;
+    // world direction of the camera ray (sx, sy, -1), unnormalised: t along it is t / |(sx, sy, 1)|
+    const float dx = ev.rinv[0] * sx + ev.rinv[1] * sy - ev.rinv[2];
+    const float dy = ev.rinv[3] * sx + ev.rinv[4] * sy - ev.rinv[5];
+    const float dz = ev.rinv[6] * sx + ev.rinv[7] * sy - ev.rinv[8];
+    float tin = -3.0e38f, tout = 3.0e38f;
+    plan_slab(ev.px, __builtin_amdgcn_rcpf(dx), v.ox, v.ox + (float)(v.nx - 1) * v.sx, tin, tout);
+    plan_slab(ev.py, __builtin_amdgcn_rcpf(dy), v.oy, v.oy + (float)(v.ny - 1) * v.sy, tin, tout);
+    plan_slab(ev.pz, __builtin_amdgcn_rcpf(dz), v.oz, v.oz + (float)(v.nz - 1) * v.sz, tin, tout);
+    if (!(tin <= tout)) return 0;
+    const float s = sc * __builtin_amdgcn_sqrtf(sx * sx + sy * sy + 1.f);  // samples per unit of t
+    const float klo = fminf(fmaxf(ceilf(tin * s - 0.5f), 0.f), (float)ev.n);
+    const float khi = fmaxf(fminf(floorf(tout * s - 0.5f), (float)(ev.n - 1)), -1.f);
+    if (ev.semantics == BIRP_SEM_REFERENCE && klo > 0.f) return 0;  // first sample outside: dead ray
+    return khi >= klo ? (int)(khi - klo) + 1 : 0;
+}
+
+// grid: ceil(4 nblocks / BIRP_BLOCK) blocks of BIRP_BLOCK threads, thread 4b + w estimating wave
+// w of tile b (a tile's four waves sit in neighbouring lanes). The last block to take a ticket
+// (which it resets for the next plan) sorts. The estimates go between blocks as agent-scope
+// atomic stores and loads, and each block waits for its stores before it takes its ticket:
+// no cache-wide fence. The sort deals out exactly the tiles 0..nblocks-1 whatever estimates it
+// reads, so the order is a permutation in every case. plan = [order | estimates | ticket], each
+// of the first two BIRP_PLAN_MAX_BLOCKS ints, BIRP_PLAN_INTS in all; the ticket starts at zero.
+template <int S>
+__global__ __launch_bounds__(BIRP_BLOCK) void k_exact_plan(DevVolume v, ExactView ev, int wg_x, int nblocks, int* __restrict__ plan) {
+    constexpr int TW = ExactTile<S>::TW, TH = ExactTile<S>::TH, NB = BIRP_PLAN_BUCKETS, T = BIRP_BLOCK;
+    constexpr int PER_LANE = NB / 64;  // buckets per lane of the scanning wave
+    static_assert(NB % 64 == 0, "plan shape");
+    static_assert(4 * BIRP_MAX_EXACT_SAMPLES <= 65535, "tile estimates are kept as u16");
+    int* order = plan;
+    int* cost = plan + BIRP_PLAN_MAX_BLOCKS;
+    int* ticket = plan + 2 * BIRP_PLAN_MAX_BLOCKS;
+    __shared__ unsigned short s_key[BIRP_PLAN_MAX_BLOCKS];
+    __shared__ int s_start[NB];
+    __shared__ int s_lo, s_hi, s_last;
+    const int tid = threadIdx.x;
+    const int q = blockIdx.x * T + tid, b = q >> 2, w = q & 3;
+    int m = 0;
+    if (b < nblocks) {
+        const int i0 = ((b % wg_x) * 2 + (w & 1)) * TW, j0 = ((b / wg_x) * 2 + (w >> 1)) * TH;
+        if (i0 < ev.W && j0 < ev.H) {
+            const float sc = (float)ev.n / ev.L, rW = 1.f / (float)ev.W, rH = 1.f / (float)ev.H;
+            const int i1 = min(i0 + TW - 1, ev.W - 1), j1 = min(j0 + TH - 1, ev.H - 1);
+            m = plan_ray_len(v, ev, (i0 + i1) >> 1, (j0 + j1) >> 1, sc, rW, rH);
+            m = max(m, plan_ray_len(v, ev, i0, j0, sc, rW, rH));
+            m = max(m, plan_ray_len(v, ev, i1, j0, sc, rW, rH));
+            m = max(m, plan_ray_len(v, ev, i0, j1, sc, rW, rH));
+            m = max(m, plan_ray_len(v, ev, i1, j1, sc, rW, rH));
+        }
+    }
+    m += __shfl_xor(m, 1);
+    m += __shfl_xor(m, 2);
+    if (b < nblocks && w == 0) __hip_atomic_store(cost + b, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);  // this thread's store has completed
+    __syncthreads();
+    if (tid == 0) s_last = atomicAdd(ticket, 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (!s_last) return;
+
+    for (int k = tid; k < NB; k += T) s_start[k] = 0;
+    if (tid == 0) { s_lo = 0x7fffffff; s_hi = 0; }
+    int lo = 0x7fffffff, hi = 0;
+    for (int t0 = 0; t0 < nblocks; t0 += 8 * T) {
+        int c[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)  // eight loads in flight; the index is clamped into the area
+            c[k] = __hip_atomic_load(cost + min(t0 + k * T + tid, BIRP_PLAN_MAX_BLOCKS - 1), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int t = t0 + k * T + tid;
+            if (t < nblocks) {
+                const int cv = min(max(c[k], 0), 4 * BIRP_MAX_EXACT_SAMPLES);
+                s_key[t] = (unsigned short)cv;
+                lo = min(lo, cv);
+                hi = max(hi, cv);
+            }
+        }
+    }
+    __syncthreads();
+    atomicMin(&s_lo, lo);
+    atomicMax(&s_hi, hi);
+    __syncthreads();
+    // NB buckets over [lo, hi], the heaviest in bucket 0
+    const int base = s_lo, span = s_hi - s_lo;
+    for (int t = tid; t < nblocks; t += T) {
+        const int qq = span > 0 ? (int)(((long long)(s_key[t] - base) * (NB - 1)) / span) : 0;
+        s_key[t] = (unsigned short)(NB - 1 - qq);
+        atomicAdd(&s_start[NB - 1 - qq], 1);
+    }
+    __syncthreads();
+    // exclusive prefix over the buckets by the first wave: lane l owns PER_LANE buckets in a row
+    if (tid < 64) {
+        int cnt[PER_LANE], sum = 0;
+#pragma unroll
+        for (int k = 0; k < PER_LANE; ++k) {
+            cnt[k] = s_start[tid * PER_LANE + k];
+            sum += cnt[k];
+        }
+        int incl = sum;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int up = __shfl_up(incl, off);
+            if (tid >= off) incl += up;
+        }
+        int run = incl - sum;
+#pragma unroll
+        for (int k = 0; k < PER_LANE; ++k) {
+            s_start[tid * PER_LANE + k] = run;
+            run += cnt[k];
+        }
+    }
+    __syncthreads();
+    // every tile takes one place in its bucket's range: the places are 0..nblocks-1, each once
+    for (int t = tid; t < nblocks; t += T) order[atomicAdd(&s_start[s_key[t]], 1)] = t;
+    if (tid == 0) atomicExch(ticket, 0);  // ready for the next plan
 }
 
 // Many views (each with its own volume, image and sample-count slots) in one launch: items are
@@ -1293,7 +1444,7 @@ size_t exact_lds_bytes(const DevVolume& v, int n, int mode) {
 }
 
 template <int S>
-static int launch_exact_s(const DevVolume& v, const ExactView& ev, int pipe, hipStream_t st) {
+static int launch_exact_s(const DevVolume& v, const ExactView& ev, int pipe, hipStream_t st, const ExactOrder* ord) {
     constexpr int TW = ExactTile<S>::TW, TH = ExactTile<S>::TH;
     int wg_x = (ev.W + 2 * TW - 1) / (2 * TW);
     int wg_y = (ev.H + 2 * TH - 1) / (2 * TH);
@@ -1302,17 +1453,31 @@ static int launch_exact_s(const DevVolume& v, const ExactView& ev, int pipe, hip
     const int mode = exact_mode_of(v, ev);
     const size_t lds = exact_lds_bytes(v, ev.n, mode);
     if (lds > 64 * 1024) return BIRP_ERR_UNSUPPORTED;
+    const int* order = nullptr;
+    int ncu = 0;
+    if (ord && ord->order) {
+        if (nblocks > BIRP_PLAN_MAX_BLOCKS || ord->ncu <= 0 || mode == SM_CMEM) return BIRP_ERR_INVALID_ARGUMENT;
+        if (ord->replan) {
+            hipLaunchKernelGGL((k_exact_plan<S>), dim3((4 * nblocks + BIRP_BLOCK - 1) / BIRP_BLOCK), dim3(BIRP_BLOCK), 0, st, v, ev,
+                               wg_x, nblocks, ord->order);
+            const int rc = hip_status(hipGetLastError());
+            if (rc) return rc;
+        }
+        order = ord->order;
+        ncu = ord->ncu;
+        grid = nblocks;
+    }
     switch (mode) {
         case SM_QUAD_POS:
-            if (pipe >= 4) hipLaunchKernelGGL((k_render_exact<S, SM_QUAD_POS, 4>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks);
-            else if (pipe == 3) hipLaunchKernelGGL((k_render_exact<S, SM_QUAD_POS, 3>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks);
-            else if (pipe == 1) hipLaunchKernelGGL((k_render_exact<S, SM_QUAD_POS, 1>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks);
-            else hipLaunchKernelGGL((k_render_exact<S, SM_QUAD_POS, 2>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks);
+            if (pipe >= 4) hipLaunchKernelGGL((k_render_exact<S, SM_QUAD_POS, 4>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks, order, ncu);
+            else if (pipe == 3) hipLaunchKernelGGL((k_render_exact<S, SM_QUAD_POS, 3>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks, order, ncu);
+            else if (pipe == 1) hipLaunchKernelGGL((k_render_exact<S, SM_QUAD_POS, 1>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks, order, ncu);
+            else hipLaunchKernelGGL((k_render_exact<S, SM_QUAD_POS, 2>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks, order, ncu);
             break;
-        case SM_QUAD_GEN: hipLaunchKernelGGL((k_render_exact<S, SM_QUAD_GEN>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks); break;
-        case SM_LIN_TRI: hipLaunchKernelGGL((k_render_exact<S, SM_LIN_TRI>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks); break;
-        case SM_LIN_NEAREST: hipLaunchKernelGGL((k_render_exact<S, SM_LIN_NEAREST>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks); break;
-        default: hipLaunchKernelGGL((k_render_exact<S, SM_CMEM>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks); break;
+        case SM_QUAD_GEN: hipLaunchKernelGGL((k_render_exact<S, SM_QUAD_GEN>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks, order, ncu); break;
+        case SM_LIN_TRI: hipLaunchKernelGGL((k_render_exact<S, SM_LIN_TRI>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks, order, ncu); break;
+        case SM_LIN_NEAREST: hipLaunchKernelGGL((k_render_exact<S, SM_LIN_NEAREST>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks, order, ncu); break;
+        default: hipLaunchKernelGGL((k_render_exact<S, SM_CMEM>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, ev, wg_x, nblocks, order, ncu); break;
     }
     return hip_status(hipGetLastError());
 }
@@ -1346,12 +1511,31 @@ int launch_exact_multi(const DevVolume* vols, const ExactView* views, const int*
     return hip_status(hipGetLastError());
 }
 
-int launch_exact(const DevVolume& v, const ExactView& ev, int lanes_per_ray, int pipe, void* stream) {
+int launch_exact(const DevVolume& v, const ExactView& ev, int lanes_per_ray, int pipe, void* stream, const ExactOrder* ord) {
     if (ev.n < 2 || ev.n > BIRP_MAX_EXACT_SAMPLES || ev.W <= 0 || ev.H <= 0) return BIRP_ERR_INVALID_ARGUMENT;
     hipStream_t st = (hipStream_t)stream;
-    if (lanes_per_ray == 4) return launch_exact_s<4>(v, ev, pipe, st);
-    if (lanes_per_ray == 2) return launch_exact_s<2>(v, ev, pipe, st);
-    return launch_exact_s<1>(v, ev, pipe, st);
+    if (lanes_per_ray == 4) return launch_exact_s<4>(v, ev, pipe, st, ord);
+    if (lanes_per_ray == 2) return launch_exact_s<2>(v, ev, pipe, st, ord);
+    return launch_exact_s<1>(v, ev, pipe, st, ord);
+}
+
+template <int S>
+static int launch_plan_s(const DevVolume& v, const ExactView& ev, int* order, hipStream_t st) {
+    constexpr int TW = ExactTile<S>::TW, TH = ExactTile<S>::TH;
+    const int wg_x = (ev.W + 2 * TW - 1) / (2 * TW);
+    const int nblocks = wg_x * ((ev.H + 2 * TH - 1) / (2 * TH));
+    if (nblocks > BIRP_PLAN_MAX_BLOCKS) return BIRP_ERR_INVALID_ARGUMENT;
+    hipLaunchKernelGGL((k_exact_plan<S>), dim3((4 * nblocks + BIRP_BLOCK - 1) / BIRP_BLOCK), dim3(BIRP_BLOCK), 0, st, v, ev, wg_x,
+                       nblocks, order);
+    return hip_status(hipGetLastError());
+}
+
+int launch_exact_plan(const DevVolume& v, const ExactView& ev, int lanes_per_ray, int* order, void* stream) {
+    if (ev.n < 2 || ev.W <= 0 || ev.H <= 0 || !order) return BIRP_ERR_INVALID_ARGUMENT;
+    hipStream_t st = (hipStream_t)stream;
+    if (lanes_per_ray == 4) return launch_plan_s<4>(v, ev, order, st);
+    if (lanes_per_ray == 2) return launch_plan_s<2>(v, ev, order, st);
+    return launch_plan_s<1>(v, ev, order, st);
 }
 
 int launch_f64(const DevVolume& v, const F64View& fv, void* stream) {
