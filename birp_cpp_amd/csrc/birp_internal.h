@@ -151,7 +151,10 @@ size_t exact_lds_bytes(const DevVolume& v, int n, int mode);
 int exact_tiles(int W, int H, int lanes_per_ray);
 // Views [0, nview) of one sample mode, tiles view-major from prefix (nview + 1 entries, device).
 int launch_exact_multi(const DevVolume* vols_dev, const ExactView* views_dev, const int* prefix_dev, int nview, int total,
-                       int mode, size_t lds, void* stream);
+                       int mode, size_t lds, void* stream, const int* order_dev = nullptr);
+// Per-item work estimates of such a list (not for SM_CMEM): cost_dev[g] for g < total.
+int launch_plan_multi(const DevVolume* vols_dev, const ExactView* views_dev, const int* prefix_dev, int nview, int total,
+                      int mode, int* cost_dev, void* stream);
 int launch_f64(const DevVolume& v, const F64View& fv, void* stream);
 int launch_sample_points(const DevVolume& v, const float* xyz, long long n, float* out, void* stream);
 // Any other Space: the sample points of rows [row0, row0 + rows) of a view (3 floats per (pixel, k),

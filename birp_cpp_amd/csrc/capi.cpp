@@ -1041,10 +1041,12 @@ void batch_release(birp_batch* b) {
     if (b->d_prefix) (void)hipFree(b->d_prefix);
     if (b->d_vols) (void)hipFree(b->d_vols);
     if (b->d_evs) (void)hipFree(b->d_evs);
+    if (b->d_plan) (void)hipFree(b->d_plan);
     b->d_poses = nullptr;
     b->d_prefix = nullptr;
     b->d_vols = nullptr;
     b->d_evs = nullptr;
+    b->d_plan = nullptr;
 }
 
 // The kernels of an exact batch on stream st: one multi-view launch per sample-mode group.
@@ -1052,17 +1054,75 @@ int exact_batch_launch(birp_batch* b, hipStream_t st) {
     int base = 0;
     for (size_t g = 0; g < b->groups.size(); g++) {
         const birp_batch::ExactGroup& G = b->groups[g];
-        int rc = launch_exact_multi(b->d_vols, b->d_evs + G.first, b->d_prefix + base, G.count, G.total, G.mode, G.lds, st);
+        const int* plan = G.plan_at >= 0 ? b->d_plan + G.plan_at : nullptr;
+        int rc = launch_exact_multi(b->d_vols, b->d_evs + G.first, b->d_prefix + base, G.count, G.total, G.mode, G.lds, st, plan);
         if (rc) return rc;
         base += G.count + 1;
     }
     return BIRP_OK;
 }
 
+// The tile plans of an exact batch, made once when the batch is set up: each cube group's tiles
+// are estimated on the device (k_plan_multi), and each view's tiles are counting-sorted on the
+// host, heaviest first, in place of their natural order; the views keep their order, so the
+// launch still walks the batch view by view. The multi-view launch hands blocks to CUs as
+// earlier ones finish, so a view's light tiles fill the gaps its heavy tiles leave instead of
+// the heaviest tiles starting last. CMEM groups (every ray takes all n samples), groups of fewer
+// tiles than CUs, and BIRP_EXACT_ORDER=band keep the natural order.
+static int exact_batch_plan(birp_batch* b, const std::vector<int>& prefix) {
+    birp_device* dev = b->vol->dev;
+    if (exact_order_policy() == 0 || dev->ncu <= 0) return BIRP_OK;
+    long long need = 0;
+    for (const birp_batch::ExactGroup& G : b->groups)
+        if (G.mode != SM_CMEM && G.total >= dev->ncu) need += G.total;
+    if (!need) return BIRP_OK;
+    hipError_t e;
+    int* d_cost = nullptr;
+    if ((e = hipMalloc(&b->d_plan, (size_t)need * sizeof(int))) != hipSuccess) return hip_fail(e, "hipMalloc(batch plan)");
+    if ((e = hipMalloc(&d_cost, (size_t)need * sizeof(int))) != hipSuccess) return hip_fail(e, "hipMalloc(plan estimates)");
+    std::vector<int> cost((size_t)need), plan((size_t)need), bucket;
+    int rc = BIRP_OK;
+    long long at = 0;
+    int base = 0;  // the group's first entry of prefix (count + 1 entries per group)
+    for (birp_batch::ExactGroup& G : b->groups) {
+        const int pbase = base;
+        base += G.count + 1;
+        if (G.mode == SM_CMEM || G.total < dev->ncu) continue;
+        rc = launch_plan_multi(b->d_vols, b->d_evs + G.first, b->d_prefix + pbase, G.count, G.total, G.mode, d_cost + at, dev->stream);
+        if (rc) break;
+        e = hipMemcpyAsync(cost.data() + at, d_cost + at, (size_t)G.total * sizeof(int), hipMemcpyDeviceToHost, dev->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
+        if (e != hipSuccess) { rc = hip_fail(e, "plan estimates"); break; }
+        int* c = cost.data() + at;
+        for (int g = 0; g < G.total; g++) c[g] = std::min(std::max(c[g], 0), 65535);  // estimates are at most 4n
+        for (int q = 0; q < G.count; q++) {  // view q of the group: items prefix[pbase + q] .. prefix[pbase + q + 1]
+            const int i0 = prefix[pbase + q], i1 = prefix[pbase + q + 1];
+            int top = 0;
+            for (int g = i0; g < i1; g++) top = std::max(top, c[g]);
+            bucket.assign((size_t)top + 2, 0);
+            for (int g = i0; g < i1; g++) bucket[top - c[g] + 1]++;  // heaviest first: key top - c
+            for (size_t k = 1; k < bucket.size(); k++) bucket[k] += bucket[k - 1];
+            for (int g = i0; g < i1; g++) plan[at + i0 + bucket[top - c[g]]++] = g;
+        }
+        G.plan_at = at;
+        at += G.total;
+    }
+    if (rc == BIRP_OK) {
+        e = hipMemcpyAsync(b->d_plan, plan.data(), (size_t)at * sizeof(int), hipMemcpyHostToDevice, dev->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "batch plan upload");
+    }
+    (void)hipFree(d_cost);
+    if (rc != BIRP_OK)
+        for (birp_batch::ExactGroup& G : b->groups) G.plan_at = -1;
+    return rc;
+}
+
 // Exact batches: per-view ExactView descriptors (the arithmetic of birp_render's exact path), grouped
-// by sample mode, the tile prefix sums of each group, the volume table; then the graph.
+// by sample mode, the tile prefix sums of each group, the volume table, the tile plans; then the graph.
 int exact_batch_setup(birp_batch* b) {
     birp_device* dev = b->vol->dev;
+    int rc;
     batch_release(b);
     const int n = (int)b->views.size();
     std::vector<ExactView> evs(n);
@@ -1070,8 +1130,7 @@ int exact_batch_setup(birp_batch* b) {
     for (int k = 0; k < n; k++) {
         const birp_volume* vol = b->evols[b->vol_of[k]];
         float* out = (b->host_out_dev ? b->host_out_dev : b->d_images) + b->offsets[k];
-        int rc = exact_view(vol, &b->views[k], out, evs[k]);
-        if (rc) return rc;
+        if ((rc = exact_view(vol, &b->views[k], out, evs[k]))) return rc;
         evs[k].vol = b->vol_of[k];
         mode[k] = exact_mode_of(vol->dv, evs[k]);
     }
@@ -1124,9 +1183,10 @@ int exact_batch_setup(birp_batch* b) {
     if (e == hipSuccess) e = hipMemcpyAsync(b->d_vols, vt.data(), sizeof(DevVolume) * vt.size(), hipMemcpyHostToDevice, dev->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
     if (e != hipSuccess) return hip_fail(e, "exact batch upload");
+    if ((rc = exact_batch_plan(b, prefix)) != BIRP_OK) return rc;
     hipStream_t cs = b->capture_stream;
     if ((e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal)) != hipSuccess) return hip_fail(e, "begin capture");
-    int rc = exact_batch_launch(b, cs);
+    rc = exact_batch_launch(b, cs);
     e = hipStreamEndCapture(cs, &b->graph);
     if (rc != BIRP_OK) return fail(rc, "exact batch capture launch failed");
     if (e != hipSuccess) return hip_fail(e, "end capture");

@@ -105,12 +105,14 @@ struct birp_batch {
     struct ExactGroup {
         int mode = 0, first = 0, count = 0, total = 0;  // views order[first, first + count)
         size_t lds = 0;
+        long long plan_at = -1;  // the group's tile plan in d_plan (total entries), or -1: none
     };
     std::vector<ExactGroup> groups;
     std::vector<int> order;           // view indices, group by group
     std::vector<size_t> count_off;    // each view's first sample-count slot
     size_t count_slots = 0;
     birp::ExactView* d_evs = nullptr;
+    int* d_plan = nullptr;            // tile plans of the groups (each view's tiles, heaviest first)
     unsigned* d_counts = nullptr;     // one slot per wave: its in-cube samples of the last launch
 };
 

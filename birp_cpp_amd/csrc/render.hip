@@ -709,17 +709,21 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_exact_plan(DevVolume v, ExactVie
 }
 
 // Many views (each with its own volume, image and sample-count slots) in one launch: items are
-// the views' tiles, view-major (prefix[k] = first item of view k), and each XCD takes a contiguous
-// band of items as above. Every block renders one tile with exactly the arithmetic of
-// k_render_exact, so each image is bit-identical to its single-view render; with more tiles in
-// flight than one view has, the SIMDs stay fed while the longest rays of a view finish.
+// the views' tiles, view-major (prefix[k] = first item of view k), and block g renders item g.
+// Consecutive blocks go round-robin over the 8 XCDs, so every XCD renders a fair share of every
+// view: contiguous bands, one per XCD, left whole XCDs with the heavy views (a 256-view orbit
+// batch took 6.07 ms in bands and 4.89 ms interleaved). Every block renders one tile with exactly
+// the arithmetic of k_render_exact, so each image is bit-identical to its single-view render; with
+// more tiles in flight than one view has, the SIMDs stay fed while the longest rays of a view finish.
+// `order` (may be null): block b renders item order[b] instead of item b (a batch's tile plan).
 template <int S, int MODE, int P = 1>
 __global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact_multi(const DevVolume* __restrict__ vols,
                                                                    const ExactView* __restrict__ views,
-                                                                   const int* __restrict__ prefix, int nview, int total) {
-    const int per_xcd = (total + 7) / 8;
-    const int g = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    if (g >= total) return;
+                                                                   const int* __restrict__ prefix, int nview, int total,
+                                                                   const int* __restrict__ order) {
+    if ((int)blockIdx.x >= total) return;
+    const int g = order ? order[blockIdx.x] : (int)blockIdx.x;
+    if (g < 0 || g >= total) return;
     int lo = 0, hi = nview - 1;  // the view of item g: the last k with prefix[k] <= g
     while (lo < hi) {
         const int mid = __builtin_amdgcn_readfirstlane((lo + hi + 1) >> 1);
@@ -730,6 +734,41 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact_multi(const DevVolu
     const ExactView& ev = views[lo];
     constexpr int TW = ExactTile<S>::TW;
     exact_block<S, MODE, P>(vols[ev.vol], ev, g - prefix[lo], (ev.W + 2 * TW - 1) / (2 * TW));
+}
+
+// Tile estimates of a multi-view item list (the batch form of k_exact_plan's first phase): thread
+// 4g + w estimates wave w of item g; cost[g] = the sum over the item's four waves.
+template <int S>
+__global__ __launch_bounds__(BIRP_BLOCK) void k_plan_multi(const DevVolume* __restrict__ vols, const ExactView* __restrict__ views,
+                                                           const int* __restrict__ prefix, int nview, int total, int* __restrict__ cost) {
+    constexpr int TW = ExactTile<S>::TW, TH = ExactTile<S>::TH;
+    const long long q = (long long)blockIdx.x * BIRP_BLOCK + threadIdx.x;
+    const int g = (int)(q >> 2), w = (int)(q & 3);
+    int m = 0;
+    if (g < total) {
+        int lo = 0, hi = nview - 1;  // the view of item g: the last k with prefix[k] <= g
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= g) lo = mid;
+            else hi = mid - 1;
+        }
+        const ExactView& ev = views[lo];
+        const DevVolume& v = vols[ev.vol];
+        const int lb = g - prefix[lo], wg_x = (ev.W + 2 * TW - 1) / (2 * TW);
+        const int i0 = ((lb % wg_x) * 2 + (w & 1)) * TW, j0 = ((lb / wg_x) * 2 + (w >> 1)) * TH;
+        if (i0 < ev.W && j0 < ev.H) {
+            const float sc = (float)ev.n / ev.L, rW = 1.f / (float)ev.W, rH = 1.f / (float)ev.H;
+            const int i1 = min(i0 + TW - 1, ev.W - 1), j1 = min(j0 + TH - 1, ev.H - 1);
+            m = plan_ray_len(v, ev, (i0 + i1) >> 1, (j0 + j1) >> 1, sc, rW, rH);
+            m = max(m, plan_ray_len(v, ev, i0, j0, sc, rW, rH));
+            m = max(m, plan_ray_len(v, ev, i1, j0, sc, rW, rH));
+            m = max(m, plan_ray_len(v, ev, i0, j1, sc, rW, rH));
+            m = max(m, plan_ray_len(v, ev, i1, j1, sc, rW, rH));
+        }
+    }
+    m += __shfl_xor(m, 1);
+    m += __shfl_xor(m, 2);
+    if (g < total && w == 0) cost[g] = m;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1495,19 +1534,28 @@ int exact_tiles(int W, int H, int lanes_per_ray) {
 int exact_wave_slots(int W, int H, int lanes_per_ray) { return 4 * exact_tiles(W, H, lanes_per_ray); }
 
 int launch_exact_multi(const DevVolume* vols, const ExactView* views, const int* prefix, int nview, int total, int mode,
-                       size_t lds, void* stream) {
+                       size_t lds, void* stream, const int* order) {
     if (nview <= 0 || total <= 0) return BIRP_OK;
     if (lds > 64 * 1024) return BIRP_ERR_UNSUPPORTED;
     hipStream_t st = (hipStream_t)stream;
-    const int grid = ((total + 7) / 8) * 8;
+    const int grid = total;
     switch (mode) {
-        case SM_QUAD_POS: hipLaunchKernelGGL((k_render_exact_multi<2, SM_QUAD_POS, 2>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total); break;
-        case SM_QUAD_GEN: hipLaunchKernelGGL((k_render_exact_multi<2, SM_QUAD_GEN>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total); break;
-        case SM_LIN_TRI: hipLaunchKernelGGL((k_render_exact_multi<2, SM_LIN_TRI>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total); break;
-        case SM_LIN_NEAREST: hipLaunchKernelGGL((k_render_exact_multi<2, SM_LIN_NEAREST>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total); break;
-        case SM_CMEM: hipLaunchKernelGGL((k_render_exact_multi<4, SM_CMEM>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total); break;
+        case SM_QUAD_POS: hipLaunchKernelGGL((k_render_exact_multi<2, SM_QUAD_POS, 2>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total, order); break;
+        case SM_QUAD_GEN: hipLaunchKernelGGL((k_render_exact_multi<2, SM_QUAD_GEN>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total, order); break;
+        case SM_LIN_TRI: hipLaunchKernelGGL((k_render_exact_multi<2, SM_LIN_TRI>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total, order); break;
+        case SM_LIN_NEAREST: hipLaunchKernelGGL((k_render_exact_multi<2, SM_LIN_NEAREST>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total, order); break;
+        case SM_CMEM: hipLaunchKernelGGL((k_render_exact_multi<4, SM_CMEM>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total, order); break;
         default: return BIRP_ERR_INVALID_ARGUMENT;
     }
+    return hip_status(hipGetLastError());
+}
+
+int launch_plan_multi(const DevVolume* vols, const ExactView* views, const int* prefix, int nview, int total, int mode, int* cost,
+                      void* stream) {
+    if (nview <= 0 || total <= 0 || mode == SM_CMEM) return BIRP_ERR_INVALID_ARGUMENT;
+    const long long threads = 4LL * total;
+    hipLaunchKernelGGL((k_plan_multi<2>), dim3((unsigned)((threads + BIRP_BLOCK - 1) / BIRP_BLOCK)), dim3(BIRP_BLOCK), 0,
+                       (hipStream_t)stream, vols, views, prefix, nview, total, cost);
     return hip_status(hipGetLastError());
 }
 

@@ -182,3 +182,24 @@ def test_async_renders_follow_the_plan_cache(device, cubes):
         got = img.download()
         img.free()
         assert np.array_equal(bits(got), bits(want[(key, id(v))])), f"render {k} ({key}, {v.width}x{v.height})"
+
+
+def test_batch_plans_render_the_same_bits(device, cubes):
+    """Exact batches made with per-view tile plans (the default) and in the natural order: every
+    image and in-cube count is the same, over two cubes, several image sizes and a repeated view."""
+    vols, _ = cubes
+    pos = capi.synth_orbit(24, 11)
+    views = [W.view(*W.SIZES[k % 3], 0.1 if k % 2 else 0.25, pos=tuple(float(c) for c in p)) for k, p in enumerate(pos)]
+    views += [W.view_576x336()] * 3
+    which = [k % 2 for k in range(len(views))]
+    got = {}
+    for pol in (0, 1):
+        capi.exact_order(pol)
+        b = capi.Batch.exact([vols["shell"], vols["signed"]], views, which)
+        b.launch()
+        b.wait()
+        got[pol] = ([b.image(k) for k in range(len(views))], b.samples())
+        b.free()
+    for k in range(len(views)):
+        assert np.array_equal(bits(got[0][0][k]), bits(got[1][0][k])), f"view {k}"
+    assert list(got[0][1]) == list(got[1][1])
