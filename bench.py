@@ -122,10 +122,10 @@ def bench_plain(capi, W, dev, steps, warmup):
         capi.render_async(vol, v, img)
     dev_ms = dev.timer_end()  # waits for the last render
     wall = time.perf_counter() - t0
-    # the same renders under the other tile orders: the XCD-band order without a plan, and a plan
+    # the same renders under the other tile orders: the natural order without a plan, and a plan
     # made on every render (what a stream of new poses pays); the timed loop above reuses its plan
     orders = {}
-    for name, pol in (("band_ms", 0), ("plan_every_render_ms", 2)):
+    for name, pol in (("natural_order_ms", 0), ("plan_every_render_ms", 2)):
         capi.exact_order(pol)
         for _ in range(warmup):
             capi.render_async(vol, v, img)

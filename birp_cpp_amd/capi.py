@@ -433,7 +433,7 @@ class Server:
 
 
 def exact_order(policy: int) -> int:
-    """Set the tile order of single exact renders (0 XCD band, 1 planned and reused, 2 planned
+    """Set the tile order of single exact renders (0 natural order, 1 planned and reused, 2 planned
     every render, -1 the default); returns the policy in force before."""
     prev = C.c_int()
     check(lib().birp_debug_exact_order(int(policy), C.byref(prev)), "birp_debug_exact_order")
@@ -443,9 +443,10 @@ def exact_order(policy: int) -> int:
 def exact_plan(vol: "Volume", view: View) -> np.ndarray:
     """The device's tile plan for `view` over `vol` (tile ids, heaviest estimate first)."""
     nb = C.c_int()
-    cap = 8192
-    order = np.zeros(cap, np.int32)
-    check(lib().birp_debug_exact_plan(vol.h, C.byref(view), order.ctypes.data_as(_IP), cap, C.byref(nb)),
+    probe = np.zeros(1, np.int32)
+    lib().birp_debug_exact_plan(vol.h, C.byref(view), probe.ctypes.data_as(_IP), 0, C.byref(nb))  # sets nb only
+    order = np.zeros(max(nb.value, 1), np.int32)
+    check(lib().birp_debug_exact_plan(vol.h, C.byref(view), order.ctypes.data_as(_IP), order.size, C.byref(nb)),
           "birp_debug_exact_plan")
     return order[:nb.value].copy()
 

@@ -123,7 +123,7 @@ namespace birp {
 // BIRP_PLAN_INTS ints (the order first, then the plan kernel's tile estimates and its ticket,
 // which starts at zero); `replan` launches the plan kernel into it first, otherwise it holds the
 // plan of the same view geometry, cube box and tile shape. ncu: the device's CU count.
-#define BIRP_PLAN_MAX_BLOCKS 8192
+#define BIRP_PLAN_MAX_BLOCKS 16384
 #define BIRP_PLAN_BUCKETS 1024
 #define BIRP_PLAN_INTS (2 * BIRP_PLAN_MAX_BLOCKS + 64)
 struct ExactOrder {

@@ -204,15 +204,16 @@ int default_exact_pipe() {
     }();
     return pipe;
 }
-// BIRP_EXACT_ORDER: "band" renders single exact views in the XCD-band tile order, "replan"
-// plans the tile order on every render, and by default a plan is reused while the view's
-// geometry, the cube's box and the tile shape stay the same. 0 band, 1 cached, 2 replan.
+// BIRP_EXACT_ORDER: "natural" renders exact views in their natural tile order (block b renders
+// tile b), "replan" plans the tile order on every single render, and by default a plan is reused
+// while the view's geometry, the cube's box and the tile shape stay the same. 0 natural,
+// 1 cached, 2 replan.
 // birp_debug_exact_order overrides it (-1: no override).
 static std::atomic<int> g_exact_order_override{-1};
 int exact_order_policy() {
     static int policy = [] {
         const char* e = getenv("BIRP_EXACT_ORDER");
-        if (e && strcmp(e, "band") == 0) return 0;
+        if (e && strcmp(e, "natural") == 0) return 0;
         if (e && strcmp(e, "replan") == 0) return 2;
         return 1;
     }();
@@ -725,7 +726,7 @@ static int render_buffer(birp_device* dev, size_t npix, float** img, unsigned lo
 // The tile order of a single exact render (k_exact_plan): planned on the device when the view's
 // geometry, the cube's box or the tile shape differ from those of the device's last plan, reused
 // otherwise. Views of fewer tiles than CUs or of more than BIRP_PLAN_MAX_BLOCKS, and CMEM views
-// (every ray takes all n samples), keep the XCD-band order (null).
+// (every ray takes all n samples), keep the natural order (null).
 static const ExactOrder* exact_order(birp_device* dev, const DevVolume& dv, const ExactView& ev, int lanes, ExactOrder& ord) {
     const int policy = exact_order_policy();
     if (policy == 0 || !dev->plan_order || dev->ncu <= 0 || dv.kind == SK_CMEM) return nullptr;
@@ -1068,7 +1069,7 @@ int exact_batch_launch(birp_batch* b, hipStream_t st) {
 // launch still walks the batch view by view. The multi-view launch hands blocks to CUs as
 // earlier ones finish, so a view's light tiles fill the gaps its heavy tiles leave instead of
 // the heaviest tiles starting last. CMEM groups (every ray takes all n samples), groups of fewer
-// tiles than CUs, and BIRP_EXACT_ORDER=band keep the natural order.
+// tiles than CUs, and BIRP_EXACT_ORDER=natural keep the natural order.
 static int exact_batch_plan(birp_batch* b, const std::vector<int>& prefix) {
     birp_device* dev = b->vol->dev;
     if (exact_order_policy() == 0 || dev->ncu <= 0) return BIRP_OK;

@@ -547,8 +547,9 @@ __device__ __forceinline__ void exact_block(const DevVolume& v, const ExactView&
 }
 
 // One view: block b renders tile lb(b).
-// Without a plan, consecutive dispatch slots go round-robin over the 8 XCDs, so each XCD is
-// given a contiguous band of tiles (neighbouring rays share cube bricks in its L2).
+// Without a plan, lb = b: consecutive tiles go round-robin over the 8 XCDs, so every XCD renders
+// a fair share of every part of the image (contiguous bands, one per XCD, left the XCDs with the
+// middle rows of the cube's image working longest).
 // With a plan (`order`: the tiles sorted heaviest first by k_exact_plan), slot b takes the
 // sorted tile at its place in a snake over the CUs: round r of ncu slots runs forward when r is
 // even and backward when odd, so that every CU draws one tile from each round and the heavy
@@ -556,16 +557,14 @@ __device__ __forceinline__ void exact_block(const DevVolume& v, const ExactView&
 template <int S, int MODE, int P = 1>
 __global__ __launch_bounds__(BIRP_BLOCK) void k_render_exact(DevVolume v, ExactView ev, int wg_x, int nblocks,
                                                              const int* __restrict__ order, int ncu) {
-    int lb;
+    if ((int)blockIdx.x >= nblocks) return;
+    int lb = blockIdx.x;
     if (order) {
         const int b = blockIdx.x, r = b / ncu, base = r * ncu, c = b - base;
         const int nr = min(ncu, nblocks - base);
         lb = order[base + ((r & 1) ? nr - 1 - c : c)];
-    } else {
-        const int per_xcd = (nblocks + 7) / 8;
-        lb = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     }
-    if (lb >= nblocks) return;
+    if (lb < 0 || lb >= nblocks) return;
     exact_block<S, MODE, P>(v, ev, lb, wg_x);
 }
 
@@ -1488,7 +1487,7 @@ static int launch_exact_s(const DevVolume& v, const ExactView& ev, int pipe, hip
     int wg_x = (ev.W + 2 * TW - 1) / (2 * TW);
     int wg_y = (ev.H + 2 * TH - 1) / (2 * TH);
     int nblocks = wg_x * wg_y;
-    int grid = ((nblocks + 7) / 8) * 8;
+    const int grid = nblocks;
     const int mode = exact_mode_of(v, ev);
     const size_t lds = exact_lds_bytes(v, ev.n, mode);
     if (lds > 64 * 1024) return BIRP_ERR_UNSUPPORTED;
@@ -1504,7 +1503,6 @@ static int launch_exact_s(const DevVolume& v, const ExactView& ev, int pipe, hip
         }
         order = ord->order;
         ncu = ord->ncu;
-        grid = nblocks;
     }
     switch (mode) {
         case SM_QUAD_POS:

@@ -346,10 +346,11 @@ int birp_synth_orbit(int count, float perigee, float apogee, float incl_deg, uin
  * they reach the GPU (host-side failure injection for the serving tests). */
 int birp_debug_inject_failure(birp_device* dev, int count);
 /* Tile order of single exact renders (birp_render, birp_render_async, birp_count_samples):
- * 0 renders in the XCD-band order, 1 deals the tiles heaviest first over the CUs from a plan that
+ * 0 renders in the natural order (block b renders tile b; exact batches are then made without
+ * plans), 1 deals the tiles heaviest first over the CUs from a plan that
  * is reused while the view's geometry, the cube's box and the tile shape stay the same (the
  * default), 2 plans on every render; -1 returns to the default or to what BIRP_EXACT_ORDER
- * ("band", "replan") asks. The order never changes an image. `previous` (may be null) receives
+ * ("natural", "replan") asks. The order never changes an image. `previous` (may be null) receives
  * the policy in force before the call. Process-wide. */
 int birp_debug_exact_order(int policy, int* previous);
 /* Plans the tile order of `view` over a cube on the device and copies it into order[0..n),

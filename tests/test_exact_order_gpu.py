@@ -1,9 +1,9 @@
 """Tile order of single exact renders. A plan (k_exact_plan) sorts a view's tiles by their
-estimated trip counts, heaviest first, and the render deals them over the CUs; the XCD-band
-order needs no plan. Checked here: every plan is a permutation of the view's tiles, ranked by the
+estimated trip counts, heaviest first, and the render deals them over the CUs; the natural
+order (block b renders tile b) needs no plan. Checked here: every plan is a permutation of the view's tiles, ranked by the
 oracle's own per-tile work (the longest in-cube run of samples of each wave, read back from the
 oracle's FP32 render of a cube of ones); images and in-cube counts are bit for bit the same under
-the band order, a reused plan and a plan made on every render, through birp_render and
+the natural order, a reused plan and a plan made on every render, through birp_render and
 birp_render_async; and a reused plan follows changes of pose, image size, cube box and semantics.
 """
 import os
@@ -87,6 +87,7 @@ PLAN_VIEWS = [
     ("ragged", lambda: W.view(601, 333, 0.1, pos=(-3.0, -24.0, 6.0), aim=(8.0, 2.0, -1.0)), capi.SEM_EXTENDED),
     ("far", lambda: W.view(576, 336, 0.1, pos=(5.0, -55.0, 20.0), aim=(5.0, 0.0, 0.0)), capi.SEM_EXTENDED),
     ("inside", lambda: W.view(512, 288, 0.1, pos=(4.0, 1.0, -2.0), aim=(18.0, 6.0, 3.0)), capi.SEM_REFERENCE),
+    ("large", lambda: W.view(1600, 900, 0.25, pos=(2.0, -26.0, 8.0), aim=(6.0, 0.0, 1.0)), capi.SEM_EXTENDED),
 ]
 
 
@@ -133,6 +134,8 @@ ORDER_CASES = [
     ("shell-inside-ref", "shell", lambda: W.view(512, 288, 0.1, pos=(4.0, 1.0, -2.0), aim=(18.0, 6.0, 3.0)), capi.SEM_REFERENCE),
     ("signed", "signed", lambda: W.view(400, 300, 0.05, pos=(6.0, -14.0, 4.0), aim=(5.0, -1.0, 0.0)), capi.SEM_EXTENDED),
     ("nearest", "nearest", lambda: W.view(400, 300, 0.05, pos=(6.0, -14.0, 4.0), aim=(5.0, -1.0, 0.0)), capi.SEM_EXTENDED),
+    # more tiles (18,432) than a plan holds: every policy renders the natural order
+    ("shell-beyond-plans", "shell", lambda: W.view(2048, 1152, 0.5, pos=(1.0, -22.0, 9.0), aim=(9.0, 0.0, 0.0)), capi.SEM_EXTENDED),
 ]
 
 
@@ -147,7 +150,7 @@ def test_orders_render_the_same_bits(cubes, oracle, name, key, mk, sem):
     for pol in (1, 2):
         assert np.array_equal(bits(res[pol][0]), bits(base)), f"policy {pol}"
         assert res[pol][1] == cnt, f"policy {pol}: {res[pol][1]} != {cnt}"
-    if name == "shell-ragged":  # and one of them against the oracle, bit for bit
+    if name in ("shell-ragged", "shell-beyond-plans"):  # and one of them against the oracle, bit for bit
         vals, org, spc, _ = W.cube_256()
         ref, rcnt = oracle.render_f32(oracle.OracleCube(vals, org, spc), oview(oracle, v), sem, nthreads=NT)
         assert np.array_equal(bits(base), bits(ref))
@@ -157,7 +160,7 @@ def test_orders_render_the_same_bits(cubes, oracle, name, key, mk, sem):
 def test_async_renders_follow_the_plan_cache(device, cubes):
     """Device images through birp_render_async, with the plan reused, replaced by a new pose, a
     new image size, another cube and other semantics, and reused again; each equals its
-    band-order render, and every render starts from an image full of NaNs so a tile left out
+    natural-order render, and every render starts from an image full of NaNs so a tile left out
     would show."""
     vols, _ = cubes
     a = W.view_576x336()
