@@ -161,9 +161,10 @@ int launch_sample_points(const DevVolume& v, const float* xyz, long long n, floa
 // k fastest), and the integration of the caller's values (one float per (pixel, k)) into the rows.
 int launch_space_points(const ExactView& ev, int row0, int rows, float* xyz, void* stream);
 int launch_space_integrate(const ExactView& ev, int row0, int rows, const float* vals, float* img, void* stream);
+// tab_entries: the largest nx + ny + nz of the volumes the launch reads (<= 2048; any value for CMEM)
 int launch_fast_batch(const DevVolume* vols_dev, int dtype, const FastPose* poses_dev, const int* tile_prefix_dev,
                       int npose, int total_tiles, unsigned* work_counter, int tile_w, int tile_h,
-                      int lanes_per_ray, int unroll, int grid, void* stream);
+                      int lanes_per_ray, int unroll, int grid, void* stream, int tab_entries);
 int launch_reset_counter(unsigned* counter, void* stream);
 int launch_zero_f32(float* p, size_t n, void* stream);
 int launch_scan_class(const float* values, size_t n, unsigned* flags_dev, void* stream);
@@ -186,7 +187,10 @@ inline int fast_kernel_dtype(const DevVolume& d) {
     const int base = (d.dtype == BIRP_DTYPE_F16 && d.f16_uniform) ? BIRP_KDT_F16_UNIFORM : d.dtype;
     return base | (d.vclass != VC_POSITIVE ? BIRP_KDT_SIGNED : 0);
 }
-int fast_grid_size(int dtype, int lanes_per_ray, int unroll);
+int fast_grid_size(int dtype, int lanes_per_ray, int unroll, int tab_entries);
+// per-wave LDS table words of a fast launch whose largest table has tab_entries entries
+int fast_tab_stride(int dtype, int tab_entries);
+inline int volume_tab_entries(const DevVolume& d) { return d.kind == SK_CMEM ? 0 : d.nx + d.ny + d.nz; }
 
 // birp_render_views_noalloc: its fast launches read their descriptors from a fixed device array
 // (render.hip) laid out as [DevVolume | FastPose x BIRP_NOALLOC_MAX_VIEWS | tile prefix sums of the
@@ -205,5 +209,5 @@ int noalloc_stage_head(const DevVolume& v, const int* prefix, void* scratch, voi
 int noalloc_stage_poses(const FastPose* poses, int first, int count, void* scratch, void* stream);
 // The grid of one persistent launch over `tiles` tiles: the occupancy-limited grid, or about one
 // wave per tile when that is smaller (idle waves cost serialized work-queue claims on exit).
-int fast_grid_for(int dtype, int lanes_per_ray, int unroll, int tiles);
+int fast_grid_for(int dtype, int lanes_per_ray, int unroll, int tiles, int tab_entries);
 }  // namespace birp

@@ -1221,14 +1221,14 @@ int batch_setup(birp_batch* b) {
     if (e == hipSuccess) e = hipMemcpyAsync(b->d_vols, &b->vol->dv, sizeof(DevVolume), hipMemcpyHostToDevice, dev->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
     if (e != hipSuccess) return hip_fail(e, "batch upload");
-    b->grid = fast_grid_for(fast_kernel_dtype(b->vol->dv), t.lanes_per_ray, t.unroll, b->total_tiles);
+    b->grid = fast_grid_for(fast_kernel_dtype(b->vol->dv), t.lanes_per_ray, t.unroll, b->total_tiles, volume_tab_entries(b->vol->dv));
     // capture: reset the work counter (a kernel, not a memset node), then the persistent kernel
     hipStream_t cs = b->capture_stream;
     if ((e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal)) != hipSuccess) return hip_fail(e, "begin capture");
     int rc = launch_reset_counter(b->d_work, cs);
     if (rc == BIRP_OK)
         rc = launch_fast_batch(b->d_vols, fast_kernel_dtype(b->vol->dv), b->d_poses, b->d_prefix, n, b->total_tiles, b->d_work, t.tile_w,
-                               t.tile_h, t.lanes_per_ray, t.unroll, b->grid, cs);
+                               t.tile_h, t.lanes_per_ray, t.unroll, b->grid, cs, volume_tab_entries(b->vol->dv));
     e = hipStreamEndCapture(cs, &b->graph);
     if (rc != BIRP_OK) return fail(rc, "batch capture launch failed");
     if (e != hipSuccess) return hip_fail(e, "end capture");
@@ -1402,7 +1402,8 @@ extern "C" int birp_batch_launch_eager(birp_batch* b) {
     int rc = launch_reset_counter(b->d_work, dev->stream);
     if (rc == BIRP_OK)
         rc = launch_fast_batch(b->d_vols, fast_kernel_dtype(b->vol->dv), b->d_poses, b->d_prefix, (int)b->views.size(),
-                               b->total_tiles, b->d_work, t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll, b->grid, dev->stream);
+                               b->total_tiles, b->d_work, t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll, b->grid, dev->stream,
+                               volume_tab_entries(b->vol->dv));
     if (rc) return fail(rc, "batch launch failed");
     (void)hipEventRecord(b->ev1, dev->stream);
     b->launched = true;
@@ -1512,10 +1513,10 @@ extern "C" int birp_render_views_noalloc(birp_volume* vol, const birp_view* view
         const int* pre_dev = reinterpret_cast<const int*>(s + BIRP_NOALLOC_PREFIX_OFF) + g * (BIRP_NOALLOC_MAX_VIEWS + 1);
         const int total = prefix[g * (BIRP_NOALLOC_MAX_VIEWS + 1) + nfast[g]];
         const birp_tuning& t = gt[g];
-        const int grid = fast_grid_for(kdt, t.lanes_per_ray, t.unroll, total);
+        const int grid = fast_grid_for(kdt, t.lanes_per_ray, t.unroll, total, volume_tab_entries(vol->dv));
         rc = launch_fast_batch(reinterpret_cast<const DevVolume*>(s), kdt, reinterpret_cast<const FastPose*>(s + BIRP_NOALLOC_POSE_OFF) + first,
                                pre_dev, nfast[g], total, reinterpret_cast<unsigned*>(s + BIRP_NOALLOC_CTR_OFF) + g * 256, t.tile_w, t.tile_h,
-                               t.lanes_per_ray, t.unroll, grid, dev->stream);
+                               t.lanes_per_ray, t.unroll, grid, dev->stream, volume_tab_entries(vol->dv));
         if (rc) return fail(rc, "fast launch failed");
         first += nfast[g];
     }
@@ -1566,7 +1567,7 @@ static int render_fast_into(birp_volume* vol, const birp_view* view, float* dimg
     unsigned* dwork = (unsigned*)(db + 3072);
     if ((rc = launch_reset_counter(dwork, dev->stream))) return rc;
     const int g = fast_kernel_dtype(vol->dv);
-    const int grid = fast_grid_for(g, t.lanes_per_ray, t.unroll, prefix[1]);
+    const int grid = fast_grid_for(g, t.lanes_per_ray, t.unroll, prefix[1], volume_tab_entries(vol->dv));
     return launch_fast_batch((const DevVolume*)db, g, (const FastPose*)(db + 1024), (const int*)(db + 2048), 1, prefix[1], dwork,
-                             t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll, grid, dev->stream);
+                             t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll, grid, dev->stream, volume_tab_entries(vol->dv));
 }

@@ -34,7 +34,7 @@ namespace birp {
 
 #define BIRP_BLOCK 256
 #define BIRP_MAX_EXACT_SAMPLES 8192
-#define BIRP_FAST_TAB 2048  // per-wave LDS table entries of the fast kernel (nx+ny+nz)
+#define BIRP_FAST_TAB 2048  // largest per-wave LDS table of the fast kernel (nx+ny+nz entries)
 #define BIRP_RSRC_WORD3 0x00020000
 
 // ------------------------------------------------------------------------------------------
@@ -1202,18 +1202,22 @@ __device__ __forceinline__ void kahan_add(float x, float& sum, float& nc) {
 // the other queues when its own is drained. Every wave leaves after finding all 8 queues empty.
 // KV is a kernel variant (fast_kernel_dtype): the storage kernel in its low bits, plus
 // BIRP_KDT_SIGNED for volumes with values <= 0, whose rays stop at their first negative sample.
+// Each wave keeps its volume's offset tables in tab_stride words of dynamic LDS, sized per launch
+// to the largest table it reads: 768 words for a 256^3 cube, 1536 for 512^3. With the tables sized
+// so, six blocks fit a CU where fixed 2048-word tables let five (the F16 orbit batch: 2.69 -> 2.51
+// ms), and the launch bound keeps the registers within six waves per SIMD.
 template <int KV, int S, int U>
-__global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __restrict__ vols, const FastPose* __restrict__ poses,
-                                                            const int* __restrict__ prefix, int npose, int total,
-                                                            unsigned* counters, int tw_log2) {
+__global__ __launch_bounds__(BIRP_BLOCK, 6) void k_render_fast(const DevVolume* __restrict__ vols, const FastPose* __restrict__ poses,
+                                                               const int* __restrict__ prefix, int npose, int total,
+                                                               unsigned* counters, int tw_log2, int tab_stride) {
     constexpr bool CMEM = KV == BIRP_KDT_CMEM;
     constexpr int DT = CMEM ? BIRP_DTYPE_F32 : (KV & 3);
     constexpr bool SIGNED = CMEM || (KV & BIRP_KDT_SIGNED) != 0;
     static_assert(!SIGNED || U == 1, "signed kernels take one sample per lane per trip");
-    __shared__ unsigned s_tab[BIRP_BLOCK / 64][BIRP_FAST_TAB];
+    extern __shared__ unsigned s_tab[];
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
-    unsigned* tab = s_tab[wave];
+    unsigned* tab = s_tab + wave * tab_stride;
     const int ray = lane / S, sub = lane % S;
     const int tw = 1 << tw_log2;
     const float K = (float)BIRP_SCALE_FOLDED;
@@ -1240,7 +1244,7 @@ __global__ __launch_bounds__(BIRP_BLOCK) void k_render_fast(const DevVolume* __r
         const DevVolume& V = vols[P.vol];
         if (!CMEM && P.vol != cached_vol) {
             const unsigned* src = (DT == BIRP_DTYPE_F32) ? V.qtab : V.otab;
-            const int ntab = V.nx + V.ny + V.nz;
+            const int ntab = min(V.nx + V.ny + V.nz, tab_stride);  // the host sizes tab_stride to cover it
             for (int q = lane; q < ntab; q += 64) tab[q] = src[q];
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -1664,14 +1668,15 @@ static void* fast_kernel_ptr() { return (void*)k_render_fast<DT, S, U>; }
 
 template <int DT, int S>
 static int launch_fast_su(int unroll, int grid, hipStream_t st, const DevVolume* vols, const FastPose* poses, const int* prefix,
-                          int npose, int total, unsigned* counter, int twl) {
+                          int npose, int total, unsigned* counter, int twl, int ts) {
+    const size_t lds = (size_t)(BIRP_BLOCK / 64) * ts * sizeof(unsigned);
     if constexpr ((DT & (BIRP_KDT_SIGNED | BIRP_KDT_CMEM)) != 0) {
-        hipLaunchKernelGGL((k_render_fast<DT, S, 1>), dim3(grid), dim3(BIRP_BLOCK), 0, st, vols, poses, prefix, npose, total, counter, twl);
+        hipLaunchKernelGGL((k_render_fast<DT, S, 1>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, poses, prefix, npose, total, counter, twl, ts);
     } else {
         switch (unroll) {
-            case 4: hipLaunchKernelGGL((k_render_fast<DT, S, 4>), dim3(grid), dim3(BIRP_BLOCK), 0, st, vols, poses, prefix, npose, total, counter, twl); break;
-            case 2: hipLaunchKernelGGL((k_render_fast<DT, S, 2>), dim3(grid), dim3(BIRP_BLOCK), 0, st, vols, poses, prefix, npose, total, counter, twl); break;
-            default: hipLaunchKernelGGL((k_render_fast<DT, S, 1>), dim3(grid), dim3(BIRP_BLOCK), 0, st, vols, poses, prefix, npose, total, counter, twl); break;
+            case 4: hipLaunchKernelGGL((k_render_fast<DT, S, 4>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, poses, prefix, npose, total, counter, twl, ts); break;
+            case 2: hipLaunchKernelGGL((k_render_fast<DT, S, 2>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, poses, prefix, npose, total, counter, twl, ts); break;
+            default: hipLaunchKernelGGL((k_render_fast<DT, S, 1>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, poses, prefix, npose, total, counter, twl, ts); break;
         }
     }
     return hip_status(hipGetLastError());
@@ -1679,25 +1684,32 @@ static int launch_fast_su(int unroll, int grid, hipStream_t st, const DevVolume*
 
 template <int DT>
 static int launch_fast_d(int S, int unroll, int grid, hipStream_t st, const DevVolume* vols, const FastPose* poses,
-                         const int* prefix, int npose, int total, unsigned* counter, int twl) {
-    if (S == 4) return launch_fast_su<DT, 4>(unroll, grid, st, vols, poses, prefix, npose, total, counter, twl);
-    if (S == 2) return launch_fast_su<DT, 2>(unroll, grid, st, vols, poses, prefix, npose, total, counter, twl);
-    return launch_fast_su<DT, 1>(unroll, grid, st, vols, poses, prefix, npose, total, counter, twl);
+                         const int* prefix, int npose, int total, unsigned* counter, int twl, int ts) {
+    if (S == 4) return launch_fast_su<DT, 4>(unroll, grid, st, vols, poses, prefix, npose, total, counter, twl, ts);
+    if (S == 2) return launch_fast_su<DT, 2>(unroll, grid, st, vols, poses, prefix, npose, total, counter, twl, ts);
+    return launch_fast_su<DT, 1>(unroll, grid, st, vols, poses, prefix, npose, total, counter, twl, ts);
+}
+
+int fast_tab_stride(int dtype, int tab_entries) {
+    if (dtype == BIRP_KDT_CMEM) return 0;  // the model has no tables
+    return ((std::max(tab_entries, 1) + 255) / 256) * 256;
 }
 
 // counters: 256 unsigned (8 queues, 128 B apart), reset by launch_reset_counter before each launch.
 int launch_fast_batch(const DevVolume* vols_dev, int dtype, const FastPose* poses_dev, const int* tile_prefix_dev, int npose,
                       int total_tiles, unsigned* work_counter, int tile_w, int tile_h, int lanes_per_ray, int unroll,
-                      int grid, void* stream) {
+                      int grid, void* stream, int tab_entries) {
     int twl = 0;
     while ((1 << twl) < tile_w) twl++;
     if ((1 << twl) != tile_w || tile_w * tile_h * lanes_per_ray != 64) return BIRP_ERR_INVALID_ARGUMENT;
     if (dtype < 0 || dtype >= BIRP_KDT_COUNT) return BIRP_ERR_INVALID_ARGUMENT;
+    if (tab_entries < 0 || tab_entries > BIRP_FAST_TAB) return BIRP_ERR_INVALID_ARGUMENT;
+    const int ts = fast_tab_stride(dtype, tab_entries);
     grid = ((grid + 7) / 8) * 8;
     hipStream_t st = (hipStream_t)stream;
     if (dtype & (BIRP_KDT_SIGNED | BIRP_KDT_CMEM)) unroll = 1;  // the signed and CMEM kernels take one sample per lane per trip
 #define BIRP_LFD(KV) \
-    case KV: return launch_fast_d<KV>(lanes_per_ray, unroll, grid, st, vols_dev, poses_dev, tile_prefix_dev, npose, total_tiles, work_counter, twl);
+    case KV: return launch_fast_d<KV>(lanes_per_ray, unroll, grid, st, vols_dev, poses_dev, tile_prefix_dev, npose, total_tiles, work_counter, twl, ts);
     switch (dtype) {
         BIRP_LFD(0) BIRP_LFD(1) BIRP_LFD(2) BIRP_LFD(3) BIRP_LFD(4) BIRP_LFD(5) BIRP_LFD(6) BIRP_LFD(7) BIRP_LFD(8)
         default: return BIRP_ERR_INVALID_ARGUMENT;
@@ -1705,22 +1717,24 @@ int launch_fast_batch(const DevVolume* vols_dev, int dtype, const FastPose* pose
 #undef BIRP_LFD
 }
 
-int fast_grid_for(int dtype, int lanes_per_ray, int unroll, int tiles) {
+int fast_grid_for(int dtype, int lanes_per_ray, int unroll, int tiles, int tab_entries) {
     // back-to-back single renders with the tuned tiles: 576x336 at n = 800 took 0.10-0.11 ms per
     // render at one tile per wave, 0.12-0.13 ms at two waves per tile or on the full grid; 144x84
     // at n = 160 took 0.033-0.036 ms against 0.086 ms on the full grid
-    return std::min(fast_grid_size(dtype, lanes_per_ray, unroll), (tiles + 3) / 4 + 1);
+    return std::min(fast_grid_size(dtype, lanes_per_ray, unroll, tab_entries), (tiles + 3) / 4 + 1);
 }
 
-int fast_grid_size(int dtype, int lanes_per_ray, int unroll) {
+int fast_grid_size(int dtype, int lanes_per_ray, int unroll, int tab_entries) {
     int dev = 0;
     (void)hipGetDevice(&dev);
-    // cached per device and kernel variant: the occupancy query costs microseconds per launch
+    // cached per device, kernel variant and table size: the occupancy query costs microseconds
     if (dtype & (BIRP_KDT_SIGNED | BIRP_KDT_CMEM)) unroll = 1;
-    static std::atomic<int> cache[16][BIRP_KDT_COUNT][5][5];
+    const int ts = fast_tab_stride(dtype, std::min(std::max(tab_entries, 0), BIRP_FAST_TAB));
+    const int tsb = ts / 256;  // 0..8
+    static std::atomic<int> cache[16][BIRP_KDT_COUNT][5][5][BIRP_FAST_TAB / 256 + 1];
     const bool cacheable = dev >= 0 && dev < 16 && dtype >= 0 && dtype < BIRP_KDT_COUNT && lanes_per_ray <= 4 && unroll <= 4;
     if (cacheable) {
-        const int c = cache[dev][dtype][lanes_per_ray][unroll].load(std::memory_order_relaxed);
+        const int c = cache[dev][dtype][lanes_per_ray][unroll][tsb].load(std::memory_order_relaxed);
         if (c > 0) return c;
     }
     int cus = 0;
@@ -1747,8 +1761,9 @@ int fast_grid_size(int dtype, int lanes_per_ray, int unroll) {
     }
 #undef BIRP_FKS
 #undef BIRP_FK
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BIRP_BLOCK, 0) != hipSuccess || per_cu <= 0) per_cu = 4;
-    if (cacheable) cache[dev][dtype][lanes_per_ray][unroll].store(cus * per_cu, std::memory_order_relaxed);
+    const size_t lds = (size_t)(BIRP_BLOCK / 64) * ts * sizeof(unsigned);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BIRP_BLOCK, lds) != hipSuccess || per_cu <= 0) per_cu = 4;
+    if (cacheable) cache[dev][dtype][lanes_per_ray][unroll][tsb].store(cus * per_cu, std::memory_order_relaxed);
     return cus * per_cu;
 }
 

@@ -304,6 +304,7 @@ int launch_slot(birp_server* S, Slot& s) {
     }
     int nexact = 0;
     size_t exact_lds = 0;
+    int tabs[kGroups] = {};  // each group's largest volume table (nx + ny + nz)
     s.h_eprefix[0] = 0;
     for (size_t k = 0; k < s.reqs.size(); k++) {
         Request* r = s.reqs[k];
@@ -322,6 +323,7 @@ int launch_slot(birp_server* S, Slot& s) {
         }
         birp_tuning t;
         const int g = request_group(r, &t);
+        tabs[g] = std::max(tabs[g], volume_tab_entries(r->vol->dv));
         rc = fast_pose(r->vol, &r->req.view, r->req.volume, out, t.tile_w, t.tile_h, s.h_poses[fill[g]++]);
         if (rc) return rc;
     }
@@ -354,9 +356,9 @@ int launch_slot(birp_server* S, Slot& s) {
         const int kdt = g / 3;
         const int total_tiles = s.h_prefix[base[g] + g + count];
         if ((rc = launch_reset_counter(s.d_work + g * 256, s.stream))) return rc;
-        const int grid = fast_grid_for(kdt, t.lanes_per_ray, t.unroll, total_tiles);
+        const int grid = fast_grid_for(kdt, t.lanes_per_ray, t.unroll, total_tiles, tabs[g]);
         rc = launch_fast_batch(S->d_vols, kdt, s.d_poses + base[g], s.d_prefix + base[g] + g, count, total_tiles, s.d_work + g * 256,
-                               t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll, grid, s.stream);
+                               t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll, grid, s.stream, tabs[g]);
         if (rc) return rc;
     }
     // without a mapped staging, the other results come back to it in one DMA; the completion thread
