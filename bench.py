@@ -87,7 +87,9 @@ def valu_roofline(profile, kern_ms):
     instruction, from a committed rocprofv3 --pmc summary (SQ_ACTIVE_INST_VALU counts one quad-cycle
     per wave64 VALU instruction, SQ_ACTIVE_INST_VALU2 the quad-cycles in which a SIMD issued two, so
     their difference is the quad-cycles with any VALU issue), over this run's measured kernel time,
-    against every SIMD issuing every cycle. This fraction cannot exceed 1."""
+    against every SIMD issuing every cycle at the 2.4 GHz peak clock. This fraction cannot exceed 1.
+    `frac_cycles` is the same ratio over the dispatch's own GRBM_GUI_ACTIVE cycles in the profiled
+    run: it does not depend on the clock the chip ran at (2.1 - 2.3 GHz under these loads)."""
     busy = read_profile_value(profile, "valu_issue_quad_cycles_per_launch")
     insts = read_profile_value(profile, "valu_wave_insts_per_launch")
     if busy is None or insts is None:
@@ -98,6 +100,7 @@ def valu_roofline(profile, kern_ms):
             "lane_instructions_per_s": float(f"{insts * 64 / (kern_ms * 1e-3):.4g}"),
             "dual_issue_frac": read_profile_value(profile, "valu_dual_issue_fraction"),
             "resident_waves_per_simd": read_profile_value(profile, "resident_waves_per_simd"),
+            "frac_cycles": read_profile_value(profile, "valu_issue_occupancy_cycles"),
             "source": "profiles/" + profile}
 
 
