@@ -14,11 +14,11 @@
 // pinned staging the same way, and when a slot's device work is done a completion thread copies
 // those into the callers' buffers (split over a pool of helper threads for large batches) and
 // releases the slot, so the dispatcher keeps launching meanwhile.
-// Every cube request is rendered with the tuning its own view selects (the one fast single renders
-// use) and the same per-tile arithmetic whatever it is batched with, so results do not depend on
-// the mix and equal birp_render's. Requests on CMEM models render with the exact arithmetic (the
-// model's only form) through the multi-view exact kernel, in the same slot launch, and equal
-// birp_render's exact images bit for bit.
+// Every cube request, and every CMEM request that asks for BIRP_MATH_FAST, is rendered with the
+// tuning its own view selects (the one fast single renders use) and the same per-tile arithmetic
+// whatever it is batched with, so results do not depend on the mix and equal birp_render's. Other
+// requests on CMEM models render with the exact arithmetic through the multi-view exact kernel, in
+// the same slot launch, and equal birp_render's exact images bit for bit.
 // No reference counterpart: the reference renders one pose per process run (main.cpp:266-316).
 #include <hip/hip_runtime_api.h>
 
@@ -274,9 +274,9 @@ int request_group(const Request* r, birp_tuning* t) {
     *t = latency_tuning(r->req.view);
     return fast_kernel_dtype(r->vol->dv) * 3 + lane_index(t->lanes_per_ray);
 }
-// CMEM requests render with the exact arithmetic of birp_render (the model's only form) through the
-// multi-view exact kernel; cube requests take the fast kernels.
-bool exact_request(const Request* r) { return r->vol->dv.kind == SK_CMEM; }
+// CMEM requests in BIRP_MATH_EXACT render with the exact arithmetic of birp_render through the
+// multi-view exact kernel; cube requests and fast CMEM requests take the fast kernels.
+bool exact_request(const Request* r) { return r->vol->dv.kind == SK_CMEM && r->req.view.math == BIRP_MATH_EXACT; }
 
 // Launch the admitted requests of slot s (dispatcher thread, lock not held). Group g's poses are
 // h_poses[base[g], base[g+1]) and its tile prefix sums h_prefix[base[g] + g, ... + count_g].
@@ -688,9 +688,11 @@ extern "C" int birp_server_submit(birp_server* S, const birp_request* req, float
         if (req->volume >= 0 && req->volume < (int)S->vols.size()) cmem = S->vols[req->volume]->dv.kind == SK_CMEM;
     }
     birp_view v = req->view;
-    v.math = cmem ? BIRP_MATH_EXACT : BIRP_MATH_FAST;
+    // cubes render with the fast kernels; CMEM models in the form the request names (exact unless
+    // it asks for BIRP_MATH_FAST)
+    v.math = (cmem && req->view.math != BIRP_MATH_FAST) ? BIRP_MATH_EXACT : BIRP_MATH_FAST;
     v.flags = 0;
-    int rc = validate_view(&v, cmem);
+    int rc = validate_view(&v, v.math == BIRP_MATH_EXACT);
     if (rc) return rc;
     Request* r = new Request();
     r->req = *req;

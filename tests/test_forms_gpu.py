@@ -322,9 +322,10 @@ def test_autotune_roundtrip(cube1, tmp_path):
 
 # ---------------------------------------------------------------------------------- server
 def test_server_serves_cmem_models_beside_cubes(device, cube1):
-    """CMEM models are resident volumes too: their requests render with the exact arithmetic in the
-    same slot launches as cube requests, each result bitwise equal to birp_render of its view (exact
-    for CMEM, fast for the cubes), in launches that mix both."""
+    """CMEM models are resident volumes too: their requests render in the math they name (the exact
+    arithmetic through the multi-view exact kernel, or the fast CMEM kernel) in the same slot
+    launches as cube requests, each result bitwise equal to birp_render of its view in that math
+    (fast for the cubes), in launches that mix all of them."""
     vals, org, spc, vols, oc = cube1
     cms = [capi.Volume.cmem(device, capi.cmem_init()),
            capi.Volume.cmem(device, capi.cmem_init(capi.cmem_params(p0=1.0, p1=4, p2=6, p3=3, B=3.6, alpha=-1.0)))]
@@ -334,6 +335,10 @@ def test_server_serves_cmem_models_beside_cubes(device, cube1):
         spaces = [vols[capi.DTYPE_F16], cms[0], cms[1], vols[capi.DTYPE_F32]]
         views = W.orbit_views_mixed(count=40, seed=13)
         which = [k % 4 for k in range(len(views))]
+        # CMEM requests: exact for odd k // 4, fast for even
+        maths = [capi.MATH_EXACT if (w in (1, 2) and (k // 4) % 2) else capi.MATH_FAST for k, w in enumerate(which)]
+        for v, m in zip(views, maths):
+            v.math = m
         ids = [srv.submit(idx[w], v) for v, w in zip(views, which)]
         mixed = 0
         for k, rid in enumerate(ids):
@@ -341,8 +346,7 @@ def test_server_serves_cmem_models_beside_cubes(device, cube1):
             assert st == capi.OK
             mixed += info.batch_size > 1
             v = capi.make_view(views[k].width, views[k].height, views[k].fov_w, views[k].fov_h, tuple(views[k].position),
-                               tuple(views[k].aim), views[k].ray_length, views[k].samples, views[k].semantics,
-                               capi.MATH_EXACT if which[k] in (1, 2) else capi.MATH_FAST)
+                               tuple(views[k].aim), views[k].ray_length, views[k].samples, views[k].semantics, maths[k])
             want, _ = spaces[which[k]].render(v)
             assert np.array_equal(img.view(np.uint32), want.view(np.uint32)), (k, which[k])
             if which[k] in (1, 2):
