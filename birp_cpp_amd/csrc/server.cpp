@@ -14,11 +14,12 @@
 // pinned staging the same way, and when a slot's device work is done a completion thread copies
 // those into the callers' buffers (split over a pool of helper threads for large batches) and
 // releases the slot, so the dispatcher keeps launching meanwhile.
-// Every cube request, and every CMEM request that asks for BIRP_MATH_FAST, is rendered with the
-// tuning its own view selects (the one fast single renders use) and the same per-tile arithmetic
-// whatever it is batched with, so results do not depend on the mix and equal birp_render's. Other
-// requests on CMEM models render with the exact arithmetic through the multi-view exact kernel, in
-// the same slot launch, and equal birp_render's exact images bit for bit.
+// Every request in BIRP_MATH_FAST is rendered with the tuning its own view selects (the one fast
+// single renders use) and the same per-tile arithmetic whatever it is batched with, so results do
+// not depend on the mix and equal birp_render's. Requests in BIRP_MATH_EXACT (FP32 cubes of any
+// interpolation and sign, CMEM models) render with the exact arithmetic through the multi-view
+// exact kernel, one launch per sample mode in the same slot, and equal birp_render's exact images
+// bit for bit.
 // No reference counterpart: the reference renders one pose per process run (main.cpp:266-316).
 #include <hip/hip_runtime_api.h>
 
@@ -185,9 +186,9 @@ struct Slot {
     int* h_prefix = nullptr;      // pinned, max_batch + kGroups tile prefix sums
     int* d_prefix = nullptr;
     unsigned* d_work = nullptr;   // work-queue counters, 256 per kernel group
-    ExactView* h_evs = nullptr;   // pinned: the exact (CMEM) requests' view constants
+    ExactView* h_evs = nullptr;   // pinned: the exact requests' view constants, grouped by sample mode
     ExactView* d_evs = nullptr;
-    int* h_eprefix = nullptr;     // pinned: their tile prefix sums (max_batch + 1)
+    int* h_eprefix = nullptr;     // pinned: each mode group's tile prefix sums (max_batch + 8)
     int* d_eprefix = nullptr;
     float* d_images = nullptr;
     float* h_images = nullptr;    // pinned
@@ -274,9 +275,10 @@ int request_group(const Request* r, birp_tuning* t) {
     *t = latency_tuning(r->req.view);
     return fast_kernel_dtype(r->vol->dv) * 3 + lane_index(t->lanes_per_ray);
 }
-// CMEM requests in BIRP_MATH_EXACT render with the exact arithmetic of birp_render through the
-// multi-view exact kernel; cube requests and fast CMEM requests take the fast kernels.
-bool exact_request(const Request* r) { return r->vol->dv.kind == SK_CMEM && r->req.view.math == BIRP_MATH_EXACT; }
+// Requests in BIRP_MATH_EXACT render with the exact arithmetic of birp_render through the
+// multi-view exact kernel, one launch per sample mode present; the others take the fast kernels.
+bool exact_request(const Request* r) { return r->req.view.math == BIRP_MATH_EXACT; }
+constexpr int kModes = SM_CMEM + 1;
 
 // Launch the admitted requests of slot s (dispatcher thread, lock not held). Group g's poses are
 // h_poses[base[g], base[g+1]) and its tile prefix sums h_prefix[base[g] + g, ... + count_g].
@@ -302,23 +304,40 @@ int launch_slot(birp_server* S, Slot& s) {
         base[g + 1] += base[g];
         fill[g] = base[g];
     }
+    // exact requests: group m (sample mode) holds views h_evs[estart[m], + ecount[m]) and prefix
+    // sums h_eprefix[estart[m] + m, + ecount[m] + 1)
+    int ecount[kModes] = {}, estart[kModes] = {}, efill[kModes] = {};
+    size_t elds[kModes] = {};
+    for (Request* r : s.reqs) {
+        if (!exact_request(r)) continue;
+        ExactView ev;
+        if ((rc = exact_view(r->vol, &r->req.view, nullptr, ev))) return rc;
+        ecount[exact_mode_of(r->vol->dv, ev)]++;
+    }
     int nexact = 0;
-    size_t exact_lds = 0;
+    for (int m = 0; m < kModes; m++) {
+        estart[m] = efill[m] = nexact;
+        nexact += ecount[m];
+        s.h_eprefix[estart[m] + m] = 0;
+    }
     int tabs[kGroups] = {};  // each group's largest volume table (nx + ny + nz)
-    s.h_eprefix[0] = 0;
     for (size_t k = 0; k < s.reqs.size(); k++) {
         Request* r = s.reqs[k];
         // a result in a birp_host_alloc buffer is written there by the kernel (zero-copy), the others
         // into the slot's pinned staging (or its device images when the staging is not mapped)
         float* out = r->dev_out ? r->dev_out : (s.h_images_dev ? s.h_images_dev : s.d_images) + s.offsets[k];
         if (exact_request(r)) {
-            ExactView& ev = s.h_evs[nexact];
+            ExactView ev;
             if ((rc = exact_view(r->vol, &r->req.view, out, ev))) return rc;
             ev.vol = r->req.volume;
-            const int mode = exact_mode_of(r->vol->dv, ev);
-            exact_lds = std::max(exact_lds, exact_lds_bytes(r->vol->dv, r->req.view.samples, mode));
-            s.h_eprefix[nexact + 1] = s.h_eprefix[nexact] + exact_tiles(r->req.view.width, r->req.view.height, exact_multi_lanes(mode));
-            nexact++;
+            const int m = exact_mode_of(r->vol->dv, ev);
+            const int q = efill[m]++;
+            s.h_evs[q] = ev;
+            elds[m] = std::max(elds[m], exact_lds_bytes(r->vol->dv, r->req.view.samples, m));
+            int* pre = s.h_eprefix + estart[m] + m;
+            const long long next = (long long)pre[q - estart[m]] + exact_tiles(r->req.view.width, r->req.view.height, exact_multi_lanes(m));
+            if (next > INT32_MAX) return fail(BIRP_ERR_UNSUPPORTED, "launch of more than 2^31 - 1 tiles");
+            pre[q - estart[m] + 1] = (int)next;
             continue;
         }
         birp_tuning t;
@@ -345,10 +364,15 @@ int launch_slot(birp_server* S, Slot& s) {
     if (e == hipSuccess && n) e = hipMemcpyAsync(s.d_prefix, s.h_prefix, sizeof(int) * (n + kGroups), hipMemcpyHostToDevice, s.stream);
     if (e == hipSuccess && nexact) e = hipMemcpyAsync(s.d_evs, s.h_evs, sizeof(ExactView) * nexact, hipMemcpyHostToDevice, s.stream);
     if (e == hipSuccess && nexact)
-        e = hipMemcpyAsync(s.d_eprefix, s.h_eprefix, sizeof(int) * (nexact + 1), hipMemcpyHostToDevice, s.stream);
+        e = hipMemcpyAsync(s.d_eprefix, s.h_eprefix, sizeof(int) * (nexact + kModes), hipMemcpyHostToDevice, s.stream);
     if (e != hipSuccess) return hip_fail(e, "server descriptor upload");
-    if (nexact && (rc = launch_exact_multi(S->d_vols, s.d_evs, s.d_eprefix, nexact, s.h_eprefix[nexact], SM_CMEM, exact_lds, s.stream)))
-        return fail(rc, "exact launch failed");
+    for (int m = 0; m < kModes; m++) {
+        if (!ecount[m]) continue;
+        const int first = estart[m] + m;
+        rc = launch_exact_multi(S->d_vols, s.d_evs + estart[m], s.d_eprefix + first, ecount[m], s.h_eprefix[first + ecount[m]], m,
+                                elds[m], s.stream);
+        if (rc) return fail(rc, "exact launch failed");
+    }
     for (int g = 0; g < kGroups; g++) {
         const int count = base[g + 1] - base[g];
         if (!count) continue;
@@ -604,8 +628,8 @@ extern "C" int birp_server_create(birp_device* dev, const birp_server_config* cf
         if ((e = hipMalloc(&s.d_work, sizeof(unsigned) * kGroups * 256)) != hipSuccess) { rc = hip_fail(e, "counter"); break; }
         if ((e = hipHostMalloc(&s.h_evs, sizeof(ExactView) * nb, hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
         if ((e = hipMalloc(&s.d_evs, sizeof(ExactView) * nb)) != hipSuccess) { rc = hip_fail(e, "views"); break; }
-        if ((e = hipHostMalloc(&s.h_eprefix, sizeof(int) * (nb + 1), hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
-        if ((e = hipMalloc(&s.d_eprefix, sizeof(int) * (nb + 1))) != hipSuccess) { rc = hip_fail(e, "prefix"); break; }
+        if ((e = hipHostMalloc(&s.h_eprefix, sizeof(int) * (nb + 8), hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
+        if ((e = hipMalloc(&s.d_eprefix, sizeof(int) * (nb + 8))) != hipSuccess) { rc = hip_fail(e, "prefix"); break; }
         if (cfg->reserve_pixels) {
             if ((rc = slot_alloc_images(s, cfg->reserve_pixels))) break;
             s.held = cfg->reserve_pixels * 4;
@@ -682,16 +706,20 @@ extern "C" int birp_server_add_volume(birp_server* S, birp_volume* vol, int* ind
 
 extern "C" int birp_server_submit(birp_server* S, const birp_request* req, float* result, uint64_t* id) {
     if (!S || !req || !result || !id) return fail(BIRP_ERR_INVALID_ARGUMENT, "null argument");
-    bool cmem = false;
+    bool exact_ok = true;  // the exact arithmetic reads an FP32 cube (or a CMEM model)
     {
         std::lock_guard<std::mutex> lk(S->mu);
-        if (req->volume >= 0 && req->volume < (int)S->vols.size()) cmem = S->vols[req->volume]->dv.kind == SK_CMEM;
+        if (req->volume >= 0 && req->volume < (int)S->vols.size()) {
+            const DevVolume& d = S->vols[req->volume]->dv;
+            exact_ok = d.kind == SK_CMEM || d.linear != nullptr;
+        }
     }
     birp_view v = req->view;
-    // cubes render with the fast kernels; CMEM models in the form the request names (exact unless
-    // it asks for BIRP_MATH_FAST)
-    v.math = (cmem && req->view.math != BIRP_MATH_FAST) ? BIRP_MATH_EXACT : BIRP_MATH_FAST;
+    // a request renders in the math its view names: BIRP_MATH_FAST through the fast kernels, any
+    // other value as BIRP_MATH_EXACT through the multi-view exact kernel
+    v.math = req->view.math == BIRP_MATH_FAST ? BIRP_MATH_FAST : BIRP_MATH_EXACT;
     v.flags = 0;
+    if (v.math == BIRP_MATH_EXACT && !exact_ok) return fail(BIRP_ERR_UNSUPPORTED, "exact requests need an FP32 cube");
     int rc = validate_view(&v, v.math == BIRP_MATH_EXACT);
     if (rc) return rc;
     Request* r = new Request();

@@ -285,9 +285,9 @@ typedef struct birp_server_config {
 
 typedef struct birp_request {
     int volume;     /* index returned by birp_server_add_volume */
-    birp_view view; /* cube volumes render with BIRP_MATH_FAST; CMEM models in the view's math
-                       (BIRP_MATH_FAST, or else BIRP_MATH_EXACT), bit-identical to birp_render of
-                       that view in that math; flags are ignored */
+    birp_view view; /* rendered in the view's math: BIRP_MATH_FAST, or else BIRP_MATH_EXACT
+                       (FP32 cubes and CMEM models; BIRP_ERR_UNSUPPORTED for 16-bit cubes),
+                       bit-identical to birp_render of that view in that math; flags are ignored */
 } birp_request;
 
 typedef struct birp_result_info {

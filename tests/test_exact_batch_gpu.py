@@ -155,3 +155,37 @@ def test_exact_batch_argument_checks(spaces):
         assert e.value.status == capi.ERR_UNSUPPORTED
     finally:
         b.free()
+
+
+def test_server_exact_requests_bitwise(device, spaces):
+    """Served requests in BIRP_MATH_EXACT over a positive cube, a signed cube (general quad mode), a
+    nearest-sampled cube and a CMEM model share slot launches (one exact launch per sample mode)
+    with fast requests, and every exact result equals birp_render of its view bit for bit; a 16-bit
+    cube refuses exact requests at submission."""
+    vols, _ = spaces
+    views, which = mixed_views(40, seed=77)
+    maths = [capi.MATH_FAST if (k % 5 == 0 and which[k] != 2) else capi.MATH_EXACT for k in range(len(views))]
+    for v, m in zip(views, maths):
+        v.math = m
+    vals128, org128, spc128, _ = W.cube_128()
+    v16 = capi.Volume.upload(device, vals128, org128, spc128, dtype=capi.DTYPE_F16)
+    srv = capi.Server(device, memory_budget=8 << 30, max_batch=24, max_inflight=2, batch_window_ms=1.0)
+    try:
+        idx = [srv.add_volume(v) for v in vols]
+        i16 = srv.add_volume(v16)
+        ids = [srv.submit(idx[w], v) for v, w in zip(views, which)]
+        batched = 0
+        for k, rid in enumerate(ids):
+            st, img, info = srv.wait(rid, 60000.0)
+            assert st == capi.OK, (k, st)
+            batched += info.batch_size > 1
+            want, _ = vols[which[k]].render(views[k])
+            assert np.array_equal(bits(img), bits(want)), (k, which[k], maths[k])
+        assert batched > 0
+        bad = capi.make_view(64, 48, 20.0, 20.0, (0.0, -20.0, 10.0), (10.0, 0.0, 0.0), 40.0, 100, capi.SEM_EXTENDED, capi.MATH_EXACT)
+        with pytest.raises(capi.BirpError) as e:
+            srv.submit(i16, bad)
+        assert e.value.status == capi.ERR_UNSUPPORTED
+    finally:
+        srv.destroy()
+        v16.free()
