@@ -456,8 +456,9 @@ def _serve(capi, srv, idx, which, views, t_arr, bufs):
 def bench_serving(capi, W, dev, num=2000, rate=200.0):
     """Config-5 serving: 8 resident cubes, Poisson arrivals at `rate`, then the same requests as
     one burst (the server saturated; timed after one untimed burst), both again with pinned
-    response buffers (birp_host_alloc), then the same requests through the plain path (one
-    birp_render and host sync per request) for the renders/s comparison."""
+    response buffers (birp_host_alloc), the FP32 cubes' requests in the exact arithmetic as one
+    burst, then the same requests through the plain path (one birp_render and host sync per
+    request) for the renders/s comparisons."""
     cubes = []
     for k, (n, dt) in enumerate(W.serving_cubes()):
         vals, org, spc, _ = capi.synth_shell(n, (7.0, 12.0), 0.1, seed=500 + k)
@@ -485,8 +486,24 @@ def bench_serving(capi, W, dev, num=2000, rate=200.0):
     out["poisson_pinned_results"] = _serve(capi, srv, idx, which, views, t_arr, pinned)
     _serve(capi, srv, idx, which, views, None, pinned)
     out["saturated_pinned_results"] = _serve(capi, srv, idx, which, views, None, pinned)
+    # the requests on the FP32 cubes again in BIRP_MATH_EXACT: the reference's own images, bit for
+    # bit, served through the multi-view exact kernel; against the same requests as single exact renders
+    dts = [dt for _, dt in W.serving_cubes()]
+    ex = [k for k in range(num) if dts[which[k]] == capi.DTYPE_F32]
+    ex_views = [capi.make_view(v.width, v.height, v.fov_w, v.fov_h, list(v.position), list(v.aim), v.ray_length, v.samples,
+                               v.semantics, capi.MATH_EXACT) for v in (views[k] for k in ex)]
+    ex_which = [which[k] for k in ex]
+    ex_bufs = [bufs[k] for k in ex]
+    _serve(capi, srv, idx, ex_which, ex_views, None, ex_bufs)
+    out["saturated_exact_fp32_cubes"] = _serve(capi, srv, idx, ex_which, ex_views, None, ex_bufs)
     srv.destroy()
     del pinned
+    t0 = time.perf_counter()
+    for w, v, b in zip(ex_which, ex_views, ex_bufs):
+        cubes[w].render(v, out=b)
+    ex_plain = len(ex_views) / (time.perf_counter() - t0)
+    out["saturated_exact_fp32_cubes"]["plain_renders_per_s_same_requests"] = round(ex_plain, 1)
+    out["saturated_exact_fp32_cubes"]["speedup_vs_plain"] = round(out["saturated_exact_fp32_cubes"]["renders_per_s"] / ex_plain, 2)
     t0 = time.perf_counter()
     for k in range(num):
         cubes[which[k]].render(views[k], out=bufs[k])
