@@ -130,6 +130,7 @@ struct ExactOrder {
     int* order;
     int ncu;
     bool replan;
+    int slot;  // the host's plan slot (bookkeeping only)
 };
 int launch_exact(const DevVolume& v, const ExactView& ev, int lanes_per_ray, int pipe, void* stream,
                  const ExactOrder* ord = nullptr);

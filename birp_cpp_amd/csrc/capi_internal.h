@@ -47,12 +47,18 @@ struct birp_device {
     bool count_known = false;          // false: clear the counter before its next use
     std::atomic<int> inject_failures{0};
     void* noalloc_scratch = nullptr;   // this GPU's fixed scratch array (birp_render_views_noalloc)
-    // single exact renders: the device's tile plan buffer (BIRP_PLAN_INTS ints, the order first)
-    // and what it was planned for
+    // single exact renders: kPlanSlots tile plans (BIRP_PLAN_INTS ints each, the order first) and
+    // what each was planned for, and the last kSeen views rendered without a plan
+    static constexpr int kPlanSlots = 4, kSeen = 4;
     int* plan_order = nullptr;
     int ncu = 0;
-    ExactPlanKey plan_key{};
-    bool plan_valid = false;
+    ExactPlanKey plan_key[kPlanSlots] = {};
+    bool plan_valid[kPlanSlots] = {};
+    unsigned long long plan_used[kPlanSlots] = {};  // last use (plan_clock), for LRU replacement
+    unsigned long long plan_clock = 0;
+    ExactPlanKey seen_key[kSeen] = {};
+    bool seen_valid[kSeen] = {};
+    int seen_next = 0;
 };
 
 struct birp_volume {

@@ -348,11 +348,12 @@ int birp_synth_orbit(int count, float perigee, float apogee, float incl_deg, uin
 int birp_debug_inject_failure(birp_device* dev, int count);
 /* Tile order of single exact renders (birp_render, birp_render_async, birp_count_samples):
  * 0 renders in the natural order (block b renders tile b; exact batches are then made without
- * plans), 1 deals the tiles heaviest first over the CUs from a plan that
- * is reused while the view's geometry, the cube's box and the tile shape stay the same (the
- * default), 2 plans on every render; -1 returns to the default or to what BIRP_EXACT_ORDER
- * ("natural", "replan") asks. The order never changes an image. `previous` (may be null) receives
- * the policy in force before the call. Process-wide. */
+ * plans); 1, the default, renders a view's first time in the natural order and, when the view
+ * comes back, deals the tiles heaviest first over the CUs from a plan that is kept (four per
+ * device) and reused while the view's geometry, the cube's box and the tile shape stay the same;
+ * 2 plans on every render. -1 returns to the default or to what BIRP_EXACT_ORDER ("natural",
+ * "replan") asks. The order never changes an image. `previous` (may be null) receives the policy
+ * in force before the call. Process-wide. */
 int birp_debug_exact_order(int policy, int* previous);
 /* Plans the tile order of `view` over a cube on the device and copies it into order[0..n),
  * n = *nblocks (the view's tile count, also set when the call fails for capacity < n). */

@@ -120,8 +120,10 @@ def test_plan_of_dead_rays_is_still_a_permutation(cubes):
 
 
 def render_all_orders(vol, v):
+    """Policy 0 (natural order), 1 (a view's first render in the natural order, then planned once it
+    comes back: the second render is the planned one) and 2 (planned on every render)."""
     out = {}
-    for pol in (0, 1, 2):
+    for pol in (0, 1, 1, 2):
         capi.exact_order(pol)
         img, st = vol.render(v)
         out[pol] = (img, st.samples_in_cube)
@@ -158,17 +160,19 @@ def test_orders_render_the_same_bits(cubes, oracle, name, key, mk, sem):
 
 
 def test_async_renders_follow_the_plan_cache(device, cubes):
-    """Device images through birp_render_async, with the plan reused, replaced by a new pose, a
-    new image size, another cube and other semantics, and reused again; each equals its
-    natural-order render, and every render starts from an image full of NaNs so a tile left out
-    would show."""
+    """Device images through birp_render_async, with plans made when a view comes back, reused,
+    replaced by a new pose, a new image size, another cube and other semantics, alternating
+    between views held in different plan slots, and reused again; each equals its natural-order
+    render, and every render starts from an image full of NaNs so a tile left out would show."""
     vols, _ = cubes
     a = W.view_576x336()
     b = W.view(576, 336, 0.1, pos=(-2.0, -22.0, 12.0), aim=(9.0, 1.0, 0.0))
     c = W.view(640, 400, 0.1)
     d = W.view(576, 336, 0.1)
     d.semantics = capi.SEM_REFERENCE
-    seq = [("shell", a), ("shell", a), ("shell", b), ("shell", a), ("shell", c), ("signed", a), ("shell", d), ("shell", a)]
+    seq = [("shell", a), ("shell", a), ("shell", b), ("shell", a), ("shell", c), ("signed", a), ("shell", d), ("shell", a),
+           ("shell", b), ("shell", c), ("shell", b), ("shell", c), ("signed", a), ("signed", a), ("shell", d), ("shell", d),
+           ("shell", a), ("shell", b)]
     capi.exact_order(0)
     want = {}
     for key, v in seq:
