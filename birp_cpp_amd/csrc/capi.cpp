@@ -1048,15 +1048,21 @@ int choose_tuning(const birp_volume* vol, const std::vector<birp_view>& views, b
 }
 
 // The tuning of a fast render launched alone or in a server batch, chosen for latency from the
-// view itself (so a request's result never depends on what it is batched with): an image with
-// few rays splits each ray over 2 or 4 lanes, shortening every lane's chain of dependent sample
-// gathers. Measured on single launches (256^3 cube, F32 and F16): 144x84 at n = 800 took 0.036
-// ms with 4 lanes against 0.06 ms with 1, at n = 160-320 2 lanes were best; 288x168 at n = 800
-// 0.053 ms with 2 lanes against 0.06 ms; 576x336 was fastest with 1. A CMEM model costs about 25
-// transcendentals per sample, whose chains want more waves per SIMD than a cube's gathers: its
-// views take 4 lanes per ray over 4x4 rays whatever their size (576x336 at n = 800: 0.724 ms,
-// against 0.817 ms with 1 lane per ray and 0.733 ms with 2).
-birp_tuning latency_tuning(const birp_view& v, bool cmem) {
+// view and the volume's storage alone (so a request's result never depends on what it is batched
+// with): an image with few rays splits each ray over 2 or 4 lanes, shortening every lane's chain
+// of dependent sample gathers. Measured on back-to-back single launches of 6 orbit poses (256^3
+// cube), ms per render for 1 / 2 / 4 lanes per ray:
+//   144x84,  n = 160: F32 0.028 / 0.030 / 0.030, F16 0.027 / 0.024 / 0.027
+//   144x84,  n = 320: F32 0.036 / 0.039 / 0.033, F16 0.034 / 0.027 / 0.028
+//   144x84,  n = 800: F32 0.060 / 0.063 / 0.044, F16 0.054 / 0.037 / 0.032
+//   288x168, n = 800: F32 0.059 / 0.067 / 0.073, F16 0.054 / 0.049 / 0.063
+//   576x336 and 288x168 at n <= 320: 1 lane, by 20-70%
+// (BF16 as F32). A CMEM model costs about 25 transcendentals per sample, whose chains want more
+// waves per SIMD than a cube's gathers: its views take 4 lanes per ray over 4x4 rays whatever
+// their size (576x336 at n = 800: 0.724 ms, against 0.817 ms with 1 lane and 0.733 ms with 2).
+birp_tuning latency_tuning(const birp_view& v, const DevVolume& vol) {
+    const bool cmem = vol.kind == SK_CMEM;
+    const bool f16 = vol.dtype == BIRP_DTYPE_F16;
     birp_tuning t;
     if (cmem) {
         t.lanes_per_ray = 4;
@@ -1067,8 +1073,8 @@ birp_tuning latency_tuning(const birp_view& v, bool cmem) {
     }
     const long long rays = (long long)v.width * v.height;
     int lanes = 1;
-    if (rays <= 20000) lanes = v.samples >= 600 ? 4 : 2;
-    else if (rays <= 60000 && v.samples >= 600) lanes = 2;
+    if (rays <= 20000) lanes = v.samples >= 600 ? 4 : (f16 ? 2 : (v.samples >= 240 ? 4 : 1));
+    else if (rays <= 60000 && v.samples >= 600 && f16) lanes = 2;
     t.lanes_per_ray = lanes;
     t.unroll = 4;
     t.tile_w = 8;
@@ -1527,7 +1533,7 @@ extern "C" int birp_render_views_noalloc(birp_volume* vol, const birp_view* view
     for (int g = 0; g < 3; g++) {  // lanes per ray 1, 2, 4: poses of group g are contiguous
         for (int k = 0; k < count; k++) {
             if (views[k].math != BIRP_MATH_FAST) continue;
-            const birp_tuning t = latency_tuning(views[k], vol->dv.kind == SK_CMEM);
+            const birp_tuning t = latency_tuning(views[k], vol->dv);
             if (t.lanes_per_ray != (1 << g)) continue;
             int rc = fast_pose(vol, &views[k], 0, images_dev[k], t.tile_w, t.tile_h, poses[nf]);
             if (rc) return rc;
@@ -1572,7 +1578,7 @@ extern "C" int birp_render_views_noalloc(birp_volume* vol, const birp_view* view
 static int render_fast_into(birp_volume* vol, const birp_view* view, float* dimg) {
     birp_device* dev = vol->dev;
     if (view->flags & BIRP_RENDER_ACCUMULATE) return fail(BIRP_ERR_UNSUPPORTED, "fast renders do not accumulate");
-    const birp_tuning t = latency_tuning(*view, vol->dv.kind == SK_CMEM);
+    const birp_tuning t = latency_tuning(*view, vol->dv);
     FastPose fp;
     int rc = fast_pose(vol, view, 0, dimg, t.tile_w, t.tile_h, fp);
     if (rc) return rc;

@@ -131,7 +131,7 @@ int exact_view(const birp_volume* vol, const birp_view* v, float* out_dev, Exact
 int tiles_of(const birp_view& v, int tile_w, int tile_h);
 bool take_injected_failure(birp_device* dev);
 int choose_tuning(const birp_volume* vol, const std::vector<birp_view>& views, birp_tuning& t);
-birp_tuning latency_tuning(const birp_view& v, bool cmem);
+birp_tuning latency_tuning(const birp_view& v, const birp::DevVolume& vol);
 // true when [p, p + bytes) lies inside one birp_host_alloc allocation (pinned host memory)
 bool pinned_range(const void* p, size_t bytes);
 // its device address when the allocation is mapped into the device's address space, else null

@@ -272,7 +272,7 @@ int slot_alloc_images(Slot& s, size_t pixels) {
 constexpr int kGroups = BIRP_KDT_COUNT * 3;
 int lane_index(int lanes) { return lanes == 4 ? 2 : (lanes == 2 ? 1 : 0); }
 int request_group(const Request* r, birp_tuning* t) {
-    *t = latency_tuning(r->req.view, r->vol->dv.kind == SK_CMEM);
+    *t = latency_tuning(r->req.view, r->vol->dv);
     return fast_kernel_dtype(r->vol->dv) * 3 + lane_index(t->lanes_per_ray);
 }
 // Requests in BIRP_MATH_EXACT render with the exact arithmetic of birp_render through the
