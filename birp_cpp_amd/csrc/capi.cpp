@@ -184,17 +184,22 @@ bool take_injected_failure(birp_device* dev) {
     return false;
 }
 
-// Tuning knobs of the exact path, read once: BIRP_EXACT_LANES (lanes per ray: 1, 2, 4; by
-// default 2 for cubes and 4 for CMEM, the measured best of each), BIRP_EXACT_PIPE (samples in
-// flight per lane in the positive-cube loop: 1..4) and BIRP_EXACT_LAYOUT ("linear" reads the
-// [z][y][x] copy instead of the quad bricks).
-int default_exact_lanes(bool cmem) {
+// Tuning knobs of the exact path, read once: BIRP_EXACT_LANES (lanes per ray: 1, 2, 4),
+// BIRP_EXACT_PIPE (samples in flight per lane in the positive-cube loop: 1..4) and
+// BIRP_EXACT_LAYOUT ("linear" reads the [z][y][x] copy instead of the quad bricks).
+// By default a single exact render takes 4 lanes per ray for CMEM and for images of at most
+// 60,000 rays, whose few waves are bound by the chain of each ray's samples (ms per render over 6
+// orbit poses, 1 / 2 / 4 lanes: 144x84 at n = 800 0.080 / 0.045 / 0.026, 288x168 at n = 800
+// 0.079 / 0.047 / 0.032), and 2 for larger images (576x336 at n = 800: 0.088 / 0.077 / 0.078; the
+// headline pose 0.105 with 2, 0.148 with 4).
+int default_exact_lanes(bool cmem, int W, int H) {
     static int lanes = [] {
         const char* e = getenv("BIRP_EXACT_LANES");
         int v = e ? atoi(e) : 0;
         return (v == 1 || v == 2 || v == 4) ? v : 0;
     }();
-    return lanes ? lanes : (cmem ? 4 : 2);
+    if (lanes) return lanes;
+    return (cmem || (long long)W * H <= 60000) ? 4 : 2;
 }
 int default_exact_pipe() {
     static int pipe = [] {
@@ -356,7 +361,7 @@ extern "C" int birp_debug_exact_plan(birp_volume* vol, const birp_view* view, in
     (void)hipSetDevice(dev->ordinal);
     ExactView ev;
     if ((rc = exact_view(vol, view, nullptr, ev))) return rc;
-    const int lanes = default_exact_lanes(false);
+    const int lanes = default_exact_lanes(false, ev.W, ev.H);
     const int nb = exact_tiles(ev.W, ev.H, lanes);
     *nblocks = nb;
     if (nb > BIRP_PLAN_MAX_BLOCKS) return fail(BIRP_ERR_UNSUPPORTED, "view has more tiles than a plan holds");
@@ -795,7 +800,7 @@ static int render_exact_into(birp_volume* vol, const birp_view* view, float* dim
     if (take_injected_failure(dev)) return fail(BIRP_ERR_INJECTED, "injected failure");
     DevVolume dv = vol->dv;
     if (exact_force_linear() && dv.vclass != VC_NONFINITE && dv.kind == SK_CUBE) dv.vclass = VC_NONFINITE;
-    const int lanes = default_exact_lanes(dv.kind == SK_CMEM);
+    const int lanes = default_exact_lanes(dv.kind == SK_CMEM, ev.W, ev.H);
     ExactOrder ord{};
     const ExactOrder* use = exact_order(dev, dv, ev, lanes, ord);
     rc = launch_exact(dv, ev, lanes, default_exact_pipe(), dev->stream, use);
@@ -823,7 +828,8 @@ extern "C" int birp_render(birp_volume* vol, const birp_view* view, float* image
     const bool exact = view->math == BIRP_MATH_EXACT;
     float* mapped = pinned_device_ptr(image, npix * 4);
     const size_t counts_at = mapped ? 256 : 256 + ((npix * 4 + 255) & ~(size_t)255);
-    const int nslots = exact ? exact_wave_slots(view->width, view->height, default_exact_lanes(vol->dv.kind == SK_CMEM)) : 0;
+    const int nslots = exact ? exact_wave_slots(view->width, view->height,
+                                                default_exact_lanes(vol->dv.kind == SK_CMEM, view->width, view->height)) : 0;
     if ((rc = pinned_staging(dev, counts_at + (size_t)nslots * 4))) return rc;
     float* staged = nullptr;  // the image's place in the pinned staging, when the kernel writes it
     unsigned* counts_dev = nullptr;
