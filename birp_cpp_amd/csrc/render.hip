@@ -58,16 +58,16 @@ __device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
     return v;
 }
 
-// DPP quad_perm broadcast: every lane of a quad receives lane M of its quad (S == 4), or
-// lane M of its pair (S == 2).
-template <int S, int M>
-__device__ __forceinline__ int quad_bcast(int v) {
-    constexpr int ctrl = (S == 4) ? (M | (M << 2) | (M << 4) | (M << 6)) : (M | (M << 2) | ((2 + M) << 4) | ((2 + M) << 6));
-    return __builtin_amdgcn_update_dpp(0, v, ctrl, 0xF, 0xF, true);
+// DPP row_shl:N: lane l receives lane l + N of its 16-lane row (0 past the row's end). With
+// the exact kernels' lane layout, sub-lane 0 of a ray reads its sub-lane m at N = 4m.
+template <int N>
+__device__ __forceinline__ int row_next(int v) {
+    if constexpr (N == 0) return v;
+    else return __builtin_amdgcn_update_dpp(0, v, 0x100 + N, 0xF, 0xF, true);
 }
-template <int S, int M>
-__device__ __forceinline__ float quad_bcast_f(float v) {
-    return __int_as_float(quad_bcast<S, M>(__float_as_int(v)));
+template <int N>
+__device__ __forceinline__ float row_next_f(float v) {
+    return __int_as_float(row_next<N>(__float_as_int(v)));
 }
 
 // std::lerp (libstdc++ 11, cmath:1894-1908) for any finite or non-finite input.
@@ -244,20 +244,17 @@ __device__ __forceinline__ float scale_pos(float vw) {
     return s;
 }
 
-// acc += the S sub-lane samples of this ray, in sub-lane (= k) order.
+// acc += the S sub-lane samples of this ray, in sub-lane (= k) order. Exact on sub-lane 0 of
+// the ray (the lane that stores the pixel); the other sub-lanes' sums are not used.
 template <int S>
 __device__ __forceinline__ float add_in_order(float acc, float s) {
-    if constexpr (S == 1) {
-        return acc + s;
-    } else {
-        acc = acc + quad_bcast_f<S, 0>(s);
-        acc = acc + quad_bcast_f<S, 1>(s);
-        if constexpr (S == 4) {
-            acc = acc + quad_bcast_f<S, 2>(s);
-            acc = acc + quad_bcast_f<S, 3>(s);
-        }
-        return acc;
+    acc = acc + s;
+    if constexpr (S >= 2) acc = acc + row_next_f<4>(s);
+    if constexpr (S == 4) {
+        acc = acc + row_next_f<8>(s);
+        acc = acc + row_next_f<12>(s);
     }
+    return acc;
 }
 
 // Trilinear sample from the [z][y][x] layout with floor/ceil corners (DataCube.cpp:224-270):
@@ -393,6 +390,24 @@ template <int S> struct ExactTile {
     static constexpr int TH = (64 / S) / TW;
 };
 
+// The lanes of a wave and the rays they trace. The wave's TW x TH rays are cut into 2x2 pixel
+// blocks; a block takes 4S lanes of one 16-lane row, its four pixels on the four lanes of a
+// quad and its S sub-lanes on S consecutive quads. A 16-byte-per-lane record load is served a
+// quad (64 B) per cycle, one cycle per distinct cache line in the quad: the four rays of a 2x2
+// block sample the same k at points less than a voxel apart, mostly inside one line, where a
+// quad of two rays' consecutive samples spans two (22.5 line lookups per 64-lane load against
+// 32.5 for the 256^3 cube's 576x336 x 800 view, by the access model that reproduces the
+// measured 32.2). Sub-lane m of a ray sits 4m lanes above sub-lane 0 (row_next<4m>).
+template <int S>
+__device__ __forceinline__ void exact_lane(int lane, int& ti, int& tj, int& sub) {
+    constexpr int TW = ExactTile<S>::TW;
+    const int r16 = lane & 15, quad = r16 >> 2;
+    sub = quad & (S - 1);
+    const int blk = (lane >> 4) * (4 / S) + quad / S;  // 2x2 block of the wave, row-major
+    ti = (blk % (TW / 2)) * 2 + (lane & 1);
+    tj = (blk / (TW / 2)) * 2 + ((lane >> 1) & 1);
+}
+
 
 // ------------------------------------------------------------------------------------------
 // the exact render: one block renders one 2x2-wave tile `lb` of a view (wg_x tiles per row of
@@ -424,9 +439,10 @@ __device__ __forceinline__ void exact_block(const DevVolume& v, const ExactView&
 
     constexpr int TW = ExactTile<S>::TW, TH = ExactTile<S>::TH;
     const int wave = threadIdx.x >> 6, lane = lane_id();
-    const int ray = lane / S, sub = lane % S;
-    const int i = ((lb % wg_x) * 2 + (wave & 1)) * TW + ray % TW;
-    const int j = ((lb / wg_x) * 2 + (wave >> 1)) * TH + ray / TW;
+    int ti, tj, sub;
+    exact_lane<S>(lane, ti, tj, sub);
+    const int i = ((lb % wg_x) * 2 + (wave & 1)) * TW + ti;
+    const int j = ((lb / wg_x) * 2 + (wave >> 1)) * TH + tj;
     const bool valid = (i < ev.W) && (j < ev.H);
 
     float dx = 0.f, dy = 0.f, dz = 1.f;
@@ -512,17 +528,19 @@ __device__ __forceinline__ void exact_block(const DevVolume& v, const ExactView&
                 added += (add && !stop) ? 1u : 0u;
                 alive = alive && !stop;
             } else {
+                // sub-lane 0 gathers the ray's S samples in k order (the other sub-lanes'
+                // sums and stops are not used)
                 float sv[S];
                 int av[S];
-                sv[0] = quad_bcast_f<S, 0>(s);
-                av[0] = quad_bcast<S, 0>(act ? 1 : 0);
-                sv[1] = quad_bcast_f<S, 1>(s);
-                av[1] = quad_bcast<S, 1>(act ? 1 : 0);
+                sv[0] = s;
+                av[0] = act ? 1 : 0;
+                sv[1] = row_next_f<4>(s);
+                av[1] = row_next<4>(act ? 1 : 0);
                 if constexpr (S == 4) {
-                    sv[2] = quad_bcast_f<S, 2>(s);
-                    av[2] = quad_bcast<S, 2>(act ? 1 : 0);
-                    sv[3] = quad_bcast_f<S, 3>(s);
-                    av[3] = quad_bcast<S, 3>(act ? 1 : 0);
+                    sv[2] = row_next_f<8>(s);
+                    av[2] = row_next<8>(act ? 1 : 0);
+                    sv[3] = row_next_f<12>(s);
+                    av[3] = row_next<12>(act ? 1 : 0);
                 }
 #pragma unroll
                 for (int mm = 0; mm < S; ++mm) {
