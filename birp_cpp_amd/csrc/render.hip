@@ -1236,8 +1236,26 @@ __global__ __launch_bounds__(BIRP_BLOCK, 6) void k_render_fast(const DevVolume* 
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
     unsigned* tab = s_tab + wave * tab_stride;
-    const int ray = lane / S, sub = lane % S;
-    const int tw = 1 << tw_log2;
+    const int tw = 1 << tw_log2, th = (64 / S) >> tw_log2;
+    // Positive kernels put 2x2 pixel blocks on lane quads, the S lanes of a ray on S consecutive
+    // quads of a 16-lane row (as exact_lane: a quad's record loads then mostly share one cache
+    // line); signed kernels keep the S lanes of a ray adjacent, since their first-negative test
+    // reads the ray's S bits of one ballot as a group.
+    const bool blocks = !SIGNED && tw >= 2 && th >= 2;
+    const int ray = lane / S;
+    int ti, tj, sub;
+    if (blocks) {
+        const int r16 = lane & 15, quad = r16 >> 2;
+        sub = quad & (S - 1);
+        const int blk = (lane >> 4) * (4 / S) + quad / S;
+        ti = (blk & ((tw >> 1) - 1)) * 2 + (lane & 1);
+        tj = (blk >> (tw_log2 - 1)) * 2 + ((lane >> 1) & 1);
+    } else {
+        sub = lane % S;
+        ti = ray & (tw - 1);
+        tj = ray >> tw_log2;
+    }
+    const int sub_step = blocks ? 4 : 1;  // lane distance between sub-lanes m and m + 1 of a ray
     const float K = (float)BIRP_SCALE_FOLDED;
     int cached_vol = -1;
     int queue = blockIdx.x & 7;
@@ -1272,8 +1290,8 @@ __global__ __launch_bounds__(BIRP_BLOCK, 6) void k_render_fast(const DevVolume* 
         const __amdgpu_buffer_rsrc_t rs = (DT == BIRP_DTYPE_F32) ? make_rsrc(V.quad, V.quad_bytes) : make_rsrc(V.oct, V.oct_bytes);
         const int tile = item - prefix[pidx];
         const int tx = tile % P.tiles_x, ty = tile / P.tiles_x;
-        const int i = tx * tw + (ray & (tw - 1));
-        const int j = ty * ((64 / S) >> tw_log2) + (ray >> tw_log2);
+        const int i = tx * tw + ti;
+        const int j = ty * th + tj;
         const bool valid = (i < P.W) && (j < P.H);
 
         int kf = P.n, kl = -1;
@@ -1408,10 +1426,10 @@ __global__ __launch_bounds__(BIRP_BLOCK, 6) void k_render_fast(const DevVolume* 
         // and the scale constant are applied once per ray (all in FP32)
         float tot = sum + nc;
         if constexpr (S > 1) {
-            float t1 = __shfl_xor(tot, 1, 64);
+            float t1 = __shfl_xor(tot, sub_step, 64);
             tot = (sub & 1) ? t1 + tot : tot + t1;
             if constexpr (S == 4) {
-                float t2 = __shfl_xor(tot, 2, 64);
+                float t2 = __shfl_xor(tot, 2 * sub_step, 64);
                 tot = (sub & 2) ? t2 + tot : tot + t2;
             }
         }
