@@ -104,6 +104,22 @@ def valu_roofline(profile, kern_ms):
             "source": "profiles/" + profile}
 
 
+def l1_pipeline(kern_ms):
+    """The vector memory pipeline of the plain kernel from the committed rocprofv3 --pmc summary:
+    the TD (data return) busy cycles per launch over this run's kernel time at 2.4 GHz on 256 CUs
+    (`frac`, time-based like the VALU fraction), and the per-CU-cycle rates of the profiled run."""
+    prof = "plain_exact_256cube_576x336_l1_pmc.txt"
+    td = read_profile_value(prof, "td_busy_cycles_per_launch")
+    if td is None:
+        return None
+    return {"unit": "TD busy cycles per CU-cycle", "frac": round(td / (256 * kern_ms * 1e-3 * 2.4e9), 3),
+            "frac_cycles": read_profile_value(prof, "td_busy_per_cu_cycle"),
+            "l1_lookups_per_cu_cycle": read_profile_value(prof, "tcp_lookups_per_cu_cycle"),
+            "l1_lookups_per_record_load": read_profile_value(prof, "tcp_lookups_per_record_load"),
+            "ta_busy_per_cu_cycle": read_profile_value(prof, "ta_busy_per_cu_cycle"),
+            "source": "profiles/" + prof}
+
+
 def read_traffic():
     """HBM bytes per launch of the plain kernel from the committed rocprofv3 --pmc summary."""
     return read_profile_value("plain_exact_256cube_576x336_pmc.txt", "traffic_bytes_per_launch")
@@ -564,6 +580,7 @@ def main():
                    "source": "profiles/plain_exact_256cube_576x336_pmc.txt"}
     roof["gathers"] = {"algorithmic_bytes": int(alg_bytes), "achieved": round(alg_bytes / (kern_ms * 1e-3) / 1e9, 1),
                        "unit": "GB/s", "note": "8 FP32 corners per in-cube sample + 4 B per pixel, served mostly by L1/L2"}
+    roof["l1"] = l1_pipeline(kern_ms)
     line = {
         "metric": METRIC, "value": round(n_in * args.steps / p["wall"] / 1e9, 3), "unit": "Gsamples/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
