@@ -17,7 +17,7 @@ from birp_cpp_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 NT = min(16, os.cpu_count() or 1)
-TW, TH = 8, 4  # wave tile of the exact kernel at two lanes per ray (block: 2x2 waves)
+TW, TH = 4, 8  # wave tile of the exact kernel at two lanes per ray (block: 2x2 waves)
 
 
 def oview(O, v):
