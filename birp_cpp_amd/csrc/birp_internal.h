@@ -144,18 +144,21 @@ int exact_wave_slots(int W, int H, int lanes_per_ray);
 // [z][y][x] copy (any data, nearest sampling); SM_CMEM: the analytic model.
 enum SampleMode { SM_QUAD_POS = 0, SM_QUAD_GEN = 1, SM_LIN_TRI = 2, SM_LIN_NEAREST = 3, SM_CMEM = 4 };
 int exact_mode_of(const DevVolume& v, const ExactView& ev);
-// Lanes per ray of multi-view exact launches: 4 for CMEM, 2 for cubes (the measured best of each).
+// Lanes per ray of multi-view exact launches of many tiles: 4 for CMEM, 2 for cubes (the measured
+// best of each). Exact batches of few small images take 4 for cubes too (exact_batch_setup).
 inline int exact_multi_lanes(int mode) { return mode == SM_CMEM ? 4 : 2; }
 // Dynamic LDS bytes of an exact block: the t table, plus the brick offset tables in quad modes.
 size_t exact_lds_bytes(const DevVolume& v, int n, int mode);
 // 2x2-wave tiles of a W x H image at `lanes_per_ray`.
 int exact_tiles(int W, int H, int lanes_per_ray);
-// Views [0, nview) of one sample mode, tiles view-major from prefix (nview + 1 entries, device).
+// Views [0, nview) of one sample mode, tiles view-major from prefix (nview + 1 entries, device),
+// at `lanes` per ray (2 or 4; SM_CMEM always takes 4) with `pipe` samples in flight per lane in the
+// positive-cube mode (2 or 4); block b renders item order_dev[b] when order_dev is given.
 int launch_exact_multi(const DevVolume* vols_dev, const ExactView* views_dev, const int* prefix_dev, int nview, int total,
-                       int mode, size_t lds, void* stream, const int* order_dev = nullptr);
-// Per-item work estimates of such a list (not for SM_CMEM): cost_dev[g] for g < total.
+                       int mode, size_t lds, void* stream, const int* order_dev = nullptr, int lanes = 2, int pipe = 2);
+// Per-item work estimates of such a list (not for SM_CMEM) at `lanes` per ray: cost_dev[g] for g < total.
 int launch_plan_multi(const DevVolume* vols_dev, const ExactView* views_dev, const int* prefix_dev, int nview, int total,
-                      int mode, int* cost_dev, void* stream);
+                      int mode, int* cost_dev, void* stream, int lanes = 2);
 int launch_f64(const DevVolume& v, const F64View& fv, void* stream);
 int launch_sample_points(const DevVolume& v, const float* xyz, long long n, float* out, void* stream);
 // Any other Space: the sample points of rows [row0, row0 + rows) of a view (3 floats per (pixel, k),

@@ -1115,7 +1115,8 @@ int exact_batch_launch(birp_batch* b, hipStream_t st) {
     for (size_t g = 0; g < b->groups.size(); g++) {
         const birp_batch::ExactGroup& G = b->groups[g];
         const int* plan = G.plan_at >= 0 ? b->d_plan + G.plan_at : nullptr;
-        int rc = launch_exact_multi(b->d_vols, b->d_evs + G.first, b->d_prefix + base, G.count, G.total, G.mode, G.lds, st, plan);
+        int rc = launch_exact_multi(b->d_vols, b->d_evs + G.first, b->d_prefix + base, G.count, G.total, G.mode, G.lds, st, plan,
+                                    G.lanes, G.pipe);
         if (rc) return rc;
         base += G.count + 1;
     }
@@ -1123,12 +1124,16 @@ int exact_batch_launch(birp_batch* b, hipStream_t st) {
 }
 
 // The tile plans of an exact batch, made once when the batch is set up: each cube group's tiles
-// are estimated on the device (k_plan_multi), and each view's tiles are counting-sorted on the
-// host, heaviest first, in place of their natural order; the views keep their order, so the
-// launch still walks the batch view by view. The multi-view launch hands blocks to CUs as
-// earlier ones finish, so a view's light tiles fill the gaps its heavy tiles leave instead of
-// the heaviest tiles starting last. CMEM groups (every ray takes all n samples), groups of fewer
-// tiles than CUs, and BIRP_EXACT_ORDER=natural keep the natural order.
+// are estimated on the device (k_plan_multi) and counting-sorted on the host, heaviest first.
+// A group of more tiles than the GPU holds at once sorts each view's tiles in place of their
+// natural order and keeps the views' order, so the launch still walks the batch view by view: the
+// multi-view launch hands blocks to CUs as earlier ones finish, and a view's light tiles fill the
+// gaps its heavy tiles leave. A group whose blocks are all resident from the start (`resident`)
+// gets no such dynamic balance, so its tiles are sorted together and dealt in a snake over the
+// CUs, as a single render's plan is (k_render_exact): slot b takes sorted tile
+// base + c in even rounds r of ncu slots and base + nr - 1 - c in odd ones. CMEM groups (every ray
+// takes all n samples), groups of fewer tiles than CUs, and BIRP_EXACT_ORDER=natural keep the
+// natural order.
 static int exact_batch_plan(birp_batch* b, const std::vector<int>& prefix) {
     birp_device* dev = b->vol->dev;
     if (exact_order_policy() == 0 || dev->ncu <= 0) return BIRP_OK;
@@ -1148,13 +1153,31 @@ static int exact_batch_plan(birp_batch* b, const std::vector<int>& prefix) {
         const int pbase = base;
         base += G.count + 1;
         if (G.mode == SM_CMEM || G.total < dev->ncu) continue;
-        rc = launch_plan_multi(b->d_vols, b->d_evs + G.first, b->d_prefix + pbase, G.count, G.total, G.mode, d_cost + at, dev->stream);
+        rc = launch_plan_multi(b->d_vols, b->d_evs + G.first, b->d_prefix + pbase, G.count, G.total, G.mode, d_cost + at, dev->stream,
+                               G.lanes);
         if (rc) break;
         e = hipMemcpyAsync(cost.data() + at, d_cost + at, (size_t)G.total * sizeof(int), hipMemcpyDeviceToHost, dev->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
         if (e != hipSuccess) { rc = hip_fail(e, "plan estimates"); break; }
         int* c = cost.data() + at;
         for (int g = 0; g < G.total; g++) c[g] = std::min(std::max(c[g], 0), 65535);  // estimates are at most 4n
+        if (G.resident) {
+            std::vector<int> sorted((size_t)G.total);
+            int top = 0;
+            for (int g = 0; g < G.total; g++) top = std::max(top, c[g]);
+            bucket.assign((size_t)top + 2, 0);
+            for (int g = 0; g < G.total; g++) bucket[top - c[g] + 1]++;
+            for (size_t k = 1; k < bucket.size(); k++) bucket[k] += bucket[k - 1];
+            for (int g = 0; g < G.total; g++) sorted[bucket[top - c[g]]++] = g;
+            const int ncu = dev->ncu;
+            for (int slot = 0; slot < G.total; slot++) {
+                const int r = slot / ncu, base = r * ncu, col = slot - base, nr = std::min(ncu, G.total - base);
+                plan[at + slot] = sorted[base + ((r & 1) ? nr - 1 - col : col)];
+            }
+            G.plan_at = at;
+            at += G.total;
+            continue;
+        }
         for (int q = 0; q < G.count; q++) {  // view q of the group: items prefix[pbase + q] .. prefix[pbase + q + 1]
             const int i0 = prefix[pbase + q], i1 = prefix[pbase + q + 1];
             int top = 0;
@@ -1205,10 +1228,30 @@ int exact_batch_setup(birp_batch* b) {
         birp_batch::ExactGroup G;
         G.mode = m;
         G.first = (int)b->order.size();
+        G.lanes = exact_multi_lanes(m);
+        // A cube group small enough that the GPU holds all its blocks at once (7 blocks of 4 waves
+        // per CU, the occupancy of the 4-samples-in-flight kernel) renders like single views:
+        // 4 lanes per ray when every image has at most 60,000 rays, 4 samples in flight, and one
+        // snake deal over the CUs (exact_batch_plan). Larger groups keep 2 and 2 (the orbit batch
+        // measured 2% slower with 4 in flight and 10% slower with 4 lanes).
+        if (m != SM_CMEM && dev->ncu > 0) {
+            const long long cap = 7LL * dev->ncu;
+            long long t2 = 0, t4 = 0;
+            bool small = true;
+            for (int k = 0; k < n; k++) {
+                if (mode[k] != m) continue;
+                t2 += exact_tiles(b->views[k].width, b->views[k].height, 2);
+                t4 += exact_tiles(b->views[k].width, b->views[k].height, 4);
+                small = small && (long long)b->views[k].width * b->views[k].height <= 60000;
+            }
+            if (small && t4 <= cap) G.lanes = 4;
+            G.resident = (G.lanes == 4 ? t4 : t2) <= cap;
+            if (G.resident && m == SM_QUAD_POS) G.pipe = 4;
+        }
         prefix.push_back(0);
         for (int k = 0; k < n; k++) {
             if (mode[k] != m) continue;
-            const int lanes = exact_multi_lanes(m);
+            const int lanes = G.lanes;
             const int tiles = exact_tiles(b->views[k].width, b->views[k].height, lanes);
             const long long next = (long long)prefix.back() + tiles;
             if (next > INT32_MAX) return fail(BIRP_ERR_UNSUPPORTED, "exact batch of more than 2^31 - 1 tiles: split it");

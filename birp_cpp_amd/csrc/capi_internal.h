@@ -110,6 +110,8 @@ struct birp_batch {
     std::vector<int> vol_of;
     struct ExactGroup {
         int mode = 0, first = 0, count = 0, total = 0;  // views order[first, first + count)
+        int lanes = 2, pipe = 2;  // lanes per ray and samples in flight per lane of its launch
+        bool resident = false;    // every block resident at once: one snake deal over the CUs
         size_t lds = 0;
         long long plan_at = -1;  // the group's tile plan in d_plan (total entries), or -1: none
     };

@@ -1574,29 +1574,44 @@ int exact_tiles(int W, int H, int lanes_per_ray) {
 
 int exact_wave_slots(int W, int H, int lanes_per_ray) { return 4 * exact_tiles(W, H, lanes_per_ray); }
 
-int launch_exact_multi(const DevVolume* vols, const ExactView* views, const int* prefix, int nview, int total, int mode,
-                       size_t lds, void* stream, const int* order) {
-    if (nview <= 0 || total <= 0) return BIRP_OK;
-    if (lds > 64 * 1024) return BIRP_ERR_UNSUPPORTED;
-    hipStream_t st = (hipStream_t)stream;
+template <int S>
+static int launch_exact_multi_s(const DevVolume* vols, const ExactView* views, const int* prefix, int nview, int total, int mode,
+                                size_t lds, hipStream_t st, const int* order, int pipe) {
     const int grid = total;
     switch (mode) {
-        case SM_QUAD_POS: hipLaunchKernelGGL((k_render_exact_multi<2, SM_QUAD_POS, 2>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total, order); break;
-        case SM_QUAD_GEN: hipLaunchKernelGGL((k_render_exact_multi<2, SM_QUAD_GEN>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total, order); break;
-        case SM_LIN_TRI: hipLaunchKernelGGL((k_render_exact_multi<2, SM_LIN_TRI>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total, order); break;
-        case SM_LIN_NEAREST: hipLaunchKernelGGL((k_render_exact_multi<2, SM_LIN_NEAREST>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total, order); break;
-        case SM_CMEM: hipLaunchKernelGGL((k_render_exact_multi<4, SM_CMEM>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total, order); break;
+        case SM_QUAD_POS:
+            if (pipe >= 4) hipLaunchKernelGGL((k_render_exact_multi<S, SM_QUAD_POS, 4>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total, order);
+            else hipLaunchKernelGGL((k_render_exact_multi<S, SM_QUAD_POS, 2>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total, order);
+            break;
+        case SM_QUAD_GEN: hipLaunchKernelGGL((k_render_exact_multi<S, SM_QUAD_GEN>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total, order); break;
+        case SM_LIN_TRI: hipLaunchKernelGGL((k_render_exact_multi<S, SM_LIN_TRI>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total, order); break;
+        case SM_LIN_NEAREST: hipLaunchKernelGGL((k_render_exact_multi<S, SM_LIN_NEAREST>), dim3(grid), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total, order); break;
         default: return BIRP_ERR_INVALID_ARGUMENT;
     }
     return hip_status(hipGetLastError());
 }
 
+int launch_exact_multi(const DevVolume* vols, const ExactView* views, const int* prefix, int nview, int total, int mode,
+                       size_t lds, void* stream, const int* order, int lanes, int pipe) {
+    if (nview <= 0 || total <= 0) return BIRP_OK;
+    if (lds > 64 * 1024) return BIRP_ERR_UNSUPPORTED;
+    hipStream_t st = (hipStream_t)stream;
+    if (mode == SM_CMEM) {  // always 4 lanes per ray (exact_multi_lanes)
+        hipLaunchKernelGGL((k_render_exact_multi<4, SM_CMEM>), dim3(total), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total, order);
+        return hip_status(hipGetLastError());
+    }
+    if (lanes == 4) return launch_exact_multi_s<4>(vols, views, prefix, nview, total, mode, lds, st, order, pipe);
+    if (lanes == 2) return launch_exact_multi_s<2>(vols, views, prefix, nview, total, mode, lds, st, order, pipe);
+    return BIRP_ERR_INVALID_ARGUMENT;
+}
+
 int launch_plan_multi(const DevVolume* vols, const ExactView* views, const int* prefix, int nview, int total, int mode, int* cost,
-                      void* stream) {
-    if (nview <= 0 || total <= 0 || mode == SM_CMEM) return BIRP_ERR_INVALID_ARGUMENT;
+                      void* stream, int lanes) {
+    if (nview <= 0 || total <= 0 || mode == SM_CMEM || (lanes != 2 && lanes != 4)) return BIRP_ERR_INVALID_ARGUMENT;
     const long long threads = 4LL * total;
-    hipLaunchKernelGGL((k_plan_multi<2>), dim3((unsigned)((threads + BIRP_BLOCK - 1) / BIRP_BLOCK)), dim3(BIRP_BLOCK), 0,
-                       (hipStream_t)stream, vols, views, prefix, nview, total, cost);
+    const dim3 grid((unsigned)((threads + BIRP_BLOCK - 1) / BIRP_BLOCK));
+    if (lanes == 4) hipLaunchKernelGGL((k_plan_multi<4>), grid, dim3(BIRP_BLOCK), 0, (hipStream_t)stream, vols, views, prefix, nview, total, cost);
+    else hipLaunchKernelGGL((k_plan_multi<2>), grid, dim3(BIRP_BLOCK), 0, (hipStream_t)stream, vols, views, prefix, nview, total, cost);
     return hip_status(hipGetLastError());
 }
 

@@ -138,6 +138,36 @@ def test_exact_batch_of_the_256_orbit_views(device, oracle):
         vol.free()
 
 
+def test_resident_exact_batches_bitwise(spaces, oracle):
+    """Batches whose blocks the GPU holds all at once take the single renders' launch shape: 4 lanes
+    per ray when every image has at most 60,000 rays, 4 samples in flight, and one plan dealt in a
+    snake over the CUs. Six 144x84 views (4 lanes, more tiles than CUs, so planned), one 576x336
+    view (2 lanes, planned) and a mix of positive and signed cubes: every image bitwise equal to
+    its single render, counts too, and two images to the oracle's FP32 render."""
+    vols, ocs = spaces
+    pos = [tuple(float(c) for c in p) for p in capi.synth_orbit(6, 23)]
+    small = [W.view(144, 84, 0.1, pos=p) for p in pos]
+    head = [W.view_576x336()]
+    mixed = [W.view(144, 84, 0.25, pos=p) for p in pos[:4]]
+    for views, which in ((small, [0] * 6), (head, [0]), (mixed, [0, 1, 0, 1])):
+        b = capi.Batch.exact(vols, views, which)
+        try:
+            for rep in range(2):
+                b.launch()
+                b.wait()
+            counts = b.samples()
+            for k, v in enumerate(views):
+                single, st = vols[which[k]].render(v)
+                assert np.array_equal(bits(b.image(k)), bits(single)), (v.width, k)
+                assert counts[k] == st.samples_in_cube, (v.width, k)
+            assert sum(counts) > 0
+            v = views[-1]
+            ref, cnt = oracle.render_f32(ocs[which[-1]], oview(oracle, v), v.semantics, nthreads=NT)
+            assert np.array_equal(bits(b.image(len(views) - 1)), bits(ref)) and counts[-1] == cnt
+        finally:
+            b.free()
+
+
 def test_exact_batch_argument_checks(spaces):
     vols, _ = spaces
     v = capi.make_view(16, 16, 30.0, 30.0, (1.0, 2.0, 3.0), (10.0, 0.0, 0.0), 40.0, 8193, 1, capi.MATH_EXACT)
