@@ -308,11 +308,30 @@ int launch_slot(birp_server* S, Slot& s) {
     // sums h_eprefix[estart[m] + m, + ecount[m] + 1)
     int ecount[kModes] = {}, estart[kModes] = {}, efill[kModes] = {};
     size_t elds[kModes] = {};
+    long long et2[kModes] = {}, et4[kModes] = {};
+    bool esmall[kModes];
+    for (int m = 0; m < kModes; m++) esmall[m] = true;
     for (Request* r : s.reqs) {
         if (!exact_request(r)) continue;
         ExactView ev;
         if ((rc = exact_view(r->vol, &r->req.view, nullptr, ev))) return rc;
-        ecount[exact_mode_of(r->vol->dv, ev)]++;
+        const int m = exact_mode_of(r->vol->dv, ev);
+        ecount[m]++;
+        et2[m] += exact_tiles(r->req.view.width, r->req.view.height, 2);
+        et4[m] += exact_tiles(r->req.view.width, r->req.view.height, 4);
+        esmall[m] = esmall[m] && (long long)r->req.view.width * r->req.view.height <= 60000;
+    }
+    // each mode's launch shape, as exact batches choose it: a cube launch whose blocks the GPU holds
+    // all at once takes 4 samples in flight, and 4 lanes per ray when every image has at most
+    // 60,000 rays (a lone request of a small image renders as fast as its single render)
+    int elanes[kModes], epipe[kModes];
+    for (int m = 0; m < kModes; m++) {
+        elanes[m] = exact_multi_lanes(m);
+        epipe[m] = 2;
+        if (m == SM_CMEM || S->dev->ncu <= 0) continue;
+        const long long cap = 7LL * S->dev->ncu;
+        if (esmall[m] && et4[m] <= cap) elanes[m] = 4;
+        if (m == SM_QUAD_POS && (elanes[m] == 4 ? et4[m] : et2[m]) <= cap) epipe[m] = 4;
     }
     int nexact = 0;
     for (int m = 0; m < kModes; m++) {
@@ -335,7 +354,7 @@ int launch_slot(birp_server* S, Slot& s) {
             s.h_evs[q] = ev;
             elds[m] = std::max(elds[m], exact_lds_bytes(r->vol->dv, r->req.view.samples, m));
             int* pre = s.h_eprefix + estart[m] + m;
-            const long long next = (long long)pre[q - estart[m]] + exact_tiles(r->req.view.width, r->req.view.height, exact_multi_lanes(m));
+            const long long next = (long long)pre[q - estart[m]] + exact_tiles(r->req.view.width, r->req.view.height, elanes[m]);
             if (next > INT32_MAX) return fail(BIRP_ERR_UNSUPPORTED, "launch of more than 2^31 - 1 tiles");
             pre[q - estart[m] + 1] = (int)next;
             continue;
@@ -370,7 +389,7 @@ int launch_slot(birp_server* S, Slot& s) {
         if (!ecount[m]) continue;
         const int first = estart[m] + m;
         rc = launch_exact_multi(S->d_vols, s.d_evs + estart[m], s.d_eprefix + first, ecount[m], s.h_eprefix[first + ecount[m]], m,
-                                elds[m], s.stream);
+                                elds[m], s.stream, nullptr, elanes[m], epipe[m]);
         if (rc) return fail(rc, "exact launch failed");
     }
     for (int g = 0; g < kGroups; g++) {
