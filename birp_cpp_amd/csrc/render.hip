@@ -398,9 +398,9 @@ template <int S> struct ExactTile {
 // quad and its S sub-lanes on S consecutive quads. A 16-byte-per-lane record load is served a
 // quad (64 B) per cycle, one cycle per distinct cache line in the quad: the four rays of a 2x2
 // block sample the same k at points less than a voxel apart, mostly inside one line, where a
-// quad of two rays' consecutive samples spans two (22.5 line lookups per 64-lane load against
-// 32.5 for the 256^3 cube's 576x336 x 800 view, by the access model that reproduces the
-// measured 32.2). Sub-lane m of a ray sits 4m lanes above sub-lane 0 (row_next<4m>).
+// quad of two rays' consecutive samples spans about two (on the 256^3 cube's 576x336 x 800 view,
+// 22.3 line lookups per 64-lane load against 32.2). Sub-lane m of a ray sits 4m lanes above
+// sub-lane 0 (row_next<4m>).
 template <int S>
 __device__ __forceinline__ void exact_lane(int lane, int& ti, int& tj, int& sub) {
     constexpr int TW = ExactTile<S>::TW;
