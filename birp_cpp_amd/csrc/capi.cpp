@@ -203,8 +203,9 @@ int default_exact_lanes(bool cmem, int W, int H) {
 }
 // Single positive-cube renders keep 4 samples in flight per lane: their waves are few and wait
 // on each sample's loads (the 256 orbit views rendered one by one: 8.04 ms against 8.85 with 2;
-// the 576x336 x 800 view 0.0914 against 0.0932 ms). Exact batches keep 2 (launch_exact_multi):
-// their many waves hide the loads already, and 4 made the orbit batch 2% slower.
+// the 576x336 x 800 view 0.0900 against 0.0916 ms). Exact batches of more blocks than the GPU
+// holds at once keep 2 (exact_batch_setup): their many waves hide the loads already, and 4 made
+// the orbit batch 2% slower.
 int default_exact_pipe() {
     static int pipe = [] {
         const char* e = getenv("BIRP_EXACT_PIPE");
@@ -1130,8 +1131,8 @@ int exact_batch_launch(birp_batch* b, hipStream_t st) {
 // multi-view launch hands blocks to CUs as earlier ones finish, and a view's light tiles fill the
 // gaps its heavy tiles leave. A group whose blocks are all resident from the start (`resident`)
 // gets no such dynamic balance, so its tiles are sorted together and dealt in a snake over the
-// CUs, as a single render's plan is (k_render_exact): slot b takes sorted tile
-// base + c in even rounds r of ncu slots and base + nr - 1 - c in odd ones. CMEM groups (every ray
+// CUs, as a single render's plan is (k_render_exact): slot b = r ncu + c takes sorted tile
+// r ncu + c in even rounds r and r ncu + nr - 1 - c in odd ones (nr slots in the round). CMEM groups (every ray
 // takes all n samples), groups of fewer tiles than CUs, and BIRP_EXACT_ORDER=natural keep the
 // natural order.
 static int exact_batch_plan(birp_batch* b, const std::vector<int>& prefix) {
@@ -1171,8 +1172,8 @@ static int exact_batch_plan(birp_batch* b, const std::vector<int>& prefix) {
             for (int g = 0; g < G.total; g++) sorted[bucket[top - c[g]]++] = g;
             const int ncu = dev->ncu;
             for (int slot = 0; slot < G.total; slot++) {
-                const int r = slot / ncu, base = r * ncu, col = slot - base, nr = std::min(ncu, G.total - base);
-                plan[at + slot] = sorted[base + ((r & 1) ? nr - 1 - col : col)];
+                const int r = slot / ncu, r0 = r * ncu, col = slot - r0, nr = std::min(ncu, G.total - r0);
+                plan[at + slot] = sorted[r0 + ((r & 1) ? nr - 1 - col : col)];
             }
             G.plan_at = at;
             at += G.total;
