@@ -184,22 +184,24 @@ bool take_injected_failure(birp_device* dev) {
     return false;
 }
 
-// Tuning knobs of the exact path, read once: BIRP_EXACT_LANES (lanes per ray: 1, 2, 4),
+// Tuning knobs of the exact path, read once: BIRP_EXACT_LANES (lanes per ray: 1, 2, 4, 8),
 // BIRP_EXACT_PIPE (samples in flight per lane in the positive-cube loop: 1..4) and
 // BIRP_EXACT_LAYOUT ("linear" reads the [z][y][x] copy instead of the quad bricks).
-// By default a single exact render takes 4 lanes per ray for CMEM and for images of at most
-// 60,000 rays, whose few waves are bound by the chain of each ray's samples (ms per render over 6
-// orbit poses, 1 / 2 / 4 lanes: 144x84 at n = 800 0.080 / 0.045 / 0.026, 288x168 at n = 800
-// 0.079 / 0.047 / 0.032), and 2 for larger images (576x336 at n = 800: 0.088 / 0.077 / 0.078; the
-// headline pose 0.105 with 2, 0.148 with 4).
+// By default a single exact render takes 8 lanes per ray for images of at most 20,000 rays, 4 for
+// CMEM and for images of at most 60,000 rays, and 2 above: a small image's few waves are bound by
+// the chain of each ray's samples, which more lanes cut (ms per render over 6 orbit poses at
+// n = 800 with 4 samples in flight, 2 / 4 / 8 lanes: 144x84 0.032 / 0.020 / 0.016, 288x168 0.038 /
+// 0.027 / 0.030, 576x336 0.071 / 0.075 / 0.089).
 int default_exact_lanes(bool cmem, int W, int H) {
     static int lanes = [] {
         const char* e = getenv("BIRP_EXACT_LANES");
         int v = e ? atoi(e) : 0;
-        return (v == 1 || v == 2 || v == 4) ? v : 0;
+        return (v == 1 || v == 2 || v == 4 || v == 8) ? v : 0;
     }();
     if (lanes) return lanes;
-    return (cmem || (long long)W * H <= 60000) ? 4 : 2;
+    if (cmem) return 4;
+    const long long rays = (long long)W * H;
+    return rays <= 20000 ? 8 : (rays <= 60000 ? 4 : 2);
 }
 // Single positive-cube renders keep 4 samples in flight per lane: their waves are few and wait
 // on each sample's loads (the 256 orbit views rendered one by one: 8.04 ms against 8.85 with 2;

@@ -245,14 +245,22 @@ __device__ __forceinline__ float scale_pos(float vw) {
 }
 
 // acc += the S sub-lane samples of this ray, in sub-lane (= k) order. Exact on sub-lane 0 of
-// the ray (the lane that stores the pixel); the other sub-lanes' sums are not used.
+// the ray (the lane that stores the pixel); the other sub-lanes' sums are not used. With 8 lanes
+// per ray the ray spans two 16-lane rows: sub-lanes 4..7 are read from the next row's copy.
 template <int S>
 __device__ __forceinline__ float add_in_order(float acc, float s) {
     acc = acc + s;
     if constexpr (S >= 2) acc = acc + row_next_f<4>(s);
-    if constexpr (S == 4) {
+    if constexpr (S >= 4) {
         acc = acc + row_next_f<8>(s);
         acc = acc + row_next_f<12>(s);
+    }
+    if constexpr (S == 8) {
+        const float s16 = __shfl_down(s, 16, 64);  // lane l + 16: the next row
+        acc = acc + s16;
+        acc = acc + row_next_f<4>(s16);
+        acc = acc + row_next_f<8>(s16);
+        acc = acc + row_next_f<12>(s16);
     }
     return acc;
 }
@@ -394,8 +402,8 @@ template <int S> struct ExactTile {
 };
 
 // The lanes of a wave and the rays they trace. The wave's TW x TH rays are cut into 2x2 pixel
-// blocks; a block takes 4S lanes of one 16-lane row, its four pixels on the four lanes of a
-// quad and its S sub-lanes on S consecutive quads. A 16-byte-per-lane record load is served a
+// blocks; a block takes 4S consecutive lanes (one 16-lane row for S <= 4, two for S = 8), its
+// four pixels on the four lanes of a quad and its S sub-lanes on S consecutive quads. A 16-byte-per-lane record load is served a
 // quad (64 B) per cycle, one cycle per distinct cache line in the quad: the four rays of a 2x2
 // block sample the same k at points less than a voxel apart, mostly inside one line, where a
 // quad of two rays' consecutive samples spans about two (on the 256^3 cube's 576x336 x 800 view,
@@ -404,9 +412,8 @@ template <int S> struct ExactTile {
 template <int S>
 __device__ __forceinline__ void exact_lane(int lane, int& ti, int& tj, int& sub) {
     constexpr int TW = ExactTile<S>::TW;
-    const int r16 = lane & 15, quad = r16 >> 2;
-    sub = quad & (S - 1);
-    const int blk = (lane >> 4) * (4 / S) + quad / S;  // 2x2 block of the wave, row-major
+    sub = (lane >> 2) & (S - 1);
+    const int blk = lane / (4 * S);  // 2x2 block of the wave, row-major
     ti = (blk % (TW / 2)) * 2 + (lane & 1);
     tj = (blk / (TW / 2)) * 2 + ((lane >> 1) & 1);
 }
@@ -535,15 +542,28 @@ __device__ __forceinline__ void exact_block(const DevVolume& v, const ExactView&
                 // sums and stops are not used)
                 float sv[S];
                 int av[S];
+                const int ai = act ? 1 : 0;
                 sv[0] = s;
-                av[0] = act ? 1 : 0;
+                av[0] = ai;
                 sv[1] = row_next_f<4>(s);
-                av[1] = row_next<4>(act ? 1 : 0);
-                if constexpr (S == 4) {
+                av[1] = row_next<4>(ai);
+                if constexpr (S >= 4) {
                     sv[2] = row_next_f<8>(s);
-                    av[2] = row_next<8>(act ? 1 : 0);
+                    av[2] = row_next<8>(ai);
                     sv[3] = row_next_f<12>(s);
-                    av[3] = row_next<12>(act ? 1 : 0);
+                    av[3] = row_next<12>(ai);
+                }
+                if constexpr (S == 8) {  // sub-lanes 4..7: the next 16-lane row
+                    const float s16 = __shfl_down(s, 16, 64);
+                    const int a16 = __shfl_down(ai, 16, 64);
+                    sv[4] = s16;
+                    av[4] = a16;
+                    sv[5] = row_next_f<4>(s16);
+                    av[5] = row_next<4>(a16);
+                    sv[6] = row_next_f<8>(s16);
+                    av[6] = row_next<8>(a16);
+                    sv[7] = row_next_f<12>(s16);
+                    av[7] = row_next<12>(a16);
                 }
 #pragma unroll
                 for (int mm = 0; mm < S; ++mm) {
@@ -1569,6 +1589,7 @@ static int exact_blocks(int W, int H) {
 }
 
 int exact_tiles(int W, int H, int lanes_per_ray) {
+    if (lanes_per_ray == 8) return exact_blocks<8>(W, H);
     return lanes_per_ray == 4 ? exact_blocks<4>(W, H) : (lanes_per_ray == 1 ? exact_blocks<1>(W, H) : exact_blocks<2>(W, H));
 }
 
@@ -1618,6 +1639,7 @@ int launch_plan_multi(const DevVolume* vols, const ExactView* views, const int* 
 int launch_exact(const DevVolume& v, const ExactView& ev, int lanes_per_ray, int pipe, void* stream, const ExactOrder* ord) {
     if (ev.n < 2 || ev.n > BIRP_MAX_EXACT_SAMPLES || ev.W <= 0 || ev.H <= 0) return BIRP_ERR_INVALID_ARGUMENT;
     hipStream_t st = (hipStream_t)stream;
+    if (lanes_per_ray == 8) return launch_exact_s<8>(v, ev, pipe, st, ord);
     if (lanes_per_ray == 4) return launch_exact_s<4>(v, ev, pipe, st, ord);
     if (lanes_per_ray == 2) return launch_exact_s<2>(v, ev, pipe, st, ord);
     return launch_exact_s<1>(v, ev, pipe, st, ord);
@@ -1637,6 +1659,7 @@ static int launch_plan_s(const DevVolume& v, const ExactView& ev, int* order, hi
 int launch_exact_plan(const DevVolume& v, const ExactView& ev, int lanes_per_ray, int* order, void* stream) {
     if (ev.n < 2 || ev.W <= 0 || ev.H <= 0 || !order) return BIRP_ERR_INVALID_ARGUMENT;
     hipStream_t st = (hipStream_t)stream;
+    if (lanes_per_ray == 8) return launch_plan_s<8>(v, ev, order, st);
     if (lanes_per_ray == 4) return launch_plan_s<4>(v, ev, order, st);
     if (lanes_per_ray == 2) return launch_plan_s<2>(v, ev, order, st);
     return launch_plan_s<1>(v, ev, order, st);
