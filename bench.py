@@ -7,7 +7,7 @@ through the C ABI with the cube resident in HBM, between two device synchronisat
 Also measured in the same run (keys under "forms"):
   plain_renders_per_s   the drop-in Camera::Render contract: launch, wait, copy the image to host
   batched               256 orbit poses, mixed 144x84/288x168/576x336 images, steps 0.1/0.25 RE,
-                        F16 storage, one persistent-kernel launch replayed from a HIP graph; plain
+                        F16 storage, one persistent-kernel launch (birp_batch_launch); plain
                         renders/s of the same 256 views for the ratio
   exact_batched         the same 256 views, and 8 copies of the headline view, as exact batches
                         (bit-identical images, one launch per sample mode) against back-to-back

@@ -118,6 +118,7 @@ SIGNATURES = [
     ("birp_batch_set_tuning", _i, [_P, C.POINTER(Tuning)]),
     ("birp_batch_launch", _i, [_P]),
     ("birp_batch_launch_eager", _i, [_P]),
+    ("birp_batch_launch_graph", _i, [_P]),
     ("birp_batch_wait", _i, [_P]),
     ("birp_batch_download", _i, [_P, _i, _FP]),
     ("birp_batch_stats", _i, [_P, C.POINTER(C.c_longlong), _DP]),
@@ -361,6 +362,9 @@ class Batch:
 
     def launch_eager(self):
         check(lib().birp_batch_launch_eager(self.h), "birp_batch_launch_eager")
+
+    def launch_graph(self):
+        check(lib().birp_batch_launch_graph(self.h), "birp_batch_launch_graph")
 
     def wait(self):
         check(lib().birp_batch_wait(self.h), "birp_batch_wait")

@@ -215,7 +215,7 @@ int birp_view_basis(const birp_view* view, float* angle_sun, float rot[9], float
 
 /* ------------------------------------------------------------------ batched renders */
 /* Many poses (and image sizes, sample counts) over one volume in one persistent-kernel launch,
- * captured in a HIP graph and replayable. Images are written into one device buffer; image i
+ * also captured in a HIP graph that can be replayed. Images are written into one device buffer; image i
  * starts at offset[i] floats. No reference counterpart: the reference renders one pose per
  * Camera (Camera.cpp:63) and batches cubes serially (main.cpp:242-264). */
 typedef struct birp_tuning {
@@ -228,11 +228,14 @@ int birp_batch_create(birp_volume* vol, const birp_view* views, int count, birp_
 int birp_batch_free(birp_batch* b);
 /* Selects kernel tuning for the batch: an explicit choice, or NULL for the autotuner's. */
 int birp_batch_set_tuning(birp_batch* b, const birp_tuning* tuning);
-/* Launch (graph replay) without waiting; birp_batch_wait waits for completion. */
+/* Launch without waiting (birp_batch_wait waits for completion): the batch's kernels go straight
+ * onto the device stream, which measured faster per launch than replaying the captured graph. */
 int birp_batch_launch(birp_batch* b);
-/* The same launch without the graph: the two kernels go straight onto the device stream (for
- * callers that cannot replay graphs, and to check graph replays against it). */
+/* The same launch; kept as the name of the direct launch beside the graph replay below. */
 int birp_batch_launch_eager(birp_batch* b);
+/* The same kernels as one replay of the HIP graph captured when the batch was made (identical
+ * images; for callers that want a graph launch). */
+int birp_batch_launch_graph(birp_batch* b);
 int birp_batch_wait(birp_batch* b);
 /* Copy image i (views[i].height * views[i].width floats) to host memory. */
 int birp_batch_download(birp_batch* b, int index, float* host);

@@ -77,7 +77,7 @@ def test_exact_batch_mixed_volumes_bitwise(spaces, oracle):
         # graph replays, with the program's own eager work in between, equal the eager launch
         for rep in range(3):
             _ = vols[0].render(W.view_144x84())
-            b.launch()
+            b.launch_graph()
             b.wait()
             for k in range(len(views)):
                 assert np.array_equal(bits(b.image(k)), bits(eager[k])), (rep, k)

@@ -234,17 +234,17 @@ def test_batch_matches_fp64_and_replays(cube1):
     views = W.orbit_views_mixed(count=24)
     b = capi.Batch(vols[capi.DTYPE_F16], views)
     try:
-        b.launch_eager()  # the same kernels without the graph
+        b.launch()  # the kernels straight onto the stream
         b.wait()
         eager = [b.image(k) for k in range(len(views))]
         for k, v in enumerate(views):
             ref = vols[capi.DTYPE_F32].render_f64(v)
             assert rel_l2(eager[k], ref) <= 1e-4, k
         # every graph replay, with the program's own eager work and syncs in between, equals
-        # the eager launch bit for bit
+        # the direct launch bit for bit
         for rep in range(4):
             _ = vols[capi.DTYPE_F32].render(W.view_144x84())
-            b.launch()
+            b.launch_graph()
             b.wait()
             for k in range(len(views)):
                 assert np.array_equal(b.image(k), eager[k]), (rep, k)
