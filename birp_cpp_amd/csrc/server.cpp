@@ -178,17 +178,24 @@ private:
     bool quit_ = false;
 };
 
+// A launch's descriptors live in one block, pinned on the host and mirrored on the device, and go
+// to the device in one copy (desc_layout): the work-queue counters first, kept zero on the host so
+// that the copy resets them, then the tile prefix sums, the fast poses and the exact views, each
+// region sized for the launch's requests.
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    FastPose* h_poses = nullptr;  // pinned, max_batch poses packed by kernel group
+    unsigned char* h_desc = nullptr;  // pinned descriptor block, sized for max_batch requests
+    unsigned char* d_desc = nullptr;  // its device copy
+    unsigned* d_work = nullptr;       // work-queue counters, 256 per kernel group, at d_desc
+    // this launch's regions in h_desc / d_desc
+    FastPose* h_poses = nullptr;  // poses packed by kernel group
     FastPose* d_poses = nullptr;
-    int* h_prefix = nullptr;      // pinned, max_batch + kGroups tile prefix sums
+    int* h_prefix = nullptr;      // each kernel group's tile prefix sums
     int* d_prefix = nullptr;
-    unsigned* d_work = nullptr;   // work-queue counters, 256 per kernel group
-    ExactView* h_evs = nullptr;   // pinned: the exact requests' view constants, grouped by sample mode
+    ExactView* h_evs = nullptr;   // the exact requests' view constants, grouped by sample mode
     ExactView* d_evs = nullptr;
-    int* h_eprefix = nullptr;     // pinned: each mode group's tile prefix sums (max_batch + 8)
+    int* h_eprefix = nullptr;     // each mode group's tile prefix sums
     int* d_eprefix = nullptr;
     float* d_images = nullptr;
     float* h_images = nullptr;    // pinned
@@ -279,6 +286,24 @@ int request_group(const Request* r, birp_tuning* t) {
 // multi-view exact kernel, one launch per sample mode present; the others take the fast kernels.
 bool exact_request(const Request* r) { return r->req.view.math == BIRP_MATH_EXACT; }
 constexpr int kModes = SM_CMEM + 1;
+constexpr size_t kCounterBytes = sizeof(unsigned) * kGroups * 256;
+
+// Offsets of a launch's descriptor regions for n fast and nexact exact requests, 256-byte aligned,
+// and the span that one copy uploads: the counters only when fast requests use them.
+struct DescLayout {
+    size_t prefix, eprefix, poses, evs, lo, hi;
+};
+DescLayout desc_layout(int n, int nexact) {
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    DescLayout L;
+    L.prefix = kCounterBytes;
+    L.eprefix = L.prefix + up(sizeof(int) * (n + kGroups));
+    L.poses = L.eprefix + up(sizeof(int) * (nexact + kModes));
+    L.evs = L.poses + up(sizeof(FastPose) * n);
+    L.hi = L.evs + up(sizeof(ExactView) * nexact);
+    L.lo = n ? 0 : L.prefix;
+    return L;
+}
 
 // Launch the admitted requests of slot s (dispatcher thread, lock not held). Group g's poses are
 // h_poses[base[g], base[g+1]) and its tile prefix sums h_prefix[base[g] + g, ... + count_g].
@@ -334,9 +359,20 @@ int launch_slot(birp_server* S, Slot& s) {
         if (m == SM_QUAD_POS && (elanes[m] == 4 ? et4[m] : et2[m]) <= cap) epipe[m] = 4;
     }
     int nexact = 0;
-    for (int m = 0; m < kModes; m++) {
-        estart[m] = efill[m] = nexact;
-        nexact += ecount[m];
+    for (int m = 0; m < kModes; m++) nexact += ecount[m];
+    const int n = base[kGroups];
+    const DescLayout L = desc_layout(n, nexact);
+    s.h_prefix = (int*)(s.h_desc + L.prefix);
+    s.d_prefix = (int*)(s.d_desc + L.prefix);
+    s.h_eprefix = (int*)(s.h_desc + L.eprefix);
+    s.d_eprefix = (int*)(s.d_desc + L.eprefix);
+    s.h_poses = (FastPose*)(s.h_desc + L.poses);
+    s.d_poses = (FastPose*)(s.d_desc + L.poses);
+    s.h_evs = (ExactView*)(s.h_desc + L.evs);
+    s.d_evs = (ExactView*)(s.d_desc + L.evs);
+    for (int m = 0, at = 0; m < kModes; m++) {
+        estart[m] = efill[m] = at;
+        at += ecount[m];
         s.h_eprefix[estart[m] + m] = 0;
     }
     int tabs[kGroups] = {};  // each group's largest volume table (nx + ny + nz)
@@ -377,13 +413,7 @@ int launch_slot(birp_server* S, Slot& s) {
             pre[k - base[g] + 1] = (int)next;
         }
     }
-    const int n = base[kGroups];
-    hipError_t e = hipSuccess;
-    if (n) e = hipMemcpyAsync(s.d_poses, s.h_poses, sizeof(FastPose) * n, hipMemcpyHostToDevice, s.stream);
-    if (e == hipSuccess && n) e = hipMemcpyAsync(s.d_prefix, s.h_prefix, sizeof(int) * (n + kGroups), hipMemcpyHostToDevice, s.stream);
-    if (e == hipSuccess && nexact) e = hipMemcpyAsync(s.d_evs, s.h_evs, sizeof(ExactView) * nexact, hipMemcpyHostToDevice, s.stream);
-    if (e == hipSuccess && nexact)
-        e = hipMemcpyAsync(s.d_eprefix, s.h_eprefix, sizeof(int) * (nexact + kModes), hipMemcpyHostToDevice, s.stream);
+    hipError_t e = hipMemcpyAsync(s.d_desc + L.lo, s.h_desc + L.lo, L.hi - L.lo, hipMemcpyHostToDevice, s.stream);
     if (e != hipSuccess) return hip_fail(e, "server descriptor upload");
     for (int m = 0; m < kModes; m++) {
         if (!ecount[m]) continue;
@@ -398,7 +428,6 @@ int launch_slot(birp_server* S, Slot& s) {
         const birp_tuning& t = gt[g];
         const int kdt = g / 3;
         const int total_tiles = s.h_prefix[base[g] + g + count];
-        if ((rc = launch_reset_counter(s.d_work + g * 256, s.stream))) return rc;
         const int grid = fast_grid_for(kdt, t.lanes_per_ray, t.unroll, total_tiles, tabs[g]);
         rc = launch_fast_batch(S->d_vols, kdt, s.d_poses + base[g], s.d_prefix + base[g] + g, count, total_tiles, s.d_work + g * 256,
                                t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll, grid, s.stream, tabs[g]);
@@ -640,15 +669,12 @@ extern "C" int birp_server_create(birp_device* dev, const birp_server_config* cf
         if (rc) break;
         if ((e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking)) != hipSuccess) { rc = hip_fail(e, "stream"); break; }
         if ((e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming)) != hipSuccess) { rc = hip_fail(e, "event"); break; }
-        if ((e = hipHostMalloc(&s.h_poses, sizeof(FastPose) * nb, hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
-        if ((e = hipMalloc(&s.d_poses, sizeof(FastPose) * nb)) != hipSuccess) { rc = hip_fail(e, "poses"); break; }
-        if ((e = hipHostMalloc(&s.h_prefix, sizeof(int) * (nb + kGroups), hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
-        if ((e = hipMalloc(&s.d_prefix, sizeof(int) * (nb + kGroups))) != hipSuccess) { rc = hip_fail(e, "prefix"); break; }
-        if ((e = hipMalloc(&s.d_work, sizeof(unsigned) * kGroups * 256)) != hipSuccess) { rc = hip_fail(e, "counter"); break; }
-        if ((e = hipHostMalloc(&s.h_evs, sizeof(ExactView) * nb, hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
-        if ((e = hipMalloc(&s.d_evs, sizeof(ExactView) * nb)) != hipSuccess) { rc = hip_fail(e, "views"); break; }
-        if ((e = hipHostMalloc(&s.h_eprefix, sizeof(int) * (nb + 8), hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
-        if ((e = hipMalloc(&s.d_eprefix, sizeof(int) * (nb + 8))) != hipSuccess) { rc = hip_fail(e, "prefix"); break; }
+        // every region of a launch fits: each is largest when all max_batch requests fill it
+        const size_t bytes = desc_layout(nb, nb).hi;
+        if ((e = hipHostMalloc(&s.h_desc, bytes, hipHostMallocDefault)) != hipSuccess) { rc = hip_fail(e, "pinned"); break; }
+        if ((e = hipMalloc(&s.d_desc, bytes)) != hipSuccess) { rc = hip_fail(e, "descriptors"); break; }
+        memset(s.h_desc, 0, kCounterBytes);  // the counters stay zero on the host
+        s.d_work = (unsigned*)s.d_desc;
         if (cfg->reserve_pixels) {
             if ((rc = slot_alloc_images(s, cfg->reserve_pixels))) break;
             s.held = cfg->reserve_pixels * 4;
@@ -690,15 +716,8 @@ extern "C" int birp_server_destroy(birp_server* S) {
         if (s.stream) (void)hipStreamSynchronize(s.stream);
         if (s.d_images) (void)hipFree(s.d_images);
         if (s.h_images) (void)hipHostFree(s.h_images);
-        if (s.h_poses) (void)hipHostFree(s.h_poses);
-        if (s.d_poses) (void)hipFree(s.d_poses);
-        if (s.h_prefix) (void)hipHostFree(s.h_prefix);
-        if (s.d_prefix) (void)hipFree(s.d_prefix);
-        if (s.d_work) (void)hipFree(s.d_work);
-        if (s.h_evs) (void)hipHostFree(s.h_evs);
-        if (s.d_evs) (void)hipFree(s.d_evs);
-        if (s.h_eprefix) (void)hipHostFree(s.h_eprefix);
-        if (s.d_eprefix) (void)hipFree(s.d_eprefix);
+        if (s.h_desc) (void)hipHostFree(s.h_desc);
+        if (s.d_desc) (void)hipFree(s.d_desc);
         if (s.done) (void)hipEventDestroy(s.done);
         if (s.stream) (void)hipStreamDestroy(s.stream);
     }
