@@ -498,7 +498,27 @@ __device__ __forceinline__ void exact_block(const DevVolume& v, const ExactView&
         unsigned t_lo = t_base + (has ? t_first : 0u), t_hi = t_base + (has ? (unsigned)kl * 4u : 0u);
         // keep the selects above out of the loop (the compiler would otherwise redo them per sample)
         asm volatile("" : "+v"(t_lo), "+v"(t_hi));
-        for (int kb = 0; kb < trips; kb += P * S) {
+        // Trips that every lane of the wave fills (kb + P S <= the wave's shortest range) need
+        // neither the clamp of t's address nor the mask, and read their P t values at constant
+        // offsets from one address. The arithmetic and the sum order are those of the loop below.
+        const int full = wave_min_i(len);
+        int kb = 0;
+        for (; kb + P * S <= full; kb += P * S) {
+            QuadFetch q[P];
+            const unsigned tb = t_lo + (unsigned)kb * 4u;
+#pragma unroll
+            for (int u = 0; u < P; ++u) {
+                const float t = *(lds_float_ptr)(uintptr_t)(tb + (unsigned)(u * S) * 4u);
+                const ExactIdx r = exact_index<false>(v, rx, ry, rz, qx, qy, qz, t);
+                q[u] = quad_fetch_inside(qc, r.xd, r.yd, r.zd);
+            }
+            float sc[P];
+#pragma unroll
+            for (int u = 0; u < P; ++u) sc[u] = scale_pos(quad_lerp<true>(q[u]) * w);  // Camera.cpp:201
+#pragma unroll
+            for (int u = 0; u < P; ++u) acc = add_in_order<S>(acc, sc[u]);
+        }
+        for (; kb < trips; kb += P * S) {
             QuadFetch q[P];
 #pragma unroll
             for (int u = 0; u < P; ++u) {
