@@ -207,7 +207,7 @@ int default_exact_lanes(bool cmem, int W, int H) {
 // on each sample's loads (the 256 orbit views rendered one by one: 8.04 ms against 8.85 with 2;
 // the 576x336 x 800 view 0.0900 against 0.0916 ms). Exact batches of more blocks than the GPU
 // holds at once keep 2 (exact_batch_setup): their many waves hide the loads already, and 4 made
-// the orbit batch 2% slower.
+// the orbit batch 1-2% slower.
 int default_exact_pipe() {
     static int pipe = [] {
         const char* e = getenv("BIRP_EXACT_PIPE");
@@ -1236,7 +1236,7 @@ int exact_batch_setup(birp_batch* b) {
         // per CU, the occupancy of the 4-samples-in-flight kernel) renders like single views:
         // 4 lanes per ray when every image has at most 60,000 rays, 4 samples in flight, and one
         // snake deal over the CUs (exact_batch_plan). Larger groups keep 2 and 2 (the orbit batch
-        // measured 2% slower with 4 in flight and 10% slower with 4 lanes).
+        // measured 1-2% slower with 4 in flight and 10% slower with 4 lanes).
         if (m != SM_CMEM && dev->ncu > 0) {
             const long long cap = 7LL * dev->ncu;
             long long t2 = 0, t4 = 0;
