@@ -130,6 +130,8 @@ int hip_fail(hipError_t e, const char* what);
 int validate_view(const birp_view* v, bool exact);
 int fast_pose(const birp_volume* vol, const birp_view* v, int vol_index, float* out_dev, int tile_w, int tile_h, FastPose& fp);
 int exact_view(const birp_volume* vol, const birp_view* v, float* out_dev, ExactView& ev);
+int default_exact_lanes(bool cmem, int W, int H);
+int default_exact_pipe();
 int tiles_of(const birp_view& v, int tile_w, int tile_h);
 bool take_injected_failure(birp_device* dev);
 int choose_tuning(const birp_volume* vol, const std::vector<birp_view>& views, birp_tuning& t);

@@ -413,9 +413,22 @@ int launch_slot(birp_server* S, Slot& s) {
             pre[k - base[g] + 1] = (int)next;
         }
     }
-    hipError_t e = hipMemcpyAsync(s.d_desc + L.lo, s.h_desc + L.lo, L.hi - L.lo, hipMemcpyHostToDevice, s.stream);
+    // A lone exact request renders as its single exact render does: its view goes to the kernel by
+    // value, with the single renders' lanes per ray and samples in flight, so that without fast
+    // requests in the launch nothing is uploaded (the same bits as in a multi-view launch)
+    const Request* lone = nullptr;
+    for (size_t k = 0; k < s.reqs.size() && nexact == 1; k++)
+        if (exact_request(s.reqs[k])) lone = s.reqs[k];
+    hipError_t e = hipSuccess;
+    if (n || !lone) e = hipMemcpyAsync(s.d_desc + L.lo, s.h_desc + L.lo, L.hi - L.lo, hipMemcpyHostToDevice, s.stream);
     if (e != hipSuccess) return hip_fail(e, "server descriptor upload");
-    for (int m = 0; m < kModes; m++) {
+    if (lone) {
+        const DevVolume& dv = lone->vol->dv;
+        rc = launch_exact(dv, s.h_evs[0], default_exact_lanes(dv.kind == SK_CMEM, lone->req.view.width, lone->req.view.height),
+                          default_exact_pipe(), s.stream, nullptr);
+        if (rc) return fail(rc, "exact launch failed");
+    }
+    for (int m = 0; m < kModes && !lone; m++) {
         if (!ecount[m]) continue;
         const int first = estart[m] + m;
         rc = launch_exact_multi(S->d_vols, s.d_evs + estart[m], s.d_eprefix + first, ecount[m], s.h_eprefix[first + ecount[m]], m,

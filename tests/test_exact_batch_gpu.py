@@ -187,6 +187,30 @@ def test_exact_batch_argument_checks(spaces):
         b.free()
 
 
+def test_server_lone_exact_requests_bitwise(device, spaces):
+    """Exact requests sent one at a time launch alone (the view by value to the single-view exact
+    kernel, nothing uploaded): over a positive cube, a signed cube, a nearest-sampled cube and a
+    CMEM model, at sizes on both sides of the lanes-per-ray thresholds, each result equals
+    birp_render of its view bit for bit, also into a pinned result buffer."""
+    vols, _ = spaces
+    views, which = mixed_views(12, seed=93)
+    views += [W.view(160, 130, 0.25), W.view(300, 210, 0.25)]  # 20,800 and 63,000 rays
+    which += [0, 0]
+    srv = capi.Server(device, memory_budget=8 << 30, max_batch=8, max_inflight=2)
+    try:
+        idx = [srv.add_volume(v) for v in vols]
+        for k, (v, w) in enumerate(zip(views, which)):
+            print(f"lone exact request {k}: space {w}, {v.width}x{v.height} n={v.samples}", flush=True)
+            out = capi.pinned_empty((v.height, v.width)) if k % 2 else None
+            st, img, info = srv.wait(srv.submit(idx[w], v, out), 60000.0)
+            assert st == capi.OK, (k, st)
+            assert info.batch_size == 1
+            want, _ = vols[w].render(v)
+            assert np.array_equal(bits(img), bits(want)), (k, w)
+    finally:
+        srv.destroy()
+
+
 def test_server_exact_requests_bitwise(device, spaces):
     """Served requests in BIRP_MATH_EXACT over a positive cube, a signed cube (general quad mode), a
     nearest-sampled cube and a CMEM model share slot launches (one exact launch per sample mode)
