@@ -213,5 +213,9 @@ int noalloc_stage_head(const DevVolume& v, const int* prefix, void* scratch, voi
 int noalloc_stage_poses(const FastPose* poses, int first, int count, void* scratch, void* stream);
 // The grid of one persistent launch over `tiles` tiles: the occupancy-limited grid, or about one
 // wave per tile when that is smaller (idle waves cost serialized work-queue claims on exit).
+// One pose over one volume, both by value, as a plain grid of its tiles (k_render_fast_one): no
+// descriptors on the device and no work counters; the same per-tile arithmetic as launch_fast_batch.
+int launch_fast_one(const DevVolume& v, int dtype, const FastPose& pose, int total_tiles, int tile_w, int tile_h, int lanes_per_ray,
+                    int unroll, void* stream, int tab_entries);
 int fast_grid_for(int dtype, int lanes_per_ray, int unroll, int tiles, int tab_entries);
 }  // namespace birp

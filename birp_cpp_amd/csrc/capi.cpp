@@ -307,10 +307,6 @@ extern "C" int birp_device_close(birp_device* dev) {
     (void)hipSetDevice(dev->ordinal);
     if (dev->stream) (void)hipStreamSynchronize(dev->stream);
     if (dev->scratch) (void)hipFree(dev->scratch);
-    if (dev->fast_d) (void)hipFree(dev->fast_d);
-    if (dev->fast_h) (void)hipHostFree(dev->fast_h);
-    for (hipEvent_t ev : dev->fast_ev)
-        if (ev) (void)hipEventDestroy(ev);
     if (dev->pinned) (void)hipHostFree(dev->pinned);
     if (dev->rbuf) (void)hipFree(dev->rbuf);
     if (dev->d_counter) (void)hipFree(dev->d_counter);
@@ -1637,6 +1633,8 @@ extern "C" int birp_render_views_noalloc(birp_volume* vol, const birp_view* view
 // A single fast render: a one-pose launch of the persistent kernel (no graph). Its descriptors go
 // through the device's ring of pinned slots, so the host waits only for the upload issued
 // kFastRing renders earlier, not for the work queued before this render.
+// A fast single render: its pose and volume go to the one-shot kernel by value, so the launch
+// needs nothing uploaded and no counters reset.
 static int render_fast_into(birp_volume* vol, const birp_view* view, float* dimg) {
     birp_device* dev = vol->dev;
     if (view->flags & BIRP_RENDER_ACCUMULATE) return fail(BIRP_ERR_UNSUPPORTED, "fast renders do not accumulate");
@@ -1644,41 +1642,8 @@ static int render_fast_into(birp_volume* vol, const birp_view* view, float* dimg
     FastPose fp;
     int rc = fast_pose(vol, view, 0, dimg, t.tile_w, t.tile_h, fp);
     if (rc) return rc;
-    const int prefix[2] = {0, tiles_of(*view, t.tile_w, t.tile_h)};
-    constexpr size_t kSlot = 4096;
-    static_assert(sizeof(DevVolume) <= 1024 && sizeof(FastPose) <= 1024, "fast render descriptor layout");
-    hipError_t e;
-    if (!dev->fast_h) {
-        if ((e = hipHostMalloc(&dev->fast_h, kSlot * birp_device::kFastRing, hipHostMallocDefault)) != hipSuccess)
-            return hip_fail(e, "hipHostMalloc(fast descriptors)");
-        if ((e = hipMalloc(&dev->fast_d, kSlot * birp_device::kFastRing)) != hipSuccess) {
-            (void)hipHostFree(dev->fast_h);
-            dev->fast_h = nullptr;
-            return hip_fail(e, "hipMalloc(fast descriptors)");
-        }
-        for (hipEvent_t& ev : dev->fast_ev)
-            if (!ev && (e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return hip_fail(e, "event");
-    }
-    const int slot = dev->fast_next;
-    if (dev->fast_pending[slot]) {
-        if ((e = hipEventSynchronize(dev->fast_ev[slot])) != hipSuccess) return hip_fail(e, "fast descriptor slot");
-        dev->fast_pending[slot] = false;
-    }
-    char* hb = (char*)dev->fast_h + slot * kSlot;
-    char* db = (char*)dev->fast_d + slot * kSlot;
-    memcpy(hb, &vol->dv, sizeof(DevVolume));
-    memcpy(hb + 1024, &fp, sizeof(FastPose));
-    memcpy(hb + 2048, prefix, sizeof(prefix));
     if (take_injected_failure(dev)) return fail(BIRP_ERR_INJECTED, "injected failure");
-    if ((e = hipMemcpyAsync(db, hb, 3072, hipMemcpyHostToDevice, dev->stream)) != hipSuccess ||
-        (e = hipEventRecord(dev->fast_ev[slot], dev->stream)) != hipSuccess)
-        return hip_fail(e, "fast render upload");
-    dev->fast_pending[slot] = true;
-    dev->fast_next = (slot + 1) % birp_device::kFastRing;
-    unsigned* dwork = (unsigned*)(db + 3072);
-    if ((rc = launch_reset_counter(dwork, dev->stream))) return rc;
-    const int g = fast_kernel_dtype(vol->dv);
-    const int grid = fast_grid_for(g, t.lanes_per_ray, t.unroll, prefix[1], volume_tab_entries(vol->dv));
-    return launch_fast_batch((const DevVolume*)db, g, (const FastPose*)(db + 1024), (const int*)(db + 2048), 1, prefix[1], dwork,
-                             t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll, grid, dev->stream, volume_tab_entries(vol->dv));
+    rc = launch_fast_one(vol->dv, fast_kernel_dtype(vol->dv), fp, tiles_of(*view, t.tile_w, t.tile_h), t.tile_w, t.tile_h,
+                         t.lanes_per_ray, t.unroll, dev->stream, volume_tab_entries(vol->dv));
+    return rc ? fail(rc, "fast render launch failed") : BIRP_OK;
 }

@@ -29,15 +29,6 @@ struct birp_device {
     void* d_counter = nullptr;         // 64 B: the upload's class-scan flags
     void* scratch = nullptr;           // image / point staging, grown on demand
     size_t scratch_bytes = 0;
-    // single fast renders: a ring of descriptor slots [DevVolume | FastPose | tile prefix | work
-    // counter], 1 KiB each part, in pinned host memory and on the device; slot k is reused once
-    // the event recorded after its upload has completed
-    static constexpr int kFastRing = 8;
-    void* fast_h = nullptr;
-    void* fast_d = nullptr;
-    hipEvent_t fast_ev[kFastRing] = {};
-    bool fast_pending[kFastRing] = {};
-    int fast_next = 0;
     void* pinned = nullptr;            // pinned host staging for birp_render's image copy
     void* pinned_dev = nullptr;        // its device address (null when the device does not map it)
     size_t pinned_bytes = 0;

@@ -1257,6 +1257,193 @@ __device__ __forceinline__ void kahan_add(float x, float& sum, float& nc) {
     sum = t;
 }
 
+// The lanes of a fast-kernel wave: a wave traces a tile of tw x th rays, S lanes per ray.
+// Positive kernels put 2x2 pixel blocks on lane quads, the S lanes of a ray on S consecutive
+// quads of a 16-lane row (as exact_lane: a quad's record loads then mostly share one cache
+// line); signed kernels keep the S lanes of a ray adjacent, since their first-negative test
+// reads the ray's S bits of one ballot as a group.
+struct FastLane {
+    int tw, th, ray, ti, tj, sub, sub_step;
+};
+template <int S, bool SIGNED>
+__device__ __forceinline__ FastLane fast_lane(int lane, int tw_log2) {
+    FastLane f;
+    f.tw = 1 << tw_log2;
+    f.th = (64 / S) >> tw_log2;
+    const bool blocks = !SIGNED && f.tw >= 2 && f.th >= 2;
+    f.ray = lane / S;
+    if (blocks) {
+        const int r16 = lane & 15, quad = r16 >> 2;
+        f.sub = quad & (S - 1);
+        const int blk = (lane >> 4) * (4 / S) + quad / S;
+        f.ti = (blk & ((f.tw >> 1) - 1)) * 2 + (lane & 1);
+        f.tj = (blk >> (tw_log2 - 1)) * 2 + ((lane >> 1) & 1);
+    } else {
+        f.sub = lane % S;
+        f.ti = f.ray & (f.tw - 1);
+        f.tj = f.ray >> tw_log2;
+    }
+    f.sub_step = blocks ? 4 : 1;  // lane distance between sub-lanes m and m + 1 of a ray
+    return f;
+}
+
+// One wave renders tile `tile` (row-major over the image's tiles) of pose P over volume V, whose
+// offset tables the wave holds in `tab`; every lane of the wave calls it.
+template <int KV, int S, int U>
+__device__ __forceinline__ void fast_tile(const DevVolume& V, const FastPose& P, int tile, const unsigned* tab, const FastLane& L) {
+    constexpr bool CMEM = KV == BIRP_KDT_CMEM;
+    constexpr int DT = CMEM ? BIRP_DTYPE_F32 : (KV & 3);
+    constexpr bool SIGNED = CMEM || (KV & BIRP_KDT_SIGNED) != 0;
+    const int tw = L.tw, th = L.th, ray = L.ray, ti = L.ti, tj = L.tj, sub = L.sub, sub_step = L.sub_step;
+    const float K = (float)BIRP_SCALE_FOLDED;
+    const __amdgpu_buffer_rsrc_t rs = (DT == BIRP_DTYPE_F32) ? make_rsrc(V.quad, V.quad_bytes) : make_rsrc(V.oct, V.oct_bytes);
+    const int tx = tile % P.tiles_x, ty = tile / P.tiles_x;
+    const int i = tx * tw + ti;
+    const int j = ty * th + tj;
+    const bool valid = (i < P.W) && (j < P.H);
+
+    int kf = P.n, kl = -1;
+    float ix0 = 0.f, iy0 = 0.f, iz0 = 0.f, dix = 0.f, diy = 0.f, diz = 0.f;
+    if (valid) {
+        // FP64 ray (the reference's formulas, in double)
+        double sx = (2 * ((i + 0.5) * P.inv_w) - 1) * P.tanx_aspect;
+        double sy = (1 - 2 * ((j + 0.5) * P.inv_h)) * P.tany;
+        // (divisions by a norm or a spacing as products with its FP64 reciprocal: a relative
+        // change of ~1e-16, far below the FP32 march)
+        const double ind = 1.0 / sqrt(sx * sx + sy * sy + 1.0);
+        double u0 = sx * ind, u1 = sy * ind, u2 = -ind;
+        double w0 = P.rinv[0] * u0 + P.rinv[1] * u1 + P.rinv[2] * u2;
+        double w1 = P.rinv[3] * u0 + P.rinv[4] * u1 + P.rinv[5] * u2;
+        double w2 = P.rinv[6] * u0 + P.rinv[7] * u1 + P.rinv[8] * u2;
+        const double iwd = 1.0 / sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+        double dx = w0 * iwd, dy = w1 * iwd, dz = w2 * iwd;
+        if constexpr (CMEM) {
+            // the model has no bounds: every sample, marched in world coordinates
+            kf = 0;
+            kl = P.n - 1;
+            const double dt = P.dt, t0 = 0.5 * dt;
+            ix0 = (float)(P.px + dx * t0);
+            iy0 = (float)(P.py + dy * t0);
+            iz0 = (float)(P.pz + dz * t0);
+            dix = (float)(dx * dt);
+            diy = (float)(dy * dt);
+            diz = (float)(dz * dt);
+        } else {
+            // FP64 cube entry/exit: inside iff 0 <= (x - o)/s < n - 1 per axis
+            double tin = -1e300, tout = 1e300;
+            slab_axis(P.px, dx, (double)V.ox, (double)V.ox + (double)(V.nx - 1) * V.sx, tin, tout);
+            slab_axis(P.py, dy, (double)V.oy, (double)V.oy + (double)(V.ny - 1) * V.sy, tin, tout);
+            slab_axis(P.pz, dz, (double)V.oz, (double)V.oz + (double)(V.nz - 1) * V.sz, tin, tout);
+            if (tin <= tout) {
+                const double sc = P.sc;
+                double a = ceil(tin * sc - 0.5), b = ceil(tout * sc - 0.5) - 1.0;
+                a = fmax(a, 0.0);
+                b = fmin(b, (double)(P.n - 1));
+                if (a <= b) {
+                    kf = (int)a;
+                    kl = (int)b;
+                }
+            }
+            if (P.semantics == BIRP_SEM_REFERENCE && kf != 0) kl = -1;
+            if (kf <= kl) {
+                const double dt = P.dt, t0 = (kf + 0.5) * dt;
+                ix0 = (float)((P.px + dx * t0 - V.ox) * V.rdx);
+                iy0 = (float)((P.py + dy * t0 - V.oy) * V.rdy);
+                iz0 = (float)((P.pz + dz * t0 - V.oz) * V.rdz);
+                dix = (float)(dx * dt * V.rdx);
+                diy = (float)(dy * dt * V.rdy);
+                diz = (float)(dz * dt * V.rdz);
+            }
+        }
+    }
+    const int cnt = (kl >= kf) ? (kl - kf + 1) : 0;
+    const int my = (cnt > sub) ? (cnt - sub + S - 1) / S : 0;  // samples of this sub-lane
+    const int trips = wave_max_i(my);
+    if (!CMEM && cnt > 0) {
+        // every sample inside [0, nextbelow(n - 1)] per axis, so that the cell index is the
+        // truncation (0 .. n-2) with no clamp per sample; a sample rounded onto the far face
+        // keeps weight ~1 on the last grid point
+        fit_march(ix0, dix, __uint_as_float(__float_as_uint(V.fnx - 1.0f) - 1u), cnt - 1);
+        fit_march(iy0, diy, __uint_as_float(__float_as_uint(V.fny - 1.0f) - 1u), cnt - 1);
+        fit_march(iz0, diz, __uint_as_float(__float_as_uint(V.fnz - 1.0f) - 1u), cnt - 1);
+    }
+    // the U samples of an iteration (neighbours on the ray, of similar size) are added plainly,
+    // and their partial sum enters the running sum by one Kahan step. The trips that every lane
+    // of the wave fills completely need no clamp and no mask; the rest of the trips run below
+    // with both (the same samples in the same order and grouping either way).
+    float sum = 0.f, nc = 0.f;
+    if constexpr (SIGNED) {
+        // Camera.cpp:207-210: a ray ends at its first negative sample. The S lanes of a ray take
+        // samples q*S + sub in trip q, so the ray's first negative sample of a trip is the lowest
+        // set bit of its S-bit group in the wave's ballot: a lane adds its sample unless its
+        // group has a negative sample at or before it, and the ray dies with any in the group.
+        const int gshift = ray * S;
+        bool alive = true;  // the same in all S lanes of a ray
+        for (int q = 0; q < trips; ++q) {
+            const int js = q * S + sub;
+            const bool act = alive && js < cnt;
+            const float fj = (float)min(js, cnt - 1);
+            const float xd = __builtin_fmaf(fj, dix, ix0);
+            const float yd = __builtin_fmaf(fj, diy, iy0);
+            const float zd = __builtin_fmaf(fj, diz, iz0);
+            float x;
+            if constexpr (CMEM) x = cmem_fast(V.cmem, xd, yd, zd);
+            else x = fast_sample<DT>(V, rs, tab, xd, yd, zd);
+            const unsigned long long neg = __ballot(act && x < 0.f);
+            const unsigned grp = (unsigned)(neg >> gshift) & ((1u << S) - 1u);
+            const bool cut = (grp & ((2u << sub) - 1u)) != 0u;
+            kahan_add((act && !cut) ? x : 0.f, sum, nc);
+            alive = alive && grp == 0u;
+            if (__ballot(alive && (q + 1) * S < cnt) == 0ull) break;  // every ray of the wave is done
+        }
+    }
+    const int tfull = SIGNED ? 0 : wave_min_i(my) / U * U;
+    int q = SIGNED ? trips : 0;
+    for (; q < tfull; q += U) {
+        float part = 0.f;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const float fj = (float)((q + u) * S + sub);
+            const float xd = __builtin_fmaf(fj, dix, ix0);
+            const float yd = __builtin_fmaf(fj, diy, iy0);
+            const float zd = __builtin_fmaf(fj, diz, iz0);
+            const float x = fast_sample<DT>(V, rs, tab, xd, yd, zd);
+            part = (u == 0) ? x : part + x;
+        }
+        kahan_add(part, sum, nc);
+    }
+    for (; q < trips; q += U) {
+        float part = 0.f;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int js = (q + u) * S + sub;  // sample offset from kf
+            const bool act = js < cnt;
+            // a lane past its ray's last sample re-reads that sample (its result is masked); an
+            // empty ray has i0 = d = 0 and samples the grid origin
+            const float fj = (float)min(js, cnt - 1);
+            const float xd = __builtin_fmaf(fj, dix, ix0);
+            const float yd = __builtin_fmaf(fj, diy, iy0);
+            const float zd = __builtin_fmaf(fj, diz, iz0);
+            const float x = fast_sample<DT>(V, rs, tab, xd, yd, zd);
+            const float xa = act ? x : 0.f;
+            part = (u == 0) ? xa : part + xa;
+        }
+        kahan_add(part, sum, nc);
+    }
+    // each sub-lane's Kahan total, then the S partials combined in sub order; the step weight
+    // and the scale constant are applied once per ray (all in FP32)
+    float tot = sum + nc;
+    if constexpr (S > 1) {
+        float t1 = __shfl_xor(tot, sub_step, 64);
+        tot = (sub & 1) ? t1 + tot : tot + t1;
+        if constexpr (S == 4) {
+            float t2 = __shfl_xor(tot, 2 * sub_step, 64);
+            tot = (sub & 2) ? t2 + tot : tot + t2;
+        }
+    }
+    if (valid && sub == 0) P.out[(size_t)j * P.W + i] = (tot * P.w) * K;
+}
+
 // Work distribution: items (pose-major, tiles row-major) are split into 8 contiguous ranges, one
 // per XCD queue. Block b starts on queue b % 8 (dispatch places consecutive blocks on successive
 // XCDs, so a queue's tiles -- neighbouring rays of one image -- meet in one L2) and moves on to
@@ -1279,27 +1466,7 @@ __global__ __launch_bounds__(BIRP_BLOCK, 6) void k_render_fast(const DevVolume* 
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
     unsigned* tab = s_tab + wave * tab_stride;
-    const int tw = 1 << tw_log2, th = (64 / S) >> tw_log2;
-    // Positive kernels put 2x2 pixel blocks on lane quads, the S lanes of a ray on S consecutive
-    // quads of a 16-lane row (as exact_lane: a quad's record loads then mostly share one cache
-    // line); signed kernels keep the S lanes of a ray adjacent, since their first-negative test
-    // reads the ray's S bits of one ballot as a group.
-    const bool blocks = !SIGNED && tw >= 2 && th >= 2;
-    const int ray = lane / S;
-    int ti, tj, sub;
-    if (blocks) {
-        const int r16 = lane & 15, quad = r16 >> 2;
-        sub = quad & (S - 1);
-        const int blk = (lane >> 4) * (4 / S) + quad / S;
-        ti = (blk & ((tw >> 1) - 1)) * 2 + (lane & 1);
-        tj = (blk >> (tw_log2 - 1)) * 2 + ((lane >> 1) & 1);
-    } else {
-        sub = lane % S;
-        ti = ray & (tw - 1);
-        tj = ray >> tw_log2;
-    }
-    const int sub_step = blocks ? 4 : 1;  // lane distance between sub-lanes m and m + 1 of a ray
-    const float K = (float)BIRP_SCALE_FOLDED;
+    const FastLane fl = fast_lane<S, SIGNED>(lane, tw_log2);
     int cached_vol = -1;
     int queue = blockIdx.x & 7;
     for (int tried = 0; tried < 8;) {
@@ -1330,154 +1497,38 @@ __global__ __launch_bounds__(BIRP_BLOCK, 6) void k_render_fast(const DevVolume* 
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             cached_vol = P.vol;
         }
-        const __amdgpu_buffer_rsrc_t rs = (DT == BIRP_DTYPE_F32) ? make_rsrc(V.quad, V.quad_bytes) : make_rsrc(V.oct, V.oct_bytes);
         const int tile = item - prefix[pidx];
-        const int tx = tile % P.tiles_x, ty = tile / P.tiles_x;
-        const int i = tx * tw + ti;
-        const int j = ty * th + tj;
-        const bool valid = (i < P.W) && (j < P.H);
-
-        int kf = P.n, kl = -1;
-        float ix0 = 0.f, iy0 = 0.f, iz0 = 0.f, dix = 0.f, diy = 0.f, diz = 0.f;
-        if (valid) {
-            // FP64 ray (the reference's formulas, in double)
-            double sx = (2 * ((i + 0.5) * P.inv_w) - 1) * P.tanx_aspect;
-            double sy = (1 - 2 * ((j + 0.5) * P.inv_h)) * P.tany;
-            // (divisions by a norm or a spacing as products with its FP64 reciprocal: a relative
-            // change of ~1e-16, far below the FP32 march)
-            const double ind = 1.0 / sqrt(sx * sx + sy * sy + 1.0);
-            double u0 = sx * ind, u1 = sy * ind, u2 = -ind;
-            double w0 = P.rinv[0] * u0 + P.rinv[1] * u1 + P.rinv[2] * u2;
-            double w1 = P.rinv[3] * u0 + P.rinv[4] * u1 + P.rinv[5] * u2;
-            double w2 = P.rinv[6] * u0 + P.rinv[7] * u1 + P.rinv[8] * u2;
-            const double iwd = 1.0 / sqrt(w0 * w0 + w1 * w1 + w2 * w2);
-            double dx = w0 * iwd, dy = w1 * iwd, dz = w2 * iwd;
-            if constexpr (CMEM) {
-                // the model has no bounds: every sample, marched in world coordinates
-                kf = 0;
-                kl = P.n - 1;
-                const double dt = P.dt, t0 = 0.5 * dt;
-                ix0 = (float)(P.px + dx * t0);
-                iy0 = (float)(P.py + dy * t0);
-                iz0 = (float)(P.pz + dz * t0);
-                dix = (float)(dx * dt);
-                diy = (float)(dy * dt);
-                diz = (float)(dz * dt);
-            } else {
-                // FP64 cube entry/exit: inside iff 0 <= (x - o)/s < n - 1 per axis
-                double tin = -1e300, tout = 1e300;
-                slab_axis(P.px, dx, (double)V.ox, (double)V.ox + (double)(V.nx - 1) * V.sx, tin, tout);
-                slab_axis(P.py, dy, (double)V.oy, (double)V.oy + (double)(V.ny - 1) * V.sy, tin, tout);
-                slab_axis(P.pz, dz, (double)V.oz, (double)V.oz + (double)(V.nz - 1) * V.sz, tin, tout);
-                if (tin <= tout) {
-                    const double sc = P.sc;
-                    double a = ceil(tin * sc - 0.5), b = ceil(tout * sc - 0.5) - 1.0;
-                    a = fmax(a, 0.0);
-                    b = fmin(b, (double)(P.n - 1));
-                    if (a <= b) {
-                        kf = (int)a;
-                        kl = (int)b;
-                    }
-                }
-                if (P.semantics == BIRP_SEM_REFERENCE && kf != 0) kl = -1;
-                if (kf <= kl) {
-                    const double dt = P.dt, t0 = (kf + 0.5) * dt;
-                    ix0 = (float)((P.px + dx * t0 - V.ox) * V.rdx);
-                    iy0 = (float)((P.py + dy * t0 - V.oy) * V.rdy);
-                    iz0 = (float)((P.pz + dz * t0 - V.oz) * V.rdz);
-                    dix = (float)(dx * dt * V.rdx);
-                    diy = (float)(dy * dt * V.rdy);
-                    diz = (float)(dz * dt * V.rdz);
-                }
-            }
-        }
-        const int cnt = (kl >= kf) ? (kl - kf + 1) : 0;
-        const int my = (cnt > sub) ? (cnt - sub + S - 1) / S : 0;  // samples of this sub-lane
-        const int trips = wave_max_i(my);
-        if (!CMEM && cnt > 0) {
-            // every sample inside [0, nextbelow(n - 1)] per axis, so that the cell index is the
-            // truncation (0 .. n-2) with no clamp per sample; a sample rounded onto the far face
-            // keeps weight ~1 on the last grid point
-            fit_march(ix0, dix, __uint_as_float(__float_as_uint(V.fnx - 1.0f) - 1u), cnt - 1);
-            fit_march(iy0, diy, __uint_as_float(__float_as_uint(V.fny - 1.0f) - 1u), cnt - 1);
-            fit_march(iz0, diz, __uint_as_float(__float_as_uint(V.fnz - 1.0f) - 1u), cnt - 1);
-        }
-        // the U samples of an iteration (neighbours on the ray, of similar size) are added plainly,
-        // and their partial sum enters the running sum by one Kahan step. The trips that every lane
-        // of the wave fills completely need no clamp and no mask; the rest of the trips run below
-        // with both (the same samples in the same order and grouping either way).
-        float sum = 0.f, nc = 0.f;
-        if constexpr (SIGNED) {
-            // Camera.cpp:207-210: a ray ends at its first negative sample. The S lanes of a ray take
-            // samples q*S + sub in trip q, so the ray's first negative sample of a trip is the lowest
-            // set bit of its S-bit group in the wave's ballot: a lane adds its sample unless its
-            // group has a negative sample at or before it, and the ray dies with any in the group.
-            const int gshift = ray * S;
-            bool alive = true;  // the same in all S lanes of a ray
-            for (int q = 0; q < trips; ++q) {
-                const int js = q * S + sub;
-                const bool act = alive && js < cnt;
-                const float fj = (float)min(js, cnt - 1);
-                const float xd = __builtin_fmaf(fj, dix, ix0);
-                const float yd = __builtin_fmaf(fj, diy, iy0);
-                const float zd = __builtin_fmaf(fj, diz, iz0);
-                float x;
-                if constexpr (CMEM) x = cmem_fast(V.cmem, xd, yd, zd);
-                else x = fast_sample<DT>(V, rs, tab, xd, yd, zd);
-                const unsigned long long neg = __ballot(act && x < 0.f);
-                const unsigned grp = (unsigned)(neg >> gshift) & ((1u << S) - 1u);
-                const bool cut = (grp & ((2u << sub) - 1u)) != 0u;
-                kahan_add((act && !cut) ? x : 0.f, sum, nc);
-                alive = alive && grp == 0u;
-                if (__ballot(alive && (q + 1) * S < cnt) == 0ull) break;  // every ray of the wave is done
-            }
-        }
-        const int tfull = SIGNED ? 0 : wave_min_i(my) / U * U;
-        int q = SIGNED ? trips : 0;
-        for (; q < tfull; q += U) {
-            float part = 0.f;
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const float fj = (float)((q + u) * S + sub);
-                const float xd = __builtin_fmaf(fj, dix, ix0);
-                const float yd = __builtin_fmaf(fj, diy, iy0);
-                const float zd = __builtin_fmaf(fj, diz, iz0);
-                const float x = fast_sample<DT>(V, rs, tab, xd, yd, zd);
-                part = (u == 0) ? x : part + x;
-            }
-            kahan_add(part, sum, nc);
-        }
-        for (; q < trips; q += U) {
-            float part = 0.f;
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int js = (q + u) * S + sub;  // sample offset from kf
-                const bool act = js < cnt;
-                // a lane past its ray's last sample re-reads that sample (its result is masked); an
-                // empty ray has i0 = d = 0 and samples the grid origin
-                const float fj = (float)min(js, cnt - 1);
-                const float xd = __builtin_fmaf(fj, dix, ix0);
-                const float yd = __builtin_fmaf(fj, diy, iy0);
-                const float zd = __builtin_fmaf(fj, diz, iz0);
-                const float x = fast_sample<DT>(V, rs, tab, xd, yd, zd);
-                const float xa = act ? x : 0.f;
-                part = (u == 0) ? xa : part + xa;
-            }
-            kahan_add(part, sum, nc);
-        }
-        // each sub-lane's Kahan total, then the S partials combined in sub order; the step weight
-        // and the scale constant are applied once per ray (all in FP32)
-        float tot = sum + nc;
-        if constexpr (S > 1) {
-            float t1 = __shfl_xor(tot, sub_step, 64);
-            tot = (sub & 1) ? t1 + tot : tot + t1;
-            if constexpr (S == 4) {
-                float t2 = __shfl_xor(tot, 2 * sub_step, 64);
-                tot = (sub & 2) ? t2 + tot : tot + t2;
-            }
-        }
-        if (valid && sub == 0) P.out[(size_t)j * P.W + i] = (tot * P.w) * K;
+        fast_tile<KV, S, U>(V, P, tile, tab, fl);
     }
+}
+
+// One pose as a plain grid, with the pose and its volume by value: the launch needs no descriptor
+// upload and no work counters (single fast renders, a lone served request). Wave w of block b
+// renders tile 4 b' + w, where blocks go round-robin over the 8 XCDs and b' numbers them XCD by
+// XCD, so that each XCD renders a contiguous range of neighbouring tiles, as the work queues of
+// k_render_fast do. The grid is a multiple of 8 blocks; waves past the last tile leave at once.
+template <int KV, int S, int U>
+__global__ __launch_bounds__(BIRP_BLOCK, 6) void k_render_fast_one(DevVolume V, FastPose P, int total, int tw_log2, int tab_stride) {
+    constexpr bool CMEM = KV == BIRP_KDT_CMEM;
+    constexpr int DT = CMEM ? BIRP_DTYPE_F32 : (KV & 3);
+    constexpr bool SIGNED = CMEM || (KV & BIRP_KDT_SIGNED) != 0;
+    static_assert(!SIGNED || U == 1, "signed kernels take one sample per lane per trip");
+    extern __shared__ unsigned s_tab[];
+    const int lane = lane_id();
+    const int wave = threadIdx.x >> 6;
+    const int b = (int)(blockIdx.x & 7u) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
+    const int item = b * (BIRP_BLOCK / 64) + wave;
+    if (item >= total) return;
+    unsigned* tab = s_tab + wave * tab_stride;
+    if constexpr (!CMEM) {
+        const unsigned* src = (DT == BIRP_DTYPE_F32) ? V.qtab : V.otab;
+        const int ntab = min(V.nx + V.ny + V.nz, tab_stride);  // the host sizes tab_stride to cover it
+        for (int q = lane; q < ntab; q += 64) tab[q] = src[q];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    fast_tile<KV, S, U>(V, P, item, tab, fast_lane<S, SIGNED>(lane, tw_log2));
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1812,6 +1863,50 @@ int launch_fast_batch(const DevVolume* vols_dev, int dtype, const FastPose* pose
         default: return BIRP_ERR_INVALID_ARGUMENT;
     }
 #undef BIRP_LFD
+}
+
+template <int DT, int S>
+static int launch_one_su(int unroll, int grid, hipStream_t st, const DevVolume& v, const FastPose& p, int total, int twl, int ts) {
+    const size_t lds = (size_t)(BIRP_BLOCK / 64) * ts * sizeof(unsigned);
+    if constexpr ((DT & (BIRP_KDT_SIGNED | BIRP_KDT_CMEM)) != 0) {
+        hipLaunchKernelGGL((k_render_fast_one<DT, S, 1>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, p, total, twl, ts);
+    } else {
+        switch (unroll) {
+            case 4: hipLaunchKernelGGL((k_render_fast_one<DT, S, 4>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, p, total, twl, ts); break;
+            case 2: hipLaunchKernelGGL((k_render_fast_one<DT, S, 2>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, p, total, twl, ts); break;
+            default: hipLaunchKernelGGL((k_render_fast_one<DT, S, 1>), dim3(grid), dim3(BIRP_BLOCK), lds, st, v, p, total, twl, ts); break;
+        }
+    }
+    return hip_status(hipGetLastError());
+}
+
+template <int DT>
+static int launch_one_d(int S, int unroll, int grid, hipStream_t st, const DevVolume& v, const FastPose& p, int total, int twl, int ts) {
+    if (S == 4) return launch_one_su<DT, 4>(unroll, grid, st, v, p, total, twl, ts);
+    if (S == 2) return launch_one_su<DT, 2>(unroll, grid, st, v, p, total, twl, ts);
+    return launch_one_su<DT, 1>(unroll, grid, st, v, p, total, twl, ts);
+}
+
+int launch_fast_one(const DevVolume& v, int dtype, const FastPose& pose, int total_tiles, int tile_w, int tile_h, int lanes_per_ray,
+                    int unroll, void* stream, int tab_entries) {
+    int twl = 0;
+    while ((1 << twl) < tile_w) twl++;
+    if ((1 << twl) != tile_w || tile_w * tile_h * lanes_per_ray != 64) return BIRP_ERR_INVALID_ARGUMENT;
+    if (dtype < 0 || dtype >= BIRP_KDT_COUNT) return BIRP_ERR_INVALID_ARGUMENT;
+    if (tab_entries < 0 || tab_entries > BIRP_FAST_TAB || total_tiles < 0) return BIRP_ERR_INVALID_ARGUMENT;
+    if (total_tiles == 0) return BIRP_OK;
+    const int ts = fast_tab_stride(dtype, tab_entries);
+    const int blocks = (int)(((long long)total_tiles + BIRP_BLOCK / 64 - 1) / (BIRP_BLOCK / 64));
+    const int grid = ((blocks + 7) / 8) * 8;
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype & (BIRP_KDT_SIGNED | BIRP_KDT_CMEM)) unroll = 1;  // the signed and CMEM kernels take one sample per lane per trip
+#define BIRP_LOD(KV) \
+    case KV: return launch_one_d<KV>(lanes_per_ray, unroll, grid, st, v, pose, total_tiles, twl, ts);
+    switch (dtype) {
+        BIRP_LOD(0) BIRP_LOD(1) BIRP_LOD(2) BIRP_LOD(3) BIRP_LOD(4) BIRP_LOD(5) BIRP_LOD(6) BIRP_LOD(7) BIRP_LOD(8)
+        default: return BIRP_ERR_INVALID_ARGUMENT;
+    }
+#undef BIRP_LOD
 }
 
 int fast_grid_for(int dtype, int lanes_per_ray, int unroll, int tiles, int tab_entries) {

@@ -414,13 +414,19 @@ int launch_slot(birp_server* S, Slot& s) {
         }
     }
     // A lone exact request renders as its single exact render does: its view goes to the kernel by
-    // value, with the single renders' lanes per ray and samples in flight, so that without fast
-    // requests in the launch nothing is uploaded (the same bits as in a multi-view launch)
+    // value, with the single renders' lanes per ray and samples in flight. A lone fast request's
+    // pose and volume go by value to the one-shot fast kernel, as a single fast render's do. Either
+    // way the request gives the same bits as in a multi-view launch, and a launch of lone requests
+    // uploads nothing.
     const Request* lone = nullptr;
-    for (size_t k = 0; k < s.reqs.size() && nexact == 1; k++)
-        if (exact_request(s.reqs[k])) lone = s.reqs[k];
+    const Request* lone_fast = nullptr;
+    for (size_t k = 0; k < s.reqs.size(); k++) {
+        if (exact_request(s.reqs[k])) lone = nexact == 1 ? s.reqs[k] : nullptr;
+        else lone_fast = n == 1 ? s.reqs[k] : nullptr;
+    }
     hipError_t e = hipSuccess;
-    if (n || !lone) e = hipMemcpyAsync(s.d_desc + L.lo, s.h_desc + L.lo, L.hi - L.lo, hipMemcpyHostToDevice, s.stream);
+    if ((n && !lone_fast) || (nexact && !lone))
+        e = hipMemcpyAsync(s.d_desc + L.lo, s.h_desc + L.lo, L.hi - L.lo, hipMemcpyHostToDevice, s.stream);
     if (e != hipSuccess) return hip_fail(e, "server descriptor upload");
     if (lone) {
         const DevVolume& dv = lone->vol->dv;
@@ -441,6 +447,12 @@ int launch_slot(birp_server* S, Slot& s) {
         const birp_tuning& t = gt[g];
         const int kdt = g / 3;
         const int total_tiles = s.h_prefix[base[g] + g + count];
+        if (lone_fast) {
+            rc = launch_fast_one(lone_fast->vol->dv, kdt, s.h_poses[base[g]], total_tiles, t.tile_w, t.tile_h, t.lanes_per_ray,
+                                 t.unroll, s.stream, tabs[g]);
+            if (rc) return fail(rc, "fast launch failed");
+            continue;
+        }
         const int grid = fast_grid_for(kdt, t.lanes_per_ray, t.unroll, total_tiles, tabs[g]);
         rc = launch_fast_batch(S->d_vols, kdt, s.d_poses + base[g], s.d_prefix + base[g] + g, count, total_tiles, s.d_work + g * 256,
                                t.tile_w, t.tile_h, t.lanes_per_ray, t.unroll, grid, s.stream, tabs[g]);
