@@ -184,24 +184,26 @@ bool take_injected_failure(birp_device* dev) {
     return false;
 }
 
-// Tuning knobs of the exact path, read once: BIRP_EXACT_LANES (lanes per ray: 1, 2, 4, 8),
+// Tuning knobs of the exact path, read once: BIRP_EXACT_LANES (lanes per ray: 1, 2, 4, 8, 16),
 // BIRP_EXACT_PIPE (samples in flight per lane in the positive-cube loop: 1..4) and
 // BIRP_EXACT_LAYOUT ("linear" reads the [z][y][x] copy instead of the quad bricks).
-// By default a single exact render takes 8 lanes per ray for images of at most 20,000 rays, 4 for
-// CMEM and for images of at most 60,000 rays, and 2 above: a small image's few waves are bound by
-// the chain of each ray's samples, which more lanes cut (ms per render over 6 orbit poses at
-// n = 800 with 4 samples in flight, 2 / 4 / 8 lanes: 144x84 0.032 / 0.020 / 0.016, 288x168 0.038 /
-// 0.027 / 0.030, 576x336 0.071 / 0.075 / 0.089).
+// By default a single exact render takes 16 lanes per ray for images of at most 8,000 rays, 8 for
+// at most 20,000, 4 for CMEM and for images of at most 60,000 rays, and 2 above: a small image's
+// few waves are bound by the chain of each ray's samples, which more lanes cut (ms per render over
+// 6 orbit poses at n = 800 with 4 samples in flight, 2 / 4 / 8 lanes: 144x84 0.032 / 0.020 / 0.016,
+// 288x168 0.038 / 0.027 / 0.030, 576x336 0.071 / 0.075 / 0.089; 8 / 16 lanes at n = 800: 72x42
+// 0.0133 / 0.0110, 100x60 0.0138 / 0.0122, 144x84 0.0152 / 0.0143, 200x120 0.0182 / 0.0202, and
+// at n = 320 144x84 0.0104 / 0.0106).
 int default_exact_lanes(bool cmem, int W, int H) {
     static int lanes = [] {
         const char* e = getenv("BIRP_EXACT_LANES");
         int v = e ? atoi(e) : 0;
-        return (v == 1 || v == 2 || v == 4 || v == 8) ? v : 0;
+        return (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) ? v : 0;
     }();
     if (lanes) return lanes;
     if (cmem) return 4;
     const long long rays = (long long)W * H;
-    return rays <= 20000 ? 8 : (rays <= 60000 ? 4 : 2);
+    return rays <= 8000 ? 16 : (rays <= 20000 ? 8 : (rays <= 60000 ? 4 : 2));
 }
 // Single positive-cube renders keep 4 samples in flight per lane: their waves are few and wait
 // on each sample's loads (the 256 orbit views rendered one by one: 8.04 ms against 8.85 with 2;

@@ -255,12 +255,13 @@ __device__ __forceinline__ float add_in_order(float acc, float s) {
         acc = acc + row_next_f<8>(s);
         acc = acc + row_next_f<12>(s);
     }
-    if constexpr (S == 8) {
-        const float s16 = __shfl_down(s, 16, 64);  // lane l + 16: the next row
-        acc = acc + s16;
-        acc = acc + row_next_f<4>(s16);
-        acc = acc + row_next_f<8>(s16);
-        acc = acc + row_next_f<12>(s16);
+#pragma unroll
+    for (int r = 1; r < S / 4; ++r) {  // 8 and 16 lanes: sub-lanes 4r..4r+3 in row r
+        const float sr = __shfl_down(s, 16 * r, 64);  // lane l + 16 r: row r's copy
+        acc = acc + sr;
+        acc = acc + row_next_f<4>(sr);
+        acc = acc + row_next_f<8>(sr);
+        acc = acc + row_next_f<12>(sr);
     }
     return acc;
 }
@@ -397,12 +398,13 @@ __device__ void exact_clip(const DevVolume& v, const ExactView& ev, const float*
 // 0.0900 ms against 0.0914 ms with 8x4 and 0.110 ms with 2x16 (4 samples in flight each), and
 // the same 4.92 - 4.94 ms as 8x4 for the 256-view orbit batch.
 template <int S> struct ExactTile {
-    static constexpr int TW = (S == 1) ? 8 : 4;
+    static constexpr int TW = (S == 1) ? 8 : (S == 16 ? 2 : 4);
     static constexpr int TH = (64 / S) / TW;
 };
 
 // The lanes of a wave and the rays they trace. The wave's TW x TH rays are cut into 2x2 pixel
-// blocks; a block takes 4S consecutive lanes (one 16-lane row for S <= 4, two for S = 8), its
+// blocks; a block takes 4S consecutive lanes (one 16-lane row for S <= 4, two for S = 8, the
+// whole wave for S = 16), its
 // four pixels on the four lanes of a quad and its S sub-lanes on S consecutive quads. A 16-byte-per-lane record load is served a
 // quad (64 B) per cycle, one cycle per distinct cache line in the quad: the four rays of a 2x2
 // block sample the same k at points less than a voxel apart, mostly inside one line, where a
@@ -573,17 +575,18 @@ __device__ __forceinline__ void exact_block(const DevVolume& v, const ExactView&
                     sv[3] = row_next_f<12>(s);
                     av[3] = row_next<12>(ai);
                 }
-                if constexpr (S == 8) {  // sub-lanes 4..7: the next 16-lane row
-                    const float s16 = __shfl_down(s, 16, 64);
-                    const int a16 = __shfl_down(ai, 16, 64);
-                    sv[4] = s16;
-                    av[4] = a16;
-                    sv[5] = row_next_f<4>(s16);
-                    av[5] = row_next<4>(a16);
-                    sv[6] = row_next_f<8>(s16);
-                    av[6] = row_next<8>(a16);
-                    sv[7] = row_next_f<12>(s16);
-                    av[7] = row_next<12>(a16);
+#pragma unroll
+                for (int rr = 1; rr < S / 4; ++rr) {  // sub-lanes 4rr..4rr+3: row rr
+                    const float sr = __shfl_down(s, 16 * rr, 64);
+                    const int ar = __shfl_down(ai, 16 * rr, 64);
+                    sv[4 * rr] = sr;
+                    av[4 * rr] = ar;
+                    sv[4 * rr + 1] = row_next_f<4>(sr);
+                    av[4 * rr + 1] = row_next<4>(ar);
+                    sv[4 * rr + 2] = row_next_f<8>(sr);
+                    av[4 * rr + 2] = row_next<8>(ar);
+                    sv[4 * rr + 3] = row_next_f<12>(sr);
+                    av[4 * rr + 3] = row_next<12>(ar);
                 }
 #pragma unroll
                 for (int mm = 0; mm < S; ++mm) {
@@ -1660,6 +1663,7 @@ static int exact_blocks(int W, int H) {
 }
 
 int exact_tiles(int W, int H, int lanes_per_ray) {
+    if (lanes_per_ray == 16) return exact_blocks<16>(W, H);
     if (lanes_per_ray == 8) return exact_blocks<8>(W, H);
     return lanes_per_ray == 4 ? exact_blocks<4>(W, H) : (lanes_per_ray == 1 ? exact_blocks<1>(W, H) : exact_blocks<2>(W, H));
 }
@@ -1710,6 +1714,7 @@ int launch_plan_multi(const DevVolume* vols, const ExactView* views, const int* 
 int launch_exact(const DevVolume& v, const ExactView& ev, int lanes_per_ray, int pipe, void* stream, const ExactOrder* ord) {
     if (ev.n < 2 || ev.n > BIRP_MAX_EXACT_SAMPLES || ev.W <= 0 || ev.H <= 0) return BIRP_ERR_INVALID_ARGUMENT;
     hipStream_t st = (hipStream_t)stream;
+    if (lanes_per_ray == 16) return launch_exact_s<16>(v, ev, pipe, st, ord);
     if (lanes_per_ray == 8) return launch_exact_s<8>(v, ev, pipe, st, ord);
     if (lanes_per_ray == 4) return launch_exact_s<4>(v, ev, pipe, st, ord);
     if (lanes_per_ray == 2) return launch_exact_s<2>(v, ev, pipe, st, ord);
@@ -1730,6 +1735,7 @@ static int launch_plan_s(const DevVolume& v, const ExactView& ev, int* order, hi
 int launch_exact_plan(const DevVolume& v, const ExactView& ev, int lanes_per_ray, int* order, void* stream) {
     if (ev.n < 2 || ev.W <= 0 || ev.H <= 0 || !order) return BIRP_ERR_INVALID_ARGUMENT;
     hipStream_t st = (hipStream_t)stream;
+    if (lanes_per_ray == 16) return launch_plan_s<16>(v, ev, order, st);
     if (lanes_per_ray == 8) return launch_plan_s<8>(v, ev, order, st);
     if (lanes_per_ray == 4) return launch_plan_s<4>(v, ev, order, st);
     if (lanes_per_ray == 2) return launch_plan_s<2>(v, ev, order, st);

@@ -136,7 +136,10 @@ ORDER_CASES = [
     ("shell-inside-ref", "shell", lambda: W.view(512, 288, 0.1, pos=(4.0, 1.0, -2.0), aim=(18.0, 6.0, 3.0)), capi.SEM_REFERENCE),
     ("signed", "signed", lambda: W.view(400, 300, 0.05, pos=(6.0, -14.0, 4.0), aim=(5.0, -1.0, 0.0)), capi.SEM_EXTENDED),
     ("nearest", "nearest", lambda: W.view(400, 300, 0.05, pos=(6.0, -14.0, 4.0), aim=(5.0, -1.0, 0.0)), capi.SEM_EXTENDED),
-    # small images: 8 lanes per ray (144x84, 378 tiles) and 4 (240x160, 600 tiles), both planned
+    # small images: 16 lanes per ray (72x42, 756 tiles), 8 (144x84, 378 tiles) and 4 (240x160, 600
+    # tiles), all planned
+    ("shell-tiny-16-lanes", "shell", lambda: W.view(72, 42, 0.1, pos=(2.0, -17.0, 8.0)), capi.SEM_EXTENDED),
+    ("signed-tiny-16-lanes", "signed", lambda: W.view(90, 50, 0.05, pos=(6.0, -14.0, 4.0), aim=(5.0, -1.0, 0.0)), capi.SEM_REFERENCE),
     ("shell-small-8-lanes", "shell", lambda: W.view(144, 84, 0.1, pos=(3.0, -18.0, 9.0)), capi.SEM_EXTENDED),
     ("signed-small-4-lanes", "signed", lambda: W.view(240, 160, 0.1, pos=(6.0, -14.0, 4.0), aim=(5.0, -1.0, 0.0)), capi.SEM_EXTENDED),
     # more tiles (18,432) than a plan holds: every policy renders the natural order
@@ -155,7 +158,7 @@ def test_orders_render_the_same_bits(cubes, oracle, name, key, mk, sem):
     for pol in (1, 2):
         assert np.array_equal(bits(res[pol][0]), bits(base)), f"policy {pol}"
         assert res[pol][1] == cnt, f"policy {pol}: {res[pol][1]} != {cnt}"
-    if name in ("shell-ragged", "shell-beyond-plans", "shell-small-8-lanes"):  # and against the oracle, bit for bit
+    if name in ("shell-ragged", "shell-beyond-plans", "shell-small-8-lanes", "shell-tiny-16-lanes"):  # and against the oracle, bit for bit
         vals, org, spc, _ = W.cube_256()
         ref, rcnt = oracle.render_f32(oracle.OracleCube(vals, org, spc), oview(oracle, v), sem, nthreads=NT)
         assert np.array_equal(bits(base), bits(ref))
