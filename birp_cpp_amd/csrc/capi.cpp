@@ -1232,21 +1232,26 @@ int exact_batch_setup(birp_batch* b) {
         G.lanes = exact_multi_lanes(m);
         // A cube group small enough that the GPU holds all its blocks at once (7 blocks of 4 waves
         // per CU, the occupancy of the 4-samples-in-flight kernel) renders like single views:
-        // 4 lanes per ray when every image has at most 60,000 rays, 4 samples in flight, and one
-        // snake deal over the CUs (exact_batch_plan). Larger groups keep 2 and 2 (the orbit batch
-        // measured 1-2% slower with 4 in flight and 10% slower with 4 lanes).
+        // 8 lanes per ray when every image has at most 20,000 rays and the 8-lane tiles fit, else 4
+        // when every image has at most 60,000 rays, 4 samples in flight, and one snake deal over
+        // the CUs (exact_batch_plan). Larger groups keep 2 and 2 (the orbit batch measured 1-2%
+        // slower with 4 in flight and 10% slower with 4 lanes).
         if (m != SM_CMEM && dev->ncu > 0) {
             const long long cap = 7LL * dev->ncu;
-            long long t2 = 0, t4 = 0;
-            bool small = true;
+            long long t2 = 0, t4 = 0, t8 = 0;
+            bool small = true, tiny = true;
             for (int k = 0; k < n; k++) {
                 if (mode[k] != m) continue;
                 t2 += exact_tiles(b->views[k].width, b->views[k].height, 2);
                 t4 += exact_tiles(b->views[k].width, b->views[k].height, 4);
-                small = small && (long long)b->views[k].width * b->views[k].height <= 60000;
+                t8 += exact_tiles(b->views[k].width, b->views[k].height, 8);
+                const long long rays = (long long)b->views[k].width * b->views[k].height;
+                small = small && rays <= 60000;
+                tiny = tiny && rays <= 20000;
             }
-            if (small && t4 <= cap) G.lanes = 4;
-            G.resident = (G.lanes == 4 ? t4 : t2) <= cap;
+            if (tiny && t8 <= cap) G.lanes = 8;
+            else if (small && t4 <= cap) G.lanes = 4;
+            G.resident = (G.lanes == 8 ? t8 : G.lanes == 4 ? t4 : t2) <= cap;
             if (G.resident && m == SM_QUAD_POS) G.pipe = 4;
         }
         prefix.push_back(0);

@@ -1696,6 +1696,7 @@ int launch_exact_multi(const DevVolume* vols, const ExactView* views, const int*
         hipLaunchKernelGGL((k_render_exact_multi<4, SM_CMEM>), dim3(total), dim3(BIRP_BLOCK), lds, st, vols, views, prefix, nview, total, order);
         return hip_status(hipGetLastError());
     }
+    if (lanes == 8) return launch_exact_multi_s<8>(vols, views, prefix, nview, total, mode, lds, st, order, pipe);
     if (lanes == 4) return launch_exact_multi_s<4>(vols, views, prefix, nview, total, mode, lds, st, order, pipe);
     if (lanes == 2) return launch_exact_multi_s<2>(vols, views, prefix, nview, total, mode, lds, st, order, pipe);
     return BIRP_ERR_INVALID_ARGUMENT;
@@ -1703,10 +1704,11 @@ int launch_exact_multi(const DevVolume* vols, const ExactView* views, const int*
 
 int launch_plan_multi(const DevVolume* vols, const ExactView* views, const int* prefix, int nview, int total, int mode, int* cost,
                       void* stream, int lanes) {
-    if (nview <= 0 || total <= 0 || mode == SM_CMEM || (lanes != 2 && lanes != 4)) return BIRP_ERR_INVALID_ARGUMENT;
+    if (nview <= 0 || total <= 0 || mode == SM_CMEM || (lanes != 2 && lanes != 4 && lanes != 8)) return BIRP_ERR_INVALID_ARGUMENT;
     const long long threads = 4LL * total;
     const dim3 grid((unsigned)((threads + BIRP_BLOCK - 1) / BIRP_BLOCK));
-    if (lanes == 4) hipLaunchKernelGGL((k_plan_multi<4>), grid, dim3(BIRP_BLOCK), 0, (hipStream_t)stream, vols, views, prefix, nview, total, cost);
+    if (lanes == 8) hipLaunchKernelGGL((k_plan_multi<8>), grid, dim3(BIRP_BLOCK), 0, (hipStream_t)stream, vols, views, prefix, nview, total, cost);
+    else if (lanes == 4) hipLaunchKernelGGL((k_plan_multi<4>), grid, dim3(BIRP_BLOCK), 0, (hipStream_t)stream, vols, views, prefix, nview, total, cost);
     else hipLaunchKernelGGL((k_plan_multi<2>), grid, dim3(BIRP_BLOCK), 0, (hipStream_t)stream, vols, views, prefix, nview, total, cost);
     return hip_status(hipGetLastError());
 }

@@ -333,9 +333,9 @@ int launch_slot(birp_server* S, Slot& s) {
     // sums h_eprefix[estart[m] + m, + ecount[m] + 1)
     int ecount[kModes] = {}, estart[kModes] = {}, efill[kModes] = {};
     size_t elds[kModes] = {};
-    long long et2[kModes] = {}, et4[kModes] = {};
-    bool esmall[kModes];
-    for (int m = 0; m < kModes; m++) esmall[m] = true;
+    long long et2[kModes] = {}, et4[kModes] = {}, et8[kModes] = {};
+    bool esmall[kModes], etiny[kModes];
+    for (int m = 0; m < kModes; m++) esmall[m] = etiny[m] = true;
     for (Request* r : s.reqs) {
         if (!exact_request(r)) continue;
         ExactView ev;
@@ -344,19 +344,23 @@ int launch_slot(birp_server* S, Slot& s) {
         ecount[m]++;
         et2[m] += exact_tiles(r->req.view.width, r->req.view.height, 2);
         et4[m] += exact_tiles(r->req.view.width, r->req.view.height, 4);
-        esmall[m] = esmall[m] && (long long)r->req.view.width * r->req.view.height <= 60000;
+        et8[m] += exact_tiles(r->req.view.width, r->req.view.height, 8);
+        const long long rays = (long long)r->req.view.width * r->req.view.height;
+        esmall[m] = esmall[m] && rays <= 60000;
+        etiny[m] = etiny[m] && rays <= 20000;
     }
     // each mode's launch shape, as exact batches choose it: a cube launch whose blocks the GPU holds
-    // all at once takes 4 samples in flight, and 4 lanes per ray when every image has at most
-    // 60,000 rays (a lone request of a small image renders as fast as its single render)
+    // all at once takes 4 samples in flight, and 8 lanes per ray when every image has at most
+    // 20,000 rays and those tiles fit, else 4 when every image has at most 60,000 rays
     int elanes[kModes], epipe[kModes];
     for (int m = 0; m < kModes; m++) {
         elanes[m] = exact_multi_lanes(m);
         epipe[m] = 2;
         if (m == SM_CMEM || S->dev->ncu <= 0) continue;
         const long long cap = 7LL * S->dev->ncu;
-        if (esmall[m] && et4[m] <= cap) elanes[m] = 4;
-        if (m == SM_QUAD_POS && (elanes[m] == 4 ? et4[m] : et2[m]) <= cap) epipe[m] = 4;
+        if (etiny[m] && et8[m] <= cap) elanes[m] = 8;
+        else if (esmall[m] && et4[m] <= cap) elanes[m] = 4;
+        if (m == SM_QUAD_POS && (elanes[m] == 8 ? et8[m] : elanes[m] == 4 ? et4[m] : et2[m]) <= cap) epipe[m] = 4;
     }
     int nexact = 0;
     for (int m = 0; m < kModes; m++) nexact += ecount[m];

@@ -139,11 +139,12 @@ def test_exact_batch_of_the_256_orbit_views(device, oracle):
 
 
 def test_resident_exact_batches_bitwise(spaces, oracle):
-    """Batches whose blocks the GPU holds all at once take the single renders' launch shape: 4 lanes
-    per ray when every image has at most 60,000 rays, 4 samples in flight, and one plan dealt in a
-    snake over the CUs. Six 144x84 views (4 lanes, more tiles than CUs, so planned), one 576x336
-    view (2 lanes, planned) and a mix of positive and signed cubes: every image bitwise equal to
-    its single render, counts too, and two images to the oracle's FP32 render."""
+    """Batches whose blocks the GPU holds all at once take the single renders' launch shape: 8 lanes
+    per ray when every image has at most 20,000 rays and those tiles fit, else 4 when every image
+    has at most 60,000 rays, 4 samples in flight, and one plan dealt in a snake over the CUs. Six
+    144x84 views (4 lanes: their 8-lane tiles do not fit), one 576x336 view (2 lanes, planned) and
+    four 144x84 views over positive and signed cubes (8 lanes): every image bitwise equal to its
+    single render, counts too, and two images to the oracle's FP32 render."""
     vols, ocs = spaces
     pos = [tuple(float(c) for c in p) for p in capi.synth_orbit(6, 23)]
     small = [W.view(144, 84, 0.1, pos=p) for p in pos]
